@@ -1,0 +1,32 @@
+"""Compile ``libbrainspike_hip.so`` for gfx950 with hipcc (in-tree)."""
+from __future__ import annotations
+
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = [os.path.join(HERE, 'csrc', 'bs_net.hip')]
+OUT = os.path.join(HERE, 'libbrainspike_hip.so')
+
+# -ffp-contract=off: every arithmetic op is one IEEE-rounded op (no FMA
+# contraction), which the FP64 bit-exact parity with the CPU oracle relies on.
+# -packed-fp32-ops off: no packed FP32 instructions are generated.
+FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared', '-ffp-contract=off',
+         '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
+
+
+def build(verbose: bool = False) -> str:
+    hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
+    tmp = OUT + '.tmp'
+    cmd = [hipcc] + FLAGS + ['-o', tmp] + SRC
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError('hipcc failed:\n' + res.stdout + res.stderr)
+    os.replace(tmp, OUT)
+    if verbose:
+        print('built', OUT)
+    return OUT
+
+
+if __name__ == '__main__':
+    build(verbose=True)

@@ -1,0 +1,153 @@
+// bs_math.h -- device arithmetic whose results are fixed bit-for-bit:
+// a counter-based RNG (Threefry-2x32, 20 rounds; Salmon et al., SC'11), a
+// Knuth product Poisson sampler built only from IEEE multiplies and compares,
+// and exp/log/pow from range reduction plus Taylor / atanh series. Every
+// operation is an exactly rounded IEEE op (the file is compiled with
+// -ffp-contract=off), so a CPU restatement performing the same sequence of
+// operations reproduces every bit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bs {
+
+// ---------------------------------------------------------------- Threefry
+__device__ __forceinline__ uint32_t rotl32(uint32_t x, int r) {
+  return (x << r) | (x >> (32 - r));
+}
+
+// Threefry-2x32 with 20 rounds: key (k0,k1), counter (c0,c1) -> (x0,x1).
+// Rotation schedule and key injection per the Random123 specification.
+__device__ __forceinline__ void threefry2x32(uint32_t k0, uint32_t k1, uint32_t c0,
+                                             uint32_t c1, uint32_t& o0, uint32_t& o1) {
+  const uint32_t ks2 = 0x1BD11BDAu ^ k0 ^ k1;
+  const uint32_t ks[3] = {k0, k1, ks2};
+  const int R[8] = {13, 15, 26, 6, 17, 29, 16, 24};
+  uint32_t x0 = c0 + k0, x1 = c1 + k1;
+#pragma unroll
+  for (int blk = 0; blk < 5; ++blk) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      x0 += x1;
+      x1 = rotl32(x1, R[(blk * 4 + r) & 7]);
+      x1 ^= x0;
+    }
+    x0 += ks[(blk + 1) % 3];
+    x1 += ks[(blk + 2) % 3] + (uint32_t)(blk + 1);
+  }
+  o0 = x0;
+  o1 = x1;
+}
+
+// Uniform in (0,1): (x + 1/2) * 2^-32, exact in double.
+__device__ __forceinline__ double u01(uint32_t x) {
+  return ((double)x + 0.5) * 2.3283064365386963e-10;
+}
+
+// Poisson(lam) count for generator element `elem` at step `step`.
+// lam is split into `pieces` equal parts (pieces = ceil(lam/16)); each part is
+// Knuth's product method against L = exp(-lam/pieces) (precomputed on the
+// host). Uniforms are consumed in order from the stream
+//   threefry(key=(seed, step), ctr=(elem, j)), j = 0,1,2,... two per block.
+// A part stops after 1024 uniforms at most so every call terminates.
+__device__ __forceinline__ uint32_t poisson_knuth(uint32_t seed, uint32_t step, uint32_t elem,
+                                                  double L, int pieces) {
+  uint32_t count = 0;
+  uint32_t j = 0;
+  uint32_t buf1 = 0;
+  bool have = false;
+  for (int p = 0; p < pieces; ++p) {
+    double prod = 1.0;
+    for (int it = 0; it < 1024; ++it) {
+      uint32_t x;
+      if (have) {
+        x = buf1;
+        have = false;
+      } else {
+        uint32_t a, b;
+        threefry2x32(seed, step, elem, j, a, b);
+        ++j;
+        x = a;
+        buf1 = b;
+        have = true;
+      }
+      prod = prod * u01(x);
+      if (prod <= L) break;
+      ++count;
+    }
+  }
+  return count;
+}
+
+// ---------------------------------------------------------------- exp/log
+// ln2 split: LN2_HI has 42 significant bits, so k*LN2_HI is exact for |k|<2048.
+#define BS_LN2_HI 0x1.62e42fefa3800p-1
+#define BS_LN2_LO 0x1.ef35793c76730p-45
+#define BS_INV_LN2 0x1.71547652b82fep+0
+
+// exp(x): k = rint(x/ln2), r = (x - k*LN2_HI) - k*LN2_LO, |r| <= 0.347;
+// exp(r) by the Taylor series to r^13 (Horner), then scaled by 2^k.
+__device__ __forceinline__ double exp_p(double x) {
+  if (x != x) return x;
+  if (x > 709.782712893384) return __builtin_inf();
+  if (x < -745.1332191019412) return 0.0;
+  const double kd = __builtin_rint(x * BS_INV_LN2);
+  const int k = (int)kd;
+  const double r = (x - kd * BS_LN2_HI) - kd * BS_LN2_LO;
+  double p = 1.0 / 6227020800.0;          // 1/13!
+  p = p * r + 1.0 / 479001600.0;          // 1/12!
+  p = p * r + 1.0 / 39916800.0;           // 1/11!
+  p = p * r + 1.0 / 3628800.0;            // 1/10!
+  p = p * r + 1.0 / 362880.0;             // 1/9!
+  p = p * r + 1.0 / 40320.0;              // 1/8!
+  p = p * r + 1.0 / 5040.0;               // 1/7!
+  p = p * r + 1.0 / 720.0;                // 1/6!
+  p = p * r + 1.0 / 120.0;                // 1/5!
+  p = p * r + 1.0 / 24.0;                 // 1/4!
+  p = p * r + 1.0 / 6.0;                  // 1/3!
+  p = p * r + 0.5;                        // 1/2!
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  return __builtin_ldexp(p, k);
+}
+
+// log(x) for x > 0: x = m 2^e, m in [sqrt(1/2), sqrt(2)); s = (m-1)/(m+1);
+// log(m) = 2 atanh(s) = 2 s (1 + z/3 + z^2/5 + ... + z^11/23), z = s^2.
+__device__ __forceinline__ double log_p(double x) {
+  if (x != x || x < 0.0) return __builtin_nan("");
+  if (x == 0.0) return -__builtin_inf();
+  if (x == __builtin_inf()) return x;
+  int e;
+  double m = __builtin_frexp(x, &e);  // m in [0.5, 1)
+  if (m < 0.7071067811865476) {
+    m = m * 2.0;
+    e = e - 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double z = s * s;
+  double q = 1.0 / 23.0;
+  q = q * z + 1.0 / 21.0;
+  q = q * z + 1.0 / 19.0;
+  q = q * z + 1.0 / 17.0;
+  q = q * z + 1.0 / 15.0;
+  q = q * z + 1.0 / 13.0;
+  q = q * z + 1.0 / 11.0;
+  q = q * z + 1.0 / 9.0;
+  q = q * z + 1.0 / 7.0;
+  q = q * z + 1.0 / 5.0;
+  q = q * z + 1.0 / 3.0;
+  q = q * z + 1.0;
+  const double lm = 2.0 * s * q;
+  const double ed = (double)e;
+  return ed * BS_LN2_HI + (ed * BS_LN2_LO + lm);
+}
+
+// x^y for x >= 0 (the only domain used: step-size control, STDP exponents).
+__device__ __forceinline__ double pow_p(double x, double y) {
+  if (y == 0.0) return 1.0;
+  if (y == 1.0) return x;
+  if (x == 0.0) return y > 0.0 ? 0.0 : __builtin_inf();
+  return exp_p(y * log_p(x));
+}
+
+}  // namespace bs

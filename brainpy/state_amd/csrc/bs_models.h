@@ -1,0 +1,267 @@
+// bs_models.h -- one-step neuron updates executed per lane, restating the
+// reference's update() bodies operation by operation (same association order,
+// no contraction) so FP64 results are bit-identical to a sequential CPU
+// restatement. Parameter-block layouts (double[40]) are filled by
+// brainpy/state_amd/models.py with host-precomputed FP64 propagators.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "bs_math.h"
+
+namespace bs {
+
+// ---- parameter indices --------------------------------------------------
+// iaf_psc_delta  (_nest_neuron/iaf_psc_delta.py:449-521)
+enum { D_EL = 0, D_DECAY, D_CI, D_VTH, D_VRESET, D_VMIN, D_REFN, D_REFIN, D_RDEC0 };
+// state: 0 V, 1 refractory_spike_buffer
+// iaf_psc_exp    (_nest_neuron/iaf_psc_exp.py:552-682)
+enum { X_EL = 0, X_THETA, X_VRESET_REL, X_P22, X_P21EX, X_P21IN, X_P20, X_P11EX, X_P11IN,
+       X_IE, X_REFN, X_1MP11EX };
+// state: 0 V, 1 i_syn_ex, 2 i_syn_in, 3 i_0, 4 i_1
+// iaf_psc_alpha  (_nest_neuron/iaf_psc_alpha.py:558-684)
+enum { A_EL = 0, A_THETA, A_VRESET_REL, A_P11EX, A_P21EX, A_P22EX, A_P11IN, A_P21IN, A_P22IN,
+       A_EXPM1, A_P30, A_P31EX, A_P32EX, A_P31IN, A_P32IN, A_EPSC, A_IPSC, A_IE, A_REFN,
+       A_VMIN_REL };
+// state: 0 V, 1 I_syn_ex, 2 I_syn_in, 3 dI_syn_ex, 4 dI_syn_in, 5 y0
+// aeif_cond_alpha (_nest_neuron/aeif_cond_alpha.py:554-747, _nest_base/utils.py:583-723)
+enum { C_VPEAK = 0, C_VRESET, C_GL, C_CM, C_EEX, C_EIN, C_EL, C_DT_, C_TAUW, C_A, C_B, C_VTH,
+       C_TAUEX, C_TAUIN, C_IE, C_ATOL, C_REFN, C_PSCEX, C_PSCIN, C_DT, C_MINH, C_VPEAKDET,
+       C_DTSAFE, C_MAXIT, C_GLDT, C_NEGGL };
+// state: 0 V, 1 dg_ex, 2 g_ex, 3 dg_in, 4 g_in, 5 w, 6 h, 7 I_stim
+
+// ---- iaf_psc_delta ------------------------------------------------------
+// v = E_L + (V-E_L)*decay + (I/C)*tau*(1-decay)  [the last term is the
+// constant D_CI]; v += sum(delta); refractory holds V; spike iff v >= V_th.
+template <typename T>
+__device__ __forceinline__ bool step_delta(const double* __restrict__ P, T* s, int& r, T din) {
+  const T EL = (T)P[D_EL];
+  const T last_v = s[0];
+  T v = EL + (last_v - EL) * (T)P[D_DECAY] + (T)P[D_CI];
+  v = v + din;
+  const bool refin = P[D_REFIN] != 0.0;
+  if (refin) v = v + s[1];
+  if (P[D_VMIN] > -1e300) v = v > (T)P[D_VMIN] ? v : (T)P[D_VMIN];
+  const bool not_ref = (r == 0);
+  const T v_post = not_ref ? v : last_v;
+  if (refin) {
+    // exp(-r dt / tau_m) for the current r, tabulated on the host.
+    const T rdec = (T)P[D_RDEC0 + (r < 23 ? r : 23)];
+    s[1] = not_ref ? (T)0 : (T)(s[1] + din * rdec);
+  }
+  int rr = not_ref ? r : r - 1;
+  const bool spike = v_post >= (T)P[D_VTH];
+  r = spike ? (int)P[D_REFN] : rr;
+  s[0] = spike ? (T)P[D_VRESET] : v_post;
+  return spike;
+}
+
+// ---- iaf_psc_exp (delta = 0, deterministic threshold) ---------------------
+template <typename T>
+__device__ __forceinline__ bool step_exp(const double* __restrict__ P, T* s, int& r, T w_all) {
+  const T EL = (T)P[X_EL];
+  T V_rel = s[0] - EL;
+  T iex = s[1], iin = s[2];
+  const T i0 = s[3], i1 = s[4];
+  const bool not_ref = (r == 0);
+  const T Vc = V_rel * (T)P[X_P22] + iex * (T)P[X_P21EX] + iin * (T)P[X_P21IN] +
+               ((T)P[X_IE] + i0) * (T)P[X_P20];
+  V_rel = not_ref ? Vc : V_rel;
+  int rr = not_ref ? r : r - 1;
+  iex = iex * (T)P[X_P11EX];
+  iin = iin * (T)P[X_P11IN];
+  iex = iex + (T)P[X_1MP11EX] * i1;
+  const T wex = w_all > (T)0 ? w_all : (T)0;
+  const T win = w_all < (T)0 ? w_all : (T)0;
+  iex = iex + wex;
+  iin = iin + win;
+  const bool spike = V_rel >= (T)P[X_THETA];
+  r = spike ? (int)P[X_REFN] : rr;
+  V_rel = spike ? (T)P[X_VRESET_REL] : V_rel;
+  s[0] = V_rel + EL;
+  s[1] = iex;
+  s[2] = iin;
+  s[3] = (T)0;  // next-step latched current: x + sum_current_inputs = 0 on this path
+  s[4] = (T)0;
+  return spike;
+}
+
+// ---- iaf_psc_alpha -------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ bool step_alpha(const double* __restrict__ P, T* s, int& r, T w_all) {
+  const T wex = w_all > (T)0 ? w_all : (T)0;
+  const T win = w_all < (T)0 ? w_all : (T)0;
+  const T EL = (T)P[A_EL];
+  T y3 = s[0] - EL;
+  T Iex = s[1], Iin = s[2], dIex = s[3], dIin = s[4];
+  const T y0 = s[5];
+  const bool not_ref = (r == 0);
+  T y3c = (T)P[A_P30] * (y0 + (T)P[A_IE]) + (T)P[A_P31EX] * dIex + (T)P[A_P32EX] * Iex +
+          (T)P[A_P31IN] * dIin + (T)P[A_P32IN] * Iin + (T)P[A_EXPM1] * y3 + y3;
+  if (P[A_VMIN_REL] > -1e300) y3c = y3c > (T)P[A_VMIN_REL] ? y3c : (T)P[A_VMIN_REL];
+  y3 = not_ref ? y3c : y3;
+  int rr = not_ref ? r : r - 1;
+  Iex = (T)P[A_P21EX] * dIex + (T)P[A_P22EX] * Iex;
+  dIex = dIex * (T)P[A_P11EX];
+  dIex = dIex + (T)P[A_EPSC] * wex;
+  Iin = (T)P[A_P21IN] * dIin + (T)P[A_P22IN] * Iin;
+  dIin = dIin * (T)P[A_P11IN];
+  dIin = dIin + (T)P[A_IPSC] * win;
+  const bool spike = y3 >= (T)P[A_THETA];
+  r = spike ? (int)P[A_REFN] : rr;
+  y3 = spike ? (T)P[A_VRESET_REL] : y3;
+  s[0] = y3 + EL;
+  s[1] = Iex;
+  s[2] = Iin;
+  s[3] = dIex;
+  s[4] = dIin;
+  s[5] = (T)0;
+  return spike;
+}
+
+// ---- aeif_cond_alpha: adaptive RKF45 per lane ----------------------------
+// Butcher tableau (Fehlberg): the coefficient values are the same double
+// expressions the reference evaluates (utils.py:266-278); zero entries are
+// skipped as _rk_weighted_sum does (utils.py:388-408).
+struct AeifY {
+  double V, dgex, gex, dgin, gin, w;
+};
+
+__device__ __forceinline__ void aeif_rhs(const double* __restrict__ P, const AeifY& y, bool refr,
+                                         double istim, AeifY& k) {
+  // jnp.minimum: NaN-propagating
+  const double veff = refr ? P[C_VRESET] : ((y.V != y.V || y.V < P[C_VPEAK]) ? y.V : P[C_VPEAK]);
+  const double isex = y.gex * (veff - P[C_EEX]);
+  const double isin = y.gin * (veff - P[C_EIN]);
+  double ea = (veff - P[C_VTH]) / P[C_DTSAFE];
+  ea = ea < -500.0 ? -500.0 : (ea > 500.0 ? 500.0 : ea);
+  const double ispk = P[C_GLDT] * exp_p(ea);
+  const double dvraw = (P[C_NEGGL] * (veff - P[C_EL]) + ispk - isex - isin - y.w + P[C_IE] + istim) /
+                       P[C_CM];
+  k.V = refr ? 0.0 : dvraw;
+  k.dgex = -y.dgex / P[C_TAUEX];
+  k.gex = y.dgex - y.gex / P[C_TAUEX];
+  k.dgin = -y.dgin / P[C_TAUIN];
+  k.gin = y.dgin - y.gin / P[C_TAUIN];
+  k.w = (P[C_A] * (veff - P[C_EL]) - y.w) / P[C_TAUW];
+}
+
+#define BS_AX(f, expr) t.f = y.f + h * (expr);
+__device__ __forceinline__ double nanmax(double a, double b) {
+  if (a != a || b != b) return __builtin_nan("");
+  return a > b ? a : b;
+}
+
+// One simulation step of aeif_cond_alpha for one neuron. Returns spike flag;
+// sets *unstable when the reference's guard would fire.
+__device__ __forceinline__ bool step_aeif(const double* __restrict__ P, double* s, int& r,
+                                          double wex_in, double win_in, int* unstable) {
+  const double A10 = 1.0 / 4.0;
+  const double A20 = 3.0 / 32.0, A21 = 9.0 / 32.0;
+  const double A30 = 1932.0 / 2197.0, A31 = -7200.0 / 2197.0, A32 = 7296.0 / 2197.0;
+  const double A40 = 439.0 / 216.0, A41 = -8.0, A42 = 3680.0 / 513.0, A43 = -845.0 / 4104.0;
+  const double A50 = -8.0 / 27.0, A51 = 2.0, A52 = -3544.0 / 2565.0, A53 = 1859.0 / 4104.0,
+               A54 = -11.0 / 40.0;
+  const double B0 = 16.0 / 135.0, B2 = 6656.0 / 12825.0, B3 = 28561.0 / 56430.0, B4 = -9.0 / 50.0,
+               B5 = 2.0 / 55.0;
+  const double E0 = 16.0 / 135.0 - 25.0 / 216.0, E2 = 6656.0 / 12825.0 - 1408.0 / 2565.0,
+               E3 = 28561.0 / 56430.0 - 2197.0 / 4104.0, E4 = -9.0 / 50.0 - -1.0 / 5.0,
+               E5 = 2.0 / 55.0 - 0.0;
+
+  AeifY y{s[0], s[1], s[2], s[3], s[4], s[5]};
+  double h = s[6];
+  const double istim = s[7];
+  const double dt = P[C_DT], minh = P[C_MINH], atol = P[C_ATOL];
+  const int refn = (int)P[C_REFN];
+  bool spiked = false;
+  h = h > minh ? h : minh;
+  double tl = 0.0;
+  const int maxit = (int)P[C_MAXIT];
+  for (int it = 0; it < maxit && tl < dt; ++it) {
+    {
+      const double rem = dt - tl;
+      const double hm = h < rem ? h : rem;
+      h = minh > hm ? minh : hm;
+    }
+    const bool refr = r > 0;
+    AeifY k0, k1, k2, k3, k4, k5, t;
+    aeif_rhs(P, y, refr, istim, k0);
+#define BS_STAGE1(f) t.f = y.f + h * (A10 * k0.f);
+    BS_STAGE1(V) BS_STAGE1(dgex) BS_STAGE1(gex) BS_STAGE1(dgin) BS_STAGE1(gin) BS_STAGE1(w)
+    aeif_rhs(P, t, refr, istim, k1);
+#define BS_STAGE2(f) t.f = y.f + h * (A20 * k0.f + A21 * k1.f);
+    BS_STAGE2(V) BS_STAGE2(dgex) BS_STAGE2(gex) BS_STAGE2(dgin) BS_STAGE2(gin) BS_STAGE2(w)
+    aeif_rhs(P, t, refr, istim, k2);
+#define BS_STAGE3(f) t.f = y.f + h * (A30 * k0.f + A31 * k1.f + A32 * k2.f);
+    BS_STAGE3(V) BS_STAGE3(dgex) BS_STAGE3(gex) BS_STAGE3(dgin) BS_STAGE3(gin) BS_STAGE3(w)
+    aeif_rhs(P, t, refr, istim, k3);
+#define BS_STAGE4(f) t.f = y.f + h * (A40 * k0.f + A41 * k1.f + A42 * k2.f + A43 * k3.f);
+    BS_STAGE4(V) BS_STAGE4(dgex) BS_STAGE4(gex) BS_STAGE4(dgin) BS_STAGE4(gin) BS_STAGE4(w)
+    aeif_rhs(P, t, refr, istim, k4);
+#define BS_STAGE5(f) t.f = y.f + h * (A50 * k0.f + A51 * k1.f + A52 * k2.f + A53 * k3.f + A54 * k4.f);
+    BS_STAGE5(V) BS_STAGE5(dgex) BS_STAGE5(gex) BS_STAGE5(dgin) BS_STAGE5(gin) BS_STAGE5(w)
+    aeif_rhs(P, t, refr, istim, k5);
+    AeifY yh, ye;
+#define BS_HIGH(f) yh.f = y.f + h * (B0 * k0.f + B2 * k2.f + B3 * k3.f + B4 * k4.f + B5 * k5.f);
+    BS_HIGH(V) BS_HIGH(dgex) BS_HIGH(gex) BS_HIGH(dgin) BS_HIGH(gin) BS_HIGH(w)
+#define BS_ERR(f) ye.f = h * (E0 * k0.f + E2 * k2.f + E3 * k3.f + E4 * k4.f + E5 * k5.f);
+    BS_ERR(V) BS_ERR(dgex) BS_ERR(gex) BS_ERR(dgin) BS_ERR(gin) BS_ERR(w)
+    // scaled error |e| / (atol + 0 * max(|y0|, |y1|)), max over the six leaves
+    double err;
+    {
+#define BS_SE(f) (__builtin_fabs(ye.f) / (atol + 0.0 * nanmax(__builtin_fabs(y.f), __builtin_fabs(yh.f))))
+      err = BS_SE(V);
+      err = nanmax(err, BS_SE(dgex));
+      err = nanmax(err, BS_SE(gex));
+      err = nanmax(err, BS_SE(dgin));
+      err = nanmax(err, BS_SE(gin));
+      err = nanmax(err, BS_SE(w));
+    }
+    if (err != err) err = __builtin_inf();
+    const bool accept = (err <= 1.0) || (h <= minh);
+    if (accept) {
+      y = yh;
+      tl = tl + h;
+      // event_fn: instability guard, refractory clamp, spike/reset/adaptation
+      if (y.V < -1e3 || y.w < -1e6 || y.w > 1e6) *unstable = 1;
+      if (r > 0) y.V = P[C_VRESET];
+      if (r <= 0 && y.V >= P[C_VPEAKDET]) {
+        spiked = true;
+        y.V = P[C_VRESET];
+        y.w = y.w + P[C_B];
+        if (refn > 0) r = refn + 1;
+      }
+    }
+    const double errsg = err > 1e-30 ? err : 1e-30;
+    const double inv = 1.0 / errsg;
+    if (accept) {
+      double fa;
+      if (errsg <= 1e-30) {
+        fa = 5.0;
+      } else {
+        fa = 0.9 * pow_p(inv, 0.2);
+        fa = fa < 0.2 ? 0.2 : (fa > 5.0 ? 5.0 : fa);
+      }
+      const double hn = h * fa;
+      h = minh > hn ? minh : hn;
+    } else {
+      double fr = 0.9 * pow_p(inv, 0.25);
+      fr = fr < 0.2 ? 0.2 : (fr > 1.0 ? 1.0 : fr);
+      const double hn = h * fr;
+      h = minh > hn ? minh : hn;
+    }
+  }
+  if (r > 0) r = r - 1;
+  y.dgex = y.dgex + P[C_PSCEX] * wex_in;
+  y.dgin = y.dgin + P[C_PSCIN] * win_in;
+  s[0] = y.V;
+  s[1] = y.dgex;
+  s[2] = y.gex;
+  s[3] = y.dgin;
+  s[4] = y.gin;
+  s[5] = y.w;
+  s[6] = h;
+  s[7] = 0.0;  // I_stim for the next step: x + sum_current_inputs = 0 on this path
+  return spiked;
+}
+
+}  // namespace bs

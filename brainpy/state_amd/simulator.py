@@ -1,0 +1,733 @@
+"""Simulator -- the NEST-style create / connect / simulate front end.
+
+Drop-in for the reference's ``Simulator`` (``_nest_network/simulator.py``):
+same method names, argument meanings, one-step holder lag, projection-order
+input summation and recorder taps. Building is host-side (connectivity is
+sampled here, once); ``simulate`` / ``cont`` hand a flat network descriptor to
+``libbrainspike_hip.so`` (``bs_net_create``) and advance it on the GPU
+(``bs_net_run``). There is no CPU fallback: without the HIP library or a GPU,
+running raises :class:`~brainpy.state_amd._lib.EngineUnavailable`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import itertools
+import math
+
+import numpy as np
+
+from . import units as u
+from . import _lib as L
+from .devices import multimeter, poisson_generator, spike_recorder
+from .dist import Distribution
+from .models import Neuron, aeif_cond_alpha, iaf_psc_delta
+from .nodeview import NodeView, _Segment, _flat_size
+from .plasticity import stdp_synapse
+from .rules import ConnRule, _OneToOne, all_to_all
+
+__all__ = ['Simulator', 'SimulationResult', 'StaticProjection', 'PlasticProjection']
+
+
+# --------------------------------------------------------------------- helpers
+class _GeneratorSpec:
+    def __init__(self, model_cls, params):
+        self.model_cls = model_cls
+        self.params = params
+
+
+class _GenSegment(_Segment):
+    def __init__(self, spec):
+        object.__setattr__(self, 'spec', spec)
+        object.__setattr__(self, 'population', None)
+        object.__setattr__(self, 'indices', np.arange(0))
+
+
+def _is_generator(model_cls) -> bool:
+    name = getattr(model_cls, '__name__', '')
+    return getattr(model_cls, '_is_generator', False) or 'generator' in name
+
+
+def _weight_dim(pop):
+    if isinstance(pop, iaf_psc_delta):
+        return 'voltage'
+    if isinstance(pop, aeif_cond_alpha):
+        return 'conductance'
+    return 'current'
+
+
+def delay_to_steps(delay, dt):
+    """Delay (ms, scalar or array) -> integer steps; delay/dt must be integral.
+
+    The reference's InputDelay reads ``floor`` and ``ceil`` frames and lerps
+    between them (delay.py:114-128); for integral ``delay/dt`` that is an exact
+    single-frame read, the only case the engine implements.
+    """
+    if delay is None:
+        return 0
+    d = np.asarray(u.magnitude(delay, 'time'), dtype=np.float64)
+    steps = d / dt
+    r = np.rint(steps)
+    if np.any(np.abs(steps - r) > 1e-9 * np.maximum(1.0, np.abs(r))):
+        raise ValueError('delays must be integer multiples of dt (fractional delays are '
+                         'not supported by the engine)')
+    if np.any(r < 0):
+        raise ValueError('delays must be >= 0')
+    return int(r) if r.ndim == 0 else r.astype(np.int32)
+
+
+def seq_sum_table(w, kmax, dtype):
+    """S[k] = ((0 + w) + w) + ... (k terms), accumulated in ``dtype``.
+
+    The CSR event matmul over equal per-edge weights produces exactly this
+    value for a post neuron with k arriving events, whatever their order.
+    """
+    wt = dtype(w)
+    out = np.zeros(kmax + 1, dtype=dtype)
+    acc = dtype(0.0)
+    for k in range(1, kmax + 1):
+        acc = dtype(acc + wt)
+        out[k] = acc
+    return out.astype(np.float64)
+
+
+# ---------------------------------------------------------------- projections
+class StaticProjection:
+    """Handle of a realised static projection (reference: EventProjection)."""
+    __module__ = 'brainpy.state_amd'
+    plastic = False
+
+    def __init__(self, ordinal, pre_pop, post_pop, pre, post, weight, delay, receptor):
+        self.ordinal = ordinal
+        self.pre_pop, self.post_pop = pre_pop, post_pop
+        self.pre_idx, self.post_idx = pre, post      # population-local, sampled order
+        self.weight = weight                         # float or (E,) array
+        self.delay = delay                           # int steps or (E,) int array
+        self.receptor = receptor
+
+    @property
+    def n_edges(self):
+        return int(self.pre_idx.shape[0])
+
+    @property
+    def homogeneous(self):
+        return np.ndim(self.weight) == 0
+
+
+class PlasticProjection(StaticProjection):
+    """Handle of a realised plastic projection (reference: EventPlasticProj)."""
+    __module__ = 'brainpy.state_amd'
+    plastic = True
+
+    def __init__(self, *args, rule=None, **kw):
+        super().__init__(*args, **kw)
+        self.rule = rule
+
+
+class _GenChannel:
+    def __init__(self, ordinal, gen, post_pop, post_idx, weight, delay, receptor, seed):
+        self.ordinal = ordinal
+        self.gen = gen
+        self.post_pop = post_pop
+        self.post_idx = post_idx
+        self.weight = weight
+        self.delay = delay
+        self.receptor = receptor
+        self.seed = seed
+
+
+# --------------------------------------------------------------------- result
+class SimulationResult:
+    """Recorded spikes and analog traces of one ``simulate`` / ``cont`` window."""
+    __module__ = 'brainpy.state_amd'
+
+    def __init__(self, recordings, duration_ms, dt, traces=None, times=None, weights=None):
+        self._rec = recordings
+        self._T = duration_ms
+        self._dt = dt
+        self._traces = dict(traces or {})
+        self._times = times
+        self._weights = dict(weights or {})
+
+    @staticmethod
+    def _key(node):
+        if isinstance(node, NodeView):
+            return id(node.segments[0].population)
+        return id(node)
+
+    def spikes(self, node):
+        """Per-step spike matrix ``(n_steps, n_recorded)`` of a recorder."""
+        return self._rec[self._key(node)]
+
+    def n_events(self, node) -> int:
+        return int(np.sum(self._rec[self._key(node)] > 0))
+
+    def rate(self, node) -> float:
+        """Mean rate in spikes/s over the recorded neurons (simulator.py:343-348)."""
+        spk = self._rec[self._key(node)]
+        return float(np.sum(spk > 0)) / spk.shape[1] / (self._T / 1000.0)
+
+    def trace(self, recorder, recordable='V_m'):
+        rid = self._key(recorder)
+        key = f'{rid}|{recordable}'
+        if key not in self._traces:
+            available = sorted(k.split('|', 1)[1] for k in self._traces if k.startswith(f'{rid}|'))
+            raise KeyError(f'recordable {recordable!r} was not recorded by this recorder; '
+                           f'recorded: {available}')
+        return self._traces[key]
+
+    def weight_trace(self, proj):
+        rid = id(proj)
+        if rid not in self._weights:
+            raise KeyError("this projection's weight was not recorded; call "
+                           'sim.record_weight(proj) before simulate()')
+        return self._weights[rid]
+
+    @property
+    def times(self):
+        return self._times
+
+
+# ------------------------------------------------------------------ simulator
+class Simulator:
+    """Explicit NEST-flavoured network builder and runner on the MI355X engine.
+
+    Parameters
+    ----------
+    dt : Quantity or float
+        Time step (ms).
+    dtype : {'float64', 'float32'}
+        Arithmetic type of the neuron state and synaptic sums (the reference's
+        ``brainstate.environ`` precision).
+    max_window : int
+        Upper bound on the steps advanced per update launch (0 = automatic).
+    """
+    __module__ = 'brainpy.state_amd'
+
+    def __init__(self, *, dt, dtype='float64', max_window=0):
+        self._dt_q = dt
+        self.dt = u.scalar(dt, 'time')
+        if self.dt <= 0:
+            raise ValueError('dt must be > 0')
+        if dtype in ('float64', 64, np.float64):
+            self.dtype = np.float64
+        elif dtype in ('float32', 32, np.float32):
+            self.dtype = np.float32
+        else:
+            raise ValueError(f"dtype must be 'float64' or 'float32', got {dtype!r}")
+        self.max_window = int(max_window)
+        self._pops = []              # neuron populations, creation order
+        self._taps = {}              # id(spike_recorder) -> (pop, idx)
+        self._analog_taps = {}       # id(multimeter) -> (pop, idx, names)
+        self._weight_taps = {}
+        self._records = []           # projections and generator channels, ordinal order
+        self._proj_counter = itertools.count()
+        self._engine = None
+        self._base_step = 0
+        self._rollout_ready = False
+
+    # -- creation --------------------------------------------------------
+    def create(self, model_cls, size=1, *, params=None, positions=None, **kw) -> NodeView:
+        if positions is not None:
+            raise NotImplementedError('spatial layers are not part of the network-step engine')
+        p = dict(params or {})
+        p.update(kw)
+        if _is_generator(model_cls):
+            k = int(np.prod(size)) if isinstance(size, (tuple, list)) else int(size)
+            if k > 1:
+                return NodeView([_GenSegment(_GeneratorSpec(
+                    model_cls, {key: _index_channel(v, i, k) for key, v in p.items()}))
+                    for i in range(k)])
+            return NodeView([_GenSegment(_GeneratorSpec(model_cls, p))])
+        mod = model_cls(size, **p)
+        if isinstance(mod, (spike_recorder, multimeter)):
+            return NodeView([_Segment(mod, np.arange(1))])
+        if not isinstance(mod, Neuron):
+            raise TypeError(f'{model_cls!r} is not a supported neuron model or device')
+        self._pops.append(mod)
+        self._invalidate()
+        return NodeView.of(mod)
+
+    # -- connection ------------------------------------------------------
+    def connect(self, pre, post, *, rule=all_to_all, weight=None, delay=None, comm='dense',
+                receptor_type=None, synapse=None, vt=None, allow_autapses=True,
+                allow_multapses=True, seed=None):
+        """Connect ``pre`` to ``post`` or register a recorder tap (simulator.py:606-683).
+
+        ``delay`` may also be a per-edge array (one entry per realised edge, in
+        the rule's edge order) or a :mod:`~brainpy.state_amd.dist` distribution
+        sampled per edge; both must be integer multiples of ``dt``.
+        """
+        if comm not in ('dense', 'sparse'):
+            raise ValueError(f"comm must be 'dense' or 'sparse', got {comm!r}")
+        if len(pre.segments) == 1 and isinstance(pre.segments[0].population, multimeter):
+            rec = pre.segments[0].population
+            if len(post.segments) != 1:
+                raise NotImplementedError('a voltmeter/multimeter records a single population segment')
+            seg = post.segments[0]
+            for name in rec.record_from:
+                if name not in seg.population.recordables:
+                    raise KeyError(f'recordable {name!r} is not available on '
+                                   f'{type(seg.population).__name__}')
+            self._analog_taps[id(rec)] = (seg.population, np.asarray(seg.indices), tuple(rec.record_from))
+            self._invalidate()
+            return None
+        if len(post.segments) == 1 and isinstance(post.segments[0].population, spike_recorder):
+            if len(pre.segments) != 1:
+                raise NotImplementedError('recording a multi-segment view requires one recorder per segment')
+            seg = pre.segments[0]
+            if seg.population is None:
+                raise NotImplementedError('recording a generator is not supported')
+            self._taps[id(post.segments[0].population)] = (seg.population, np.asarray(seg.indices))
+            self._invalidate()
+            return None
+        seg_weights = self._segment_weights(weight, len(pre.segments))
+        projs = []
+        for pre_seg, w_seg in zip(pre.segments, seg_weights):
+            for post_seg in post.segments:
+                pr = self._connect_pair(pre_seg, post_seg, rule, w_seg, delay, allow_autapses,
+                                        allow_multapses, seed, receptor_type, synapse)
+                if pr is not None:
+                    projs.append(pr)
+        self._invalidate()
+        if not projs:
+            return None
+        return projs[0] if len(projs) == 1 else projs
+
+    @staticmethod
+    def _segment_weights(weight, n_seg):
+        if n_seg > 1 and weight is not None and np.ndim(getattr(weight, 'value', weight)) == 1 \
+                and len(np.atleast_1d(getattr(weight, 'value', weight))) == n_seg:
+            return [weight[i] for i in range(n_seg)]
+        return [weight] * n_seg
+
+    @staticmethod
+    def _derive_seed(base, ordinal: int) -> int:
+        """Simulator._derive_seed (simulator.py:926-935)."""
+        b = 0 if base is None else int(base)
+        return (b * 1_000_003 + ordinal + 1) & 0x7FFFFFFF
+
+    def _receptor(self, post_pop, receptor_type):
+        if isinstance(post_pop, aeif_cond_alpha):
+            if receptor_type is None:
+                raise ValueError('aeif_cond_alpha is a two-receptor model: connect with '
+                                 'receptor_type=1 (g_ex) or 2 (g_in)')
+            k = int(receptor_type)
+            if not 1 <= k <= 2:
+                raise ValueError(f'receptor_type {k} out of range [1, 2]')
+            return k - 1
+        if receptor_type is not None:
+            raise NotImplementedError(f'{type(post_pop).__name__} has no receptor ports')
+        return 0
+
+    def _connect_pair(self, pre_seg, post_seg, rule, weight, delay, allow_autapses,
+                      allow_multapses, seed, receptor_type, synapse):
+        ordinal = next(self._proj_counter)
+        post_pop = post_seg.population
+        if post_pop is None or not isinstance(post_pop, Neuron):
+            raise TypeError('connect target must be a neuron population')
+        receptor = self._receptor(post_pop, receptor_type)
+        wdim = _weight_dim(post_pop)
+        post_idx = np.asarray(post_seg.indices, dtype=np.int32)
+        if isinstance(pre_seg, _GenSegment):
+            spec = pre_seg.spec
+            if synapse is not None:
+                raise NotImplementedError('plastic synapses from generators are not supported')
+            params = dict(spec.params)
+            params['rng_seed'] = self._derive_seed(params.get('rng_seed'), ordinal)
+            gen = spec.model_cls(len(post_idx), **params)
+            w = 1.0 if weight is None else u.magnitude(weight, wdim)
+            if np.ndim(w) != 0:
+                raise NotImplementedError('generator connections take a scalar weight')
+            rec = _GenChannel(ordinal, gen, post_pop, post_idx, float(w),
+                              delay_to_steps(delay, self.dt), receptor, params['rng_seed'])
+            if np.ndim(rec.delay) != 0:
+                raise NotImplementedError('generator connections take a scalar delay')
+            self._records.append(rec)
+            return rec
+        pre_pop = pre_seg.population
+        if not isinstance(pre_pop, Neuron):
+            raise TypeError('connect source must be a neuron population or a generator')
+        pre_sel = np.asarray(pre_seg.indices, dtype=np.int32)
+        n_pre, n_post = len(pre_sel), len(post_idx)
+        dseed = self._derive_seed(seed, ordinal)
+        if isinstance(rule, _OneToOne):
+            spec = rule.sample(n_pre, n_post, seed=dseed, pre_is_post=False,
+                               allow_autapses=True, allow_multapses=True)
+        else:
+            if not isinstance(rule, ConnRule):
+                raise TypeError(f'rule must be a connection rule, got {rule!r}')
+            spec = rule.sample(n_pre, n_post, seed=dseed, pre_is_post=(pre_pop is post_pop),
+                               allow_autapses=allow_autapses, allow_multapses=allow_multapses)
+        pre_l = pre_sel[np.asarray(spec.pre_idx, dtype=np.int64)]
+        post_l = post_idx[np.asarray(spec.post_idx, dtype=np.int64)]
+        E = int(spec.n_edges)
+        # weights (resolve_param, connectivity.py:385-405)
+        if synapse is not None:
+            if not isinstance(synapse, stdp_synapse):
+                raise NotImplementedError(f'synapse model {type(synapse).__name__} is not supported')
+            w0 = synapse.weight if weight is None else weight
+            w = u.magnitude(w0, wdim) if not isinstance(w0, Distribution) else w0
+        else:
+            w = 1.0 if weight is None else weight
+        if isinstance(w, Distribution):
+            wv = w.sample((E,), dseed + 7919)
+            w = np.asarray(u.magnitude(wv, wdim), dtype=np.float64)
+        else:
+            w = np.asarray(u.magnitude(w, wdim), dtype=np.float64)
+        if w.ndim == 0 or w.shape == (1,):
+            w = float(w.reshape(()))
+        elif w.shape != (E,):
+            raise ValueError(f'parameter array shape {w.shape} does not match target {(E,)}')
+        # delays: scalar, per-pre-neuron (axonal), per-edge, or a distribution
+        if synapse is not None and delay is None:
+            delay = synapse.delay
+        if isinstance(delay, Distribution):
+            dv = delay.sample((E,), dseed + 104729)
+            d = delay_to_steps(dv, self.dt)
+        else:
+            d = delay_to_steps(delay, self.dt)
+            if np.ndim(d) == 1:
+                if d.shape[0] == _flat_size(pre_pop) and d.shape[0] != E:
+                    d = d[pre_l]             # axonal: one delay per pre neuron
+                elif d.shape[0] != E:
+                    raise ValueError('delay array must have one entry per pre neuron or per edge')
+        if synapse is not None:
+            pr = PlasticProjection(ordinal, pre_pop, post_pop, pre_l, post_l,
+                                   np.broadcast_to(np.asarray(w, np.float64), (E,)).copy(), d,
+                                   receptor, rule=synapse)
+        else:
+            pr = StaticProjection(ordinal, pre_pop, post_pop, pre_l, post_l, w, d, receptor)
+        self._records.append(pr)
+        return pr
+
+    def record_weight(self, proj):
+        if not isinstance(proj, PlasticProjection):
+            raise TypeError('record_weight requires a plastic projection handle from '
+                            f'connect(..., synapse=spec); got {type(proj).__name__}')
+        self._weight_taps[id(proj)] = proj
+        return proj
+
+    # -- descriptor ----------------------------------------------------------
+    def _invalidate(self):
+        if self._engine is not None:
+            self._engine.close()
+        self._engine = None
+        self._rollout_ready = False
+
+    def build_desc(self):
+        """Flatten the network into the C-ABI descriptor (``bs_net_desc``)."""
+        return _NetBuilder(self).build()
+
+    # -- run ---------------------------------------------------------------
+    def _ensure_engine(self):
+        if self._engine is None:
+            L.require_device()
+            self._engine = _Engine(self.build_desc())
+        return self._engine
+
+    def reset_rollout(self, *, dt=None):
+        eng = self._ensure_engine()
+        eng.reset()
+        self._base_step = 0
+        self._rollout_ready = True
+
+    def simulate(self, duration, *, dt=None) -> SimulationResult:
+        self.reset_rollout()
+        return self._run_window(duration)
+
+    def cont(self, duration, *, dt=None) -> SimulationResult:
+        if not self._rollout_ready:
+            self.reset_rollout()
+        return self._run_window(duration)
+
+    def n_steps_of(self, duration) -> int:
+        T = u.scalar(duration, 'time')
+        return int(math.ceil(T / self.dt))   # len(arange(0, T, dt))
+
+    def _run_window(self, duration) -> SimulationResult:
+        eng = self._engine
+        T = u.scalar(duration, 'time')
+        n = self.n_steps_of(duration)
+        vm, (gids, steps) = eng.run(n)
+        return self._result(eng.desc, n, T, vm, gids, steps)
+
+    def _result(self, desc, n, T, vm, gids, steps) -> SimulationResult:
+        step0 = self._base_step
+        recs = {}
+        for rid, (pop, idx) in self._taps.items():
+            gid0 = desc.gid0[id(pop)]
+            spk = np.zeros((n, len(idx)), dtype=self.dtype)
+            sel = (gids >= gid0) & (gids < gid0 + pop.n)
+            lg = gids[sel] - gid0
+            st = steps[sel] - step0
+            cols = {}
+            for c, li in enumerate(idx):
+                cols.setdefault(int(li), []).append(c)
+            if len(cols) == len(idx):
+                colmap = np.full(pop.n, -1, dtype=np.int64)
+                colmap[np.asarray(idx)] = np.arange(len(idx))
+                cc = colmap[lg]
+                ok = cc >= 0
+                spk[st[ok], cc[ok]] = 1
+            else:
+                for a, b in zip(lg, st):
+                    for c in cols.get(int(a), ()):
+                        spk[b, c] = 1
+            recs[rid] = spk
+        traces = {}
+        for rid, (pop, idx, names) in self._analog_taps.items():
+            for name in names:
+                cols = desc.tap_cols[(rid, name)]
+                traces[f'{rid}|{name}'] = u.Quantity(vm[:, cols], _trace_unit(pop, name))
+        times = u.Quantity((step0 + np.arange(n)) * self.dt, u.ms)
+        self._base_step += n
+        return SimulationResult(recs, T, self.dt, traces=traces, times=times)
+
+    def __del__(self):
+        try:
+            if self._engine is not None:
+                self._engine.close()
+        except Exception:
+            pass
+
+
+def _trace_unit(pop, name):
+    return {'V_m': u.mV, 'g_ex': u.nS, 'g_in': u.nS, 'w': u.pA}.get(name, u.pA)
+
+
+def _index_channel(val, i, k):
+    v = getattr(val, 'value', val)
+    if np.ndim(v) == 1 and len(v) == k:
+        return val[i] if isinstance(val, u.Quantity) else v[i]
+    return val
+
+
+# ------------------------------------------------------------- descriptor
+class _Desc:
+    """ctypes descriptor plus the NumPy arrays it points into (kept alive)."""
+
+    def __init__(self):
+        self.keep = []
+        self.c = None
+        self.gid0 = {}
+        self.pop_index = {}
+        self.tap_cols = {}
+        self.n_taps = 0
+        self.proj_index = {}
+
+    def arr(self, a, ctype):
+        a = np.ascontiguousarray(a)
+        self.keep.append(a)
+        return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class _NetBuilder:
+    def __init__(self, sim: Simulator):
+        self.sim = sim
+
+    def build(self) -> _Desc:
+        sim = self.sim
+        dt = sim.dt
+        D = _Desc()
+        pops = sim._pops
+        npop = len(pops)
+        if npop == 0:
+            raise ValueError('the network has no neuron population')
+        for p, pop in enumerate(pops):
+            D.pop_index[id(pop)] = p
+        gid = 0
+        for pop in pops:
+            D.gid0[id(pop)] = gid
+            gid += pop.n
+        # channels per post population, in registration (ordinal) order
+        per_pop = [[] for _ in range(npop)]
+        for r in sim._records:
+            per_pop[D.pop_index[id(r.post_pop)]].append(r)
+        chans = (L.Channel * max(1, sum(len(x) for x in per_pop)))()
+        cpops = (L.Population * npop)()
+        tables, table_off = [], []
+        projs = []
+        c = 0
+        tdtype = np.float64 if sim.dtype == np.float64 else np.float32
+        # voltmeter taps: per population, concatenated in registration order
+        vm_lists = [[] for _ in range(npop)]
+        tap_order = []
+        for rid, (pop, idx, names) in sim._analog_taps.items():
+            for name in names:
+                if pop.recordables[name] != 0:
+                    raise NotImplementedError('only V_m is recorded by the engine')
+                p = D.pop_index[id(pop)]
+                start = len(vm_lists[p])
+                vm_lists[p].extend(int(i) for i in idx)
+                tap_order.append((rid, name, p, start, len(idx)))
+        # global tap column = concatenation over populations
+        pop_tap_base = np.cumsum([0] + [len(v) for v in vm_lists])
+        for rid, name, p, start, cnt in tap_order:
+            D.tap_cols[(rid, name)] = np.arange(pop_tap_base[p] + start, pop_tap_base[p] + start + cnt)
+        D.n_taps = int(pop_tap_base[-1])
+        rec_pops = {D.pop_index[id(pop)] for (pop, _) in sim._taps.values()}
+        off = 0
+        for p, pop in enumerate(pops):
+            P = cpops[p]
+            P.model = pop.model_id
+            P.n = pop.n
+            blk = pop.param_block(dt)
+            for k in range(L.BS_NPARAM):
+                P.params[k] = float(blk[k])
+            P.state0 = D.arr(pop.state0(dt).astype(np.float64), C.c_double)
+            P.ch_begin = c
+            P.ch_count = len(per_pop[p])
+            P.n_vm = len(vm_lists[p])
+            P.vm_idx = D.arr(np.asarray(vm_lists[p] or [0], dtype=np.int32), C.c_int32)
+            P.record_spikes = 1 if p in rec_pops else 0
+            for r in per_pop[p]:
+                ch = chans[c]
+                ch.receptor = r.receptor
+                if isinstance(r, _GenChannel):
+                    g = r.gen
+                    ch.kind = L.BS_CH_POISSON
+                    ch.weight = r.weight
+                    ch.lam = g.lam(dt)
+                    tmin, tmax = g.step_window(dt)
+                    ch.t_min_step = int(tmin)
+                    ch.t_max_step = int(min(tmax, np.iinfo(np.int64).max))
+                    ch.seed = int(r.seed) & 0xFFFFFFFF
+                    ch.delay_steps = int(r.delay)
+                    if len(r.post_idx) == pop.n and np.array_equal(r.post_idx, np.arange(pop.n)):
+                        ch.gen_index = None
+                    else:
+                        gi = np.full(pop.n, -1, dtype=np.int32)
+                        gi[r.post_idx] = np.arange(len(r.post_idx), dtype=np.int32)
+                        ch.gen_index = D.arr(gi, C.c_int32)
+                    table_off.append(-1)
+                else:
+                    pre_p = D.pop_index[id(r.pre_pop)]
+                    E = r.n_edges
+                    indeg = np.bincount(r.post_idx, minlength=pop.n) if E else np.zeros(1, np.int64)
+                    if r.homogeneous and not r.plastic:
+                        ch.kind = L.BS_CH_COUNT
+                        ch.weight = float(r.weight)
+                        kmax = int(indeg.max()) if E else 0
+                        if kmax > 65535:
+                            raise NotImplementedError('more than 65535 edges into one neuron '
+                                                      'from one projection')
+                        ch.max_count = kmax
+                        t = seq_sum_table(r.weight, kmax, tdtype)
+                        table_off.append(off)
+                        tables.append(t)
+                        off += len(t)
+                    else:
+                        ch.kind = L.BS_CH_FIXED
+                        wabs = np.abs(np.asarray(r.weight, np.float64))
+                        if r.plastic:
+                            wabs = np.maximum(wabs, abs(r.rule.Wmax))
+                        m = float(np.bincount(r.post_idx, weights=wabs, minlength=pop.n).max()) if E else 0.0
+                        ch.fixed_shift = int(min(1000, max(0, 61 - math.ceil(math.log2(m)) - 1))) if m > 0 else 0
+                        table_off.append(-1)
+                    pj = L.Projection()
+                    pj.pre_pop = pre_p
+                    pj.post_pop = p
+                    pj.channel = c
+                    pj.plastic = 1 if r.plastic else 0
+                    pj.n_edges = E
+                    pj.pre = D.arr(r.pre_idx.astype(np.int32), C.c_int32)
+                    pj.post = D.arr(r.post_idx.astype(np.int32), C.c_int32)
+                    if np.ndim(r.delay) == 0:
+                        pj.delay = None
+                        pj.delay_steps = int(r.delay)
+                    else:
+                        pj.delay = D.arr(np.asarray(r.delay, np.int32), C.c_int32)
+                        pj.delay_steps = 0
+                    if np.ndim(r.weight) != 0:
+                        pj.weight = D.arr(np.asarray(r.weight, np.float64), C.c_double)
+                    else:
+                        pj.weight = None
+                    if r.plastic:
+                        s = r.rule
+                        pj.stdp.tau_plus, pj.stdp.tau_minus = s.tau_plus, s.tau_minus
+                        pj.stdp.lambda_, pj.stdp.alpha = s.lambda_, s.alpha
+                        pj.stdp.mu_plus, pj.stdp.mu_minus, pj.stdp.Wmax = s.mu_plus, s.mu_minus, s.Wmax
+                        pj.stdp.decay_plus = float(np.exp(-dt / s.tau_plus))
+                        pj.stdp.decay_minus = float(np.exp(-dt / s.tau_minus))
+                    D.proj_index[id(r)] = len(projs)
+                    projs.append(pj)
+                    D.keep.append(pj)
+                c += 1
+        cprojs = (L.Projection * max(1, len(projs)))(*projs) if projs else (L.Projection * 1)()
+        tab = np.concatenate(tables) if tables else np.zeros(1)
+        d = L.NetDesc()
+        d.abi_version = L.BS_ABI_VERSION
+        d.dtype = L.BS_F64 if sim.dtype == np.float64 else L.BS_F32
+        d.dt_ms = dt
+        d.n_pops = npop
+        d.n_channels = c
+        d.n_projs = len(projs)
+        d.max_window = sim.max_window
+        d.pops = cpops
+        d.channels = chans
+        d.projs = cprojs
+        d.tables = D.arr(tab, C.c_double)
+        d.table_off = D.arr(np.asarray(table_off or [-1], dtype=np.int64), C.c_int64)
+        d.n_tables = len(tab)
+        D.keep += [cpops, chans, cprojs]
+        D.c = d
+        D.ntot = gid
+        return D
+
+
+class _Engine:
+    """Owns one ``bs_net`` handle."""
+
+    def __init__(self, desc: _Desc):
+        self.desc = desc
+        self.lib = L.lib()
+        h = C.c_void_p()
+        L.check(self.lib.bs_net_create(C.byref(desc.c), C.byref(h)))
+        self.h = h
+
+    def reset(self):
+        L.check(self.lib.bs_net_reset(self.h))
+
+    def set_timing(self, on):
+        L.check(self.lib.bs_net_set_timing(self.h, 1 if on else 0))
+
+    def run(self, n_steps):
+        nt = self.desc.n_taps
+        vm = np.zeros((n_steps, max(nt, 0)), dtype=np.float64)
+        ptr = vm.ctypes.data_as(C.POINTER(C.c_double)) if nt > 0 else None
+        L.check(self.lib.bs_net_run(self.h, int(n_steps), ptr))
+        cnt = C.c_int64()
+        L.check(self.lib.bs_net_spike_count(self.h, C.byref(cnt)))
+        gid = np.zeros(max(1, cnt.value), dtype=np.int32)
+        st = np.zeros(max(1, cnt.value), dtype=np.int32)
+        L.check(self.lib.bs_net_copy_spikes(self.h, gid.ctypes.data_as(C.POINTER(C.c_int32)),
+                                            st.ctypes.data_as(C.POINTER(C.c_int32)), len(gid)))
+        return vm, (gid[:cnt.value], st[:cnt.value])
+
+    def timing(self):
+        vals = [C.c_double() for _ in range(3)] + [C.c_int64() for _ in range(2)]
+        L.check(self.lib.bs_net_last_timing(self.h, *[C.byref(v) for v in vals]))
+        return dict(deliver_ms=vals[0].value, update_ms=vals[1].value, total_ms=vals[2].value,
+                    n_deliver=vals[3].value, n_update=vals[4].value)
+
+    def bytes(self):
+        a, b, n = C.c_double(), C.c_double(), C.c_int64()
+        L.check(self.lib.bs_net_last_bytes(self.h, C.byref(a), C.byref(b), C.byref(n)))
+        return dict(deliver_bytes=a.value, update_bytes=b.value, n_events=n.value)
+
+    def state(self, pop_index, var, n):
+        out = np.zeros(max(1, n))
+        L.check(self.lib.bs_net_get_state(self.h, pop_index, var,
+                                          out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out[:n]
+
+    def weights(self, proj_index, n):
+        out = np.zeros(max(1, n))
+        L.check(self.lib.bs_net_get_weights(self.h, proj_index,
+                                            out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out[:n]
+
+    def close(self):
+        if getattr(self, 'h', None) is not None and self.h.value:
+            self.lib.bs_net_destroy(self.h)
+            self.h = C.c_void_p()

@@ -1,0 +1,189 @@
+/*
+ * brainspike_hip.h -- C ABI of libbrainspike_hip.so, the MI355X (gfx950) engine
+ * behind the NEST-style Simulator network step.
+ *
+ * Plain C: pointers, sizes, enums. No torch, HIP or C++ types cross this line.
+ * Every entry point returns 0 on success or a negative BS_E* code; the message
+ * of the last failure on the calling thread is returned by bs_last_error().
+ * Nothing aborts the process. Calls on one bs_net handle must be serialised by
+ * the caller; different handles are independent (each owns one HIP stream).
+ *
+ * Reference interfaces replaced (paths relative to the brainspike tree):
+ *   bs_net_create   <- Simulator.create/connect building EventProjection,
+ *                      EventPlasticProj, InputDelay, _SparseEventMatMul
+ *                      (_nest_network/simulator.py:458-683, 1399-1514;
+ *                       event_proj.py:77-206; event_plastic.py:276-340)
+ *   bs_net_run      <- Simulator._run_window: brainstate for_loop over
+ *                      Simulator.update phases 1 and 2 plus recorder taps
+ *                      (_nest_network/simulator.py:1719-1788, 1841-1883)
+ *   bs_net_reset    <- Simulator.reset_rollout (simulator.py:1803-1820)
+ *   spikes/traces   <- SimulationResult.spikes / .trace (simulator.py:306-414)
+ *   weights         <- EventPlasticProj.weight State, record_weight
+ *                      (event_plastic.py:305, simulator.py:884-910)
+ */
+#ifndef BRAINSPIKE_HIP_H
+#define BRAINSPIKE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BS_ABI_VERSION 1
+
+/* ---- return codes ------------------------------------------------------ */
+#define BS_OK 0
+#define BS_EINVAL (-1)      /* invalid argument / descriptor             */
+#define BS_ENOMEM (-2)      /* host or device allocation failed          */
+#define BS_EHIP (-3)        /* HIP runtime error                         */
+#define BS_ECAPACITY (-4)   /* a fixed-capacity buffer would overflow    */
+#define BS_ENUMERIC (-5)    /* numerical instability flagged on device   */
+#define BS_ENODEV (-6)      /* no usable gfx950 device                   */
+
+/* ---- enums -------------------------------------------------------------- */
+enum bs_dtype { BS_F64 = 0, BS_F32 = 1 };
+
+enum bs_model {
+  BS_IAF_PSC_DELTA = 1,   /* _nest_neuron/iaf_psc_delta.py:449-521 */
+  BS_IAF_PSC_EXP = 2,     /* _nest_neuron/iaf_psc_exp.py:552-682   */
+  BS_IAF_PSC_ALPHA = 3,   /* _nest_neuron/iaf_psc_alpha.py:558-684 */
+  BS_AEIF_COND_ALPHA = 4  /* _nest_neuron/aeif_cond_alpha.py:627-747 */
+};
+
+/* Input channels of a population, summed in list order each step exactly as
+ * Dynamics.sum_delta_inputs folds its dict (_base.py:312-383). */
+enum bs_channel_kind {
+  BS_CH_COUNT = 1,   /* homogeneous-weight events: value = S[k], S[k] = sequential
+                        sum of k copies of the weight (CSR event matmul,
+                        projections.py:36-59, with per-edge equal weights)       */
+  BS_CH_FIXED = 2,   /* heterogeneous weights: exact 2^-F fixed-point sum       */
+  BS_CH_POISSON = 3  /* one_to_one poisson_generator: value = count * weight
+                        (event_proj.py:311-312, poisson_generator.py:501-576)    */
+};
+
+/* Parameter block lengths (doubles) per model; layouts are documented in
+ * brainpy/state_amd/models.py and mirrored by csrc/bs_models.h. */
+#define BS_NPARAM 40
+/* state variables per neuron, all stored in the network dtype except the
+ * refractory counter (int32). Order per model in csrc/bs_models.h. */
+#define BS_NSTATE 9
+
+typedef struct bs_population {
+  int32_t model;            /* enum bs_model                                    */
+  int32_t n;                /* neurons                                          */
+  double params[BS_NPARAM]; /* model parameters incl. host-precomputed FP64
+                               propagators                                      */
+  const double* state0;     /* [BS_NSTATE][n] initial state (row per variable);
+                               NULL = model defaults from params                 */
+  int32_t ch_begin;         /* this population's channels: channels[ch_begin ..  */
+  int32_t ch_count;         /*                       ch_begin + ch_count)       */
+  int32_t n_vm;             /* voltmeter taps: local indices vm_idx[0..n_vm)     */
+  const int32_t* vm_idx;
+  int32_t record_spikes;    /* 1: keep this population's spikes for the caller  */
+  int32_t _pad;
+} bs_population;
+
+typedef struct bs_channel {
+  int32_t kind;            /* enum bs_channel_kind                               */
+  int32_t receptor;        /* aeif_cond_alpha: 0 -> g_ex, 1 -> g_in; else 0       */
+  double weight;           /* COUNT, POISSON                                     */
+  int32_t max_count;       /* COUNT: largest event count per (post, step)        */
+  int32_t fixed_shift;     /* FIXED: F, value = sum(q_e) * 2^-F                  */
+  /* POISSON only: */
+  double lam;              /* rate * dt / 1000                                   */
+  int64_t t_min_step;      /* active iff t_min_step < step <= t_max_step        */
+  int64_t t_max_step;
+  uint32_t seed;           /* Simulator._derive_seed(rng_seed, ordinal)          */
+  int32_t delay_steps;     /* delay / dt                                         */
+  const int32_t* gen_index;/* [n_post_pop] generator element feeding each post,
+                              -1 if none; NULL = identity                        */
+} bs_channel;
+
+/* STDP parameters (stdp_synapse.py:115-179): additive/multiplicative pair rule. */
+typedef struct bs_stdp {
+  double tau_plus, tau_minus, lambda_, alpha, mu_plus, mu_minus, Wmax;
+  double decay_plus;   /* exp(-dt/tau_plus)  (event_plastic.py:411)           */
+  double decay_minus;  /* exp(-dt/tau_minus)                                   */
+} bs_stdp;
+
+typedef struct bs_projection {
+  int32_t pre_pop, post_pop; /* population indices                              */
+  int32_t channel;           /* index into channels[] (global)                   */
+  int32_t plastic;           /* 0 static, 1 stdp_synapse                          */
+  int64_t n_edges;
+  const int32_t* pre;        /* [n_edges] population-local pre index             */
+  const int32_t* post;       /* [n_edges] population-local post index            */
+  const int32_t* delay;      /* [n_edges] delay in steps, or NULL                 */
+  int32_t delay_steps;       /* homogeneous delay in steps when delay == NULL    */
+  int32_t _pad;
+  const double* weight;      /* [n_edges] initial weights (FIXED/plastic) or NULL */
+  bs_stdp stdp;
+} bs_projection;
+
+typedef struct bs_net_desc {
+  int32_t abi_version;       /* BS_ABI_VERSION                                   */
+  int32_t dtype;             /* enum bs_dtype                                    */
+  double dt_ms;
+  int32_t n_pops, n_channels, n_projs, max_window; /* max_window 0 = automatic   */
+  const bs_population* pops;
+  const bs_channel* channels;
+  const bs_projection* projs;
+  const double* tables;      /* concatenated S tables of all COUNT channels      */
+  const int64_t* table_off;  /* [n_channels] offset into tables, -1 if none      */
+  int64_t n_tables;
+} bs_net_desc;
+
+typedef struct bs_net bs_net;
+
+/* ---- library ------------------------------------------------------------ */
+int bs_abi_version(void);
+const char* bs_last_error(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+int bs_device_count(void);
+
+/* ---- network ------------------------------------------------------------ */
+/* Copies the descriptor (params, edges, delays, weights, tables) to device
+ * memory the handle owns; builds CSR rows and the arrival ring. */
+int bs_net_create(const bs_net_desc* desc, bs_net** out);
+int bs_net_destroy(bs_net* net);
+/* Restore initial state, clear rings, traces and plastic weights; step = 0. */
+int bs_net_reset(bs_net* net);
+/* Advance n_steps. vm_out: host [n_steps][total voltmeter taps] (f64) or NULL.
+ * Spikes of populations with record_spikes are kept for bs_net_copy_spikes. */
+int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out);
+/* Steps taken since create / reset. */
+int bs_net_step(bs_net* net, int64_t* step);
+/* Spikes recorded by the last bs_net_run, sorted by (step, gid). */
+int bs_net_spike_count(bs_net* net, int64_t* n);
+int bs_net_copy_spikes(bs_net* net, int32_t* gid, int32_t* step, int64_t cap);
+/* Neuron state after the last step: var in [0, BS_NSTATE), values as f64. */
+int bs_net_get_state(bs_net* net, int32_t pop, int32_t var, double* out);
+/* Plastic weights of projection proj in the reference's CSR (sorted-by-pre,
+ * stable) edge order (event_plastic.py:293-296). */
+int bs_net_get_weights(bs_net* net, int32_t proj, double* out);
+/* Per-kernel HIP-event timing of the following runs (off by default: the
+ * events add launch overhead). */
+int bs_net_set_timing(bs_net* net, int on);
+/* Timing of the last run: device milliseconds spent in the delivery kernels,
+ * the update kernels and the whole run, plus kernel launches per kind. */
+int bs_net_last_timing(bs_net* net, double* deliver_ms, double* update_ms,
+                       double* total_ms, int64_t* n_deliver, int64_t* n_update);
+/* Algorithmic bytes moved by the last run (roofline numerators). */
+int bs_net_last_bytes(bs_net* net, double* deliver_bytes, double* update_bytes,
+                      int64_t* n_events);
+
+/* ---- test hooks: device arithmetic evaluated elementwise ------------------
+ * No counterpart in the reference; they let tests compare the device's
+ * exp/log/pow, Threefry-2x32 and Poisson sampler bit-for-bit with the C
+ * oracle. op: 0 exp, 1 log, 2 pow(a, b). */
+int bs_debug_math(int32_t op, int64_t n, const double* a, const double* b, double* out);
+int bs_debug_threefry(int64_t n, const uint32_t* k0, const uint32_t* k1, const uint32_t* c0,
+                      const uint32_t* c1, uint32_t* o0, uint32_t* o1);
+int bs_debug_poisson(int64_t n, const uint32_t* seed, const uint32_t* step, const uint32_t* elem,
+                     double lam, uint32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BRAINSPIKE_HIP_H */

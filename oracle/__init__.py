@@ -1,0 +1,132 @@
+"""CPU oracle of the network step -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline``
+leg may import this package; the product (``brainpy.state_amd``) never does.
+It loads ``oracle/liboracle.so`` (``oracle.cpp``, built by ``oracle/build.py``
+with g++ -ffp-contract=off) and runs the reference's algorithm on one CPU
+thread from the same ``bs_net_desc`` the engine consumes.
+
+Pinning status of each restated function is recorded in ``oracle/PINNING.md``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, 'liboracle.so')
+_lib = None
+
+
+def build():
+    tmp = LIB + '.tmp'
+    cmd = ['g++', '-O2', '-std=c++17', '-fPIC', '-shared', '-ffp-contract=off',
+           '-o', tmp, os.path.join(HERE, 'oracle.cpp')]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError('oracle build failed:\n' + res.stdout + res.stderr)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        h = C.CDLL(LIB)
+        P = C.c_void_p
+        sig = {
+            'orc_last_error': (C.c_char_p, []),
+            'orc_create': (C.c_int, [C.c_void_p, C.POINTER(P)]),
+            'orc_destroy': (C.c_int, [P]),
+            'orc_run': (C.c_int, [P, C.c_int64, C.POINTER(C.c_double)]),
+            'orc_spike_count': (C.c_int, [P, C.POINTER(C.c_int64)]),
+            'orc_copy_spikes': (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int64]),
+            'orc_get_state': (C.c_int, [P, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
+            'orc_get_weights': (C.c_int, [P, C.c_int32, C.POINTER(C.c_double)]),
+            'orc_threefry2x32': (None, [C.c_uint32] * 4 + [C.POINTER(C.c_uint32)] * 2),
+            'orc_poisson': (C.c_uint32, [C.c_uint32] * 3 + [C.c_double]),
+            'orc_exp': (C.c_double, [C.c_double]),
+            'orc_log': (C.c_double, [C.c_double]),
+            'orc_pow': (C.c_double, [C.c_double, C.c_double]),
+        }
+        for k, (res, args) in sig.items():
+            f = getattr(h, k)
+            f.restype = res
+            f.argtypes = args
+        _lib = h
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(f'oracle error {rc}: {lib().orc_last_error().decode()}')
+
+
+def threefry2x32(k0, k1, c0, c1):
+    a, b = C.c_uint32(), C.c_uint32()
+    lib().orc_threefry2x32(k0, k1, c0, c1, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def poisson(seed, step, elem, lam):
+    return int(lib().orc_poisson(seed, step, elem, lam))
+
+
+def exp(x):
+    return lib().orc_exp(float(x))
+
+
+def log(x):
+    return lib().orc_log(float(x))
+
+
+def pow(x, y):  # noqa: A001
+    return lib().orc_pow(float(x), float(y))
+
+
+class OracleNet:
+    """The oracle instance of a network descriptor (``Simulator.build_desc()``)."""
+
+    def __init__(self, desc):
+        self.desc = desc
+        self.h = C.c_void_p()
+        _check(lib().orc_create(C.cast(C.pointer(desc.c), C.c_void_p), C.byref(self.h)))
+
+    def run(self, n_steps):
+        nt = self.desc.n_taps
+        vm = np.zeros((n_steps, nt))
+        ptr = vm.ctypes.data_as(C.POINTER(C.c_double)) if nt else None
+        _check(lib().orc_run(self.h, int(n_steps), ptr))
+        n = C.c_int64()
+        _check(lib().orc_spike_count(self.h, C.byref(n)))
+        g = np.zeros(max(1, n.value), np.int32)
+        s = np.zeros(max(1, n.value), np.int32)
+        _check(lib().orc_copy_spikes(self.h, g.ctypes.data_as(C.POINTER(C.c_int32)),
+                                     s.ctypes.data_as(C.POINTER(C.c_int32)), len(g)))
+        return vm, g[:n.value], s[:n.value]
+
+    def state(self, pop, var, n):
+        out = np.zeros(max(1, n))
+        _check(lib().orc_get_state(self.h, pop, var, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out[:n]
+
+    def weights(self, proj, n):
+        out = np.zeros(max(1, n))
+        _check(lib().orc_get_weights(self.h, proj, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out[:n]
+
+    def close(self):
+        if self.h.value:
+            lib().orc_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

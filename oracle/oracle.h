@@ -1,0 +1,53 @@
+/*
+ * oracle.h -- CPU restatement of the reference's NEST-style Simulator step.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the parity checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it. The
+ * product (brainpy/state_amd) never calls it.
+ *
+ * It consumes the same bs_net_desc as libbrainspike_hip.so and advances the
+ * network with the reference's own algorithm, step by step on one CPU thread:
+ *   phase 1  every projection reads the previous step's holder vector, pushes
+ *            it through a delay ring over the full pre-population vector
+ *            (InputDelay, _nest_network/delay.py:114-128) or a per-edge delay
+ *            (_kernel/delay_scatter.py semantics), and accumulates
+ *            y[post] += w over its CSR rows in ascending pre order
+ *            (_nest_network/projections.py:36-59, connectivity.py:39-54);
+ *            generators contribute count * weight (event_proj.py:311-312);
+ *   phase 2  every population sums its inputs in projection order
+ *            (_base.py:312-383) and runs its model update; holders take the
+ *            binarised spikes, generators their raw counts
+ *            (simulator.py:1756-1788).
+ */
+#ifndef BS_ORACLE_H
+#define BS_ORACLE_H
+#include <stdint.h>
+#include "../include/brainspike_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_net orc_net;
+
+const char* orc_last_error(void);
+int orc_create(const bs_net_desc* desc, orc_net** out);
+int orc_destroy(orc_net* net);
+int orc_run(orc_net* net, int64_t n_steps, double* vm_out);
+int orc_spike_count(orc_net* net, int64_t* n);
+int orc_copy_spikes(orc_net* net, int32_t* gid, int32_t* step, int64_t cap);
+int orc_get_state(orc_net* net, int32_t pop, int32_t var, double* out);
+int orc_get_weights(orc_net* net, int32_t proj, double* out);
+
+/* Elementwise primitives (the definitions the device must reproduce). */
+void orc_threefry2x32(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t c1, uint32_t* o0,
+                      uint32_t* o1);
+uint32_t orc_poisson(uint32_t seed, uint32_t step, uint32_t elem, double lam);
+double orc_exp(double x);
+double orc_log(double x);
+double orc_pow(double x, double y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
