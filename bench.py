@@ -1,0 +1,145 @@
+"""Benchmark: synaptic events/s and real-time factor of Brunel-11250 on one MI355X.
+
+Workload (one GPU): Brunel balanced network, NE = 9000 + NI = 2250
+iaf_psc_delta neurons, fixed in-degree CE = 900 / CI = 225 (12,656,250
+synapses), per-edge delays uniform over the integer multiples of dt in
+1.0-3.0 ms, Poisson drive at 2 nu_th (lambda = 2 per step), dt = 0.1 ms, spikes
+recorded from every neuron and V_m from 10. A bench "step" is one network time
+step (dt); ``--steps`` steps are timed after ``--warmup`` untimed steps, with
+the network already resident in HBM.
+
+The JSON line carries:
+  value        synaptic events/s (sum over spikes of the spiking neuron's
+               out-degree, divided by the timed wall time)
+  rtf          wall time / simulated time
+  roofline     the dominant kernel's algorithmic bytes per launch over its mean
+               HIP-event duration, against 8 TB/s
+  cpu_baseline the CPU oracle (one thread) on a bounded sample of the same
+               network, events/s
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=100000)
+    ap.add_argument('--warmup', type=int, default=2000)
+    ap.add_argument('--dtype', default='float64', choices=['float64', 'float32'])
+    ap.add_argument('--cpu-steps', type=int, default=3000)
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--profile-only', action='store_true',
+                    help='run warmup + timed steps only (for rocprofv3)')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported '
+              '(multi-GPU sharding is design-only, see DESIGN.md)', file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from brainpy.state_amd.networks import brunel_delta
+
+    t0 = time.time()
+    sim, h = brunel_delta(order=2250, delay=('uniform', 1.0, 3.0), n_vm=10, dtype=args.dtype)
+    desc = sim.build_desc()
+    eng = sim._ensure_engine()
+    log(f'built Brunel-11250 in {time.time() - t0:.1f}s')
+    eng.reset()
+    eng.run(args.warmup)
+    log(f'warmup {args.warmup} steps done')
+
+    # ---- timed run (bs_net_run returns after the stream has drained)
+    t1 = time.perf_counter()
+    _, (gids, steps) = eng.run(args.steps)
+    wall = time.perf_counter() - t1
+    by = eng.bytes()
+    n_events = by['n_events']
+    sim_s = args.steps * sim.dt / 1000.0
+    value = n_events / wall
+    rate = len(gids) / 11250 / sim_s
+    log(f'timed {args.steps} steps: {wall:.3f}s, {len(gids)} spikes ({rate:.1f} Hz), '
+        f'{n_events} events, {value:.3e} ev/s, RTF {wall / sim_s:.4f}')
+    if args.profile_only:
+        return
+
+    # ---- kernel timing (HIP events around every launch, separate run)
+    eng.set_timing(True)
+    nprof = min(args.steps, 20000)
+    eng.run(nprof)
+    tm = eng.timing()
+    by2 = eng.bytes()
+    eng.set_timing(False)
+    n_launch = max(1, tm['n_update'])
+    kms = tm['update_ms'] / n_launch
+    kbytes = (by2['deliver_bytes'] + by2['update_bytes']) / n_launch
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    log(f'k_window: {kms * 1e3:.2f} us/launch, {kbytes / 1e6:.3f} MB/launch '
+        f'(gather {by2["deliver_bytes"] / n_launch / 1e6:.3f} MB, update '
+        f'{by2["update_bytes"] / n_launch / 1e6:.3f} MB), {n_launch} launches')
+    roofline = {'bound': 'hbm', 'kernel': 'k_window', 'achieved': round(achieved, 2),
+                'peak': 8000.0, 'unit': 'GB/s', 'frac': round(achieved / 8000.0, 5),
+                'traffic': None, 'bytes_per_launch': round(kbytes),
+                'us_per_launch': round(kms * 1e3, 3),
+                'launch_fraction_of_wall': round(tm['update_ms'] / max(1e-9, tm['total_ms']), 4)}
+
+    # ---- CPU baseline: the oracle on a bounded sample of the same network
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        orc = oracle.OracleNet(desc)
+        t2 = time.perf_counter()
+        _, g_o, s_o = orc.run(args.cpu_steps)
+        cwall = time.perf_counter() - t2
+        rowlen = np.zeros(desc.ntot, dtype=np.int64)
+        for pj in desc.c.projs[:desc.c.n_projs]:
+            gid0 = sum(desc.c.pops[k].n for k in range(pj.pre_pop))
+            pre = np.ctypeslib.as_array(pj.pre, shape=(pj.n_edges,))
+            rowlen += np.bincount(pre + gid0, minlength=desc.ntot)
+        ev = int(rowlen[g_o].sum())
+        cpu = {'value': round(ev / cwall, 1), 'unit': 'events/s', 'cores': 1, 'kind': 'port',
+               'sample': f'first {args.cpu_steps} steps ({args.cpu_steps * sim.dt:.0f} ms) of the '
+                         f'same network, {len(g_o)} spikes, {cwall:.1f}s wall',
+               'rtf': round(cwall / (args.cpu_steps * sim.dt / 1000.0), 3)}
+        log(f'cpu oracle: {args.cpu_steps} steps in {cwall:.1f}s, {ev / cwall:.3e} ev/s')
+
+    out = {
+        'metric': 'synaptic events/s (Brunel-11250, 1 GPU)',
+        'value': round(value, 1),
+        'unit': 'events/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(wall / args.steps * 1e3, 6),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64' if args.dtype == 'float64' else 'f32',
+        'data': 'synthetic (seeded Brunel network, on-device Poisson drive)',
+        'config': {'workload': 'brunel-11250 iaf_psc_delta, CE=900 CI=225, per-edge delays '
+                               '1.0-3.0 ms, dt=0.1 ms', 'n_neurons': 11250,
+                   'n_synapses': int(desc.c.projs[0].n_edges + desc.c.projs[1].n_edges +
+                                     desc.c.projs[2].n_edges + desc.c.projs[3].n_edges),
+                   'sim_ms': args.steps * sim.dt, 'parallelism': 'single-gpu'},
+        'rtf': round(wall / sim_s, 6),
+        'firing_rate_hz': round(rate, 2),
+        'roofline': roofline,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -1,6 +1,6 @@
 // bs_math.h -- device arithmetic whose results are fixed bit-for-bit:
 // a counter-based RNG (Threefry-2x32, 20 rounds; Salmon et al., SC'11), a
-// Knuth product Poisson sampler built only from IEEE multiplies and compares,
+// Poisson sampler by inversion of a host-tabulated CDF (compares only),
 // and exp/log/pow from range reduction plus Taylor / atanh series. Every
 // operation is an exactly rounded IEEE op (the file is compiled with
 // -ffp-contract=off), so a CPU restatement performing the same sequence of
@@ -44,37 +44,30 @@ __device__ __forceinline__ double u01(uint32_t x) {
   return ((double)x + 0.5) * 2.3283064365386963e-10;
 }
 
-// Poisson(lam) count for generator element `elem` at step `step`.
-// lam is split into `pieces` equal parts (pieces = ceil(lam/16)); each part is
-// Knuth's product method against L = exp(-lam/pieces) (precomputed on the
-// host). Uniforms are consumed in order from the stream
-//   threefry(key=(seed, step), ctr=(elem, j)), j = 0,1,2,... two per block.
-// A part stops after 1024 uniforms at most so every call terminates.
-__device__ __forceinline__ uint32_t poisson_knuth(uint32_t seed, uint32_t step, uint32_t elem,
-                                                  double L, int pieces) {
+// Poisson(lam) count for generator element `elem` at step `step`, by
+// inversion of a tabulated CDF (built on the host by bs_poisson_table):
+// lam is split into `parts` equal parts; part p consumes uniform number p of
+// the stream threefry(key=(seed, step), ctr=(elem, p/2)), word p%2, as
+// u = (x + 1/2) 2^-32 in (0,1), and contributes min{k : u <= cdf[k]} (search
+// started from the 64-entry guide table, bounded by the table length).
+// Only compares and IEEE ops on exactly representable inputs are used, so
+// the count is the same on every IEEE machine.
+__device__ __forceinline__ uint32_t poisson_inv(uint32_t seed, uint32_t step, uint32_t elem,
+                                                const double* cdf, const unsigned short* guide,
+                                                int len, int parts) {
   uint32_t count = 0;
-  uint32_t j = 0;
-  uint32_t buf1 = 0;
-  bool have = false;
-  for (int p = 0; p < pieces; ++p) {
-    double prod = 1.0;
-    for (int it = 0; it < 1024; ++it) {
-      uint32_t x;
-      if (have) {
-        x = buf1;
-        have = false;
-      } else {
-        uint32_t a, b;
-        threefry2x32(seed, step, elem, j, a, b);
-        ++j;
-        x = a;
-        buf1 = b;
-        have = true;
-      }
-      prod = prod * u01(x);
-      if (prod <= L) break;
-      ++count;
+  uint32_t x1 = 0;
+  for (int p = 0; p < parts; ++p) {
+    uint32_t x;
+    if ((p & 1) == 0) {
+      threefry2x32(seed, step, elem, (uint32_t)(p >> 1), x, x1);
+    } else {
+      x = x1;
     }
+    const double u = u01(x);
+    int k = guide[(int)(u * 64.0)];
+    while (k < len - 1 && u > cdf[k]) ++k;
+    count += (uint32_t)k;
   }
   return count;
 }

@@ -29,55 +29,97 @@ enum { C_VPEAK = 0, C_VRESET, C_GL, C_CM, C_EEX, C_EIN, C_EL, C_DT_, C_TAUW, C_A
        C_DTSAFE, C_MAXIT, C_GLDT, C_NEGGL };
 // state: 0 V, 1 dg_ex, 2 g_ex, 3 dg_in, 4 g_in, 5 w, 6 h, 7 I_stim
 
+// Parameters are loaded once per launch into a per-model struct (registers),
+// converted to T exactly as the per-step (T)P[i] casts would be.
+
 // ---- iaf_psc_delta ------------------------------------------------------
 // v = E_L + (V-E_L)*decay + (I/C)*tau*(1-decay)  [the last term is the
 // constant D_CI]; v += sum(delta); refractory holds V; spike iff v >= V_th.
 template <typename T>
-__device__ __forceinline__ bool step_delta(const double* __restrict__ P, T* s, int& r, T din) {
-  const T EL = (T)P[D_EL];
+struct DeltaP {
+  T EL, decay, cI, Vth, Vreset, Vmin;
+  int refn, refin, has_vmin;
+  const double* P;  // refractory decay table (refractory_input)
+};
+template <typename T>
+__device__ __forceinline__ DeltaP<T> load_delta(const double* P) {
+  DeltaP<T> p;
+  p.EL = (T)P[D_EL];
+  p.decay = (T)P[D_DECAY];
+  p.cI = (T)P[D_CI];
+  p.Vth = (T)P[D_VTH];
+  p.Vreset = (T)P[D_VRESET];
+  p.Vmin = (T)P[D_VMIN];
+  p.refn = (int)P[D_REFN];
+  p.refin = P[D_REFIN] != 0.0;
+  p.has_vmin = P[D_VMIN] > -1e300;
+  p.P = P;
+  return p;
+}
+template <typename T>
+__device__ __forceinline__ bool step_delta(const DeltaP<T>& p, T* s, int& r, T din) {
   const T last_v = s[0];
-  T v = EL + (last_v - EL) * (T)P[D_DECAY] + (T)P[D_CI];
+  T v = p.EL + (last_v - p.EL) * p.decay + p.cI;
   v = v + din;
-  const bool refin = P[D_REFIN] != 0.0;
-  if (refin) v = v + s[1];
-  if (P[D_VMIN] > -1e300) v = v > (T)P[D_VMIN] ? v : (T)P[D_VMIN];
+  if (p.refin) v = v + s[1];
+  if (p.has_vmin) v = v > p.Vmin ? v : p.Vmin;
   const bool not_ref = (r == 0);
   const T v_post = not_ref ? v : last_v;
-  if (refin) {
+  if (p.refin) {
     // exp(-r dt / tau_m) for the current r, tabulated on the host.
-    const T rdec = (T)P[D_RDEC0 + (r < 23 ? r : 23)];
+    const T rdec = (T)p.P[D_RDEC0 + (r < 23 ? r : 23)];
     s[1] = not_ref ? (T)0 : (T)(s[1] + din * rdec);
   }
   int rr = not_ref ? r : r - 1;
-  const bool spike = v_post >= (T)P[D_VTH];
-  r = spike ? (int)P[D_REFN] : rr;
-  s[0] = spike ? (T)P[D_VRESET] : v_post;
+  const bool spike = v_post >= p.Vth;
+  r = spike ? p.refn : rr;
+  s[0] = spike ? p.Vreset : v_post;
   return spike;
 }
 
 // ---- iaf_psc_exp (delta = 0, deterministic threshold) ---------------------
 template <typename T>
-__device__ __forceinline__ bool step_exp(const double* __restrict__ P, T* s, int& r, T w_all) {
-  const T EL = (T)P[X_EL];
-  T V_rel = s[0] - EL;
+struct ExpP {
+  T EL, theta, Vreset_rel, P22, P21ex, P21in, P20, P11ex, P11in, Ie, one_m_P11ex;
+  int refn;
+};
+template <typename T>
+__device__ __forceinline__ ExpP<T> load_exp(const double* P) {
+  ExpP<T> p;
+  p.EL = (T)P[X_EL];
+  p.theta = (T)P[X_THETA];
+  p.Vreset_rel = (T)P[X_VRESET_REL];
+  p.P22 = (T)P[X_P22];
+  p.P21ex = (T)P[X_P21EX];
+  p.P21in = (T)P[X_P21IN];
+  p.P20 = (T)P[X_P20];
+  p.P11ex = (T)P[X_P11EX];
+  p.P11in = (T)P[X_P11IN];
+  p.Ie = (T)P[X_IE];
+  p.one_m_P11ex = (T)P[X_1MP11EX];
+  p.refn = (int)P[X_REFN];
+  return p;
+}
+template <typename T>
+__device__ __forceinline__ bool step_exp(const ExpP<T>& p, T* s, int& r, T w_all) {
+  T V_rel = s[0] - p.EL;
   T iex = s[1], iin = s[2];
   const T i0 = s[3], i1 = s[4];
   const bool not_ref = (r == 0);
-  const T Vc = V_rel * (T)P[X_P22] + iex * (T)P[X_P21EX] + iin * (T)P[X_P21IN] +
-               ((T)P[X_IE] + i0) * (T)P[X_P20];
+  const T Vc = V_rel * p.P22 + iex * p.P21ex + iin * p.P21in + (p.Ie + i0) * p.P20;
   V_rel = not_ref ? Vc : V_rel;
   int rr = not_ref ? r : r - 1;
-  iex = iex * (T)P[X_P11EX];
-  iin = iin * (T)P[X_P11IN];
-  iex = iex + (T)P[X_1MP11EX] * i1;
+  iex = iex * p.P11ex;
+  iin = iin * p.P11in;
+  iex = iex + p.one_m_P11ex * i1;
   const T wex = w_all > (T)0 ? w_all : (T)0;
   const T win = w_all < (T)0 ? w_all : (T)0;
   iex = iex + wex;
   iin = iin + win;
-  const bool spike = V_rel >= (T)P[X_THETA];
-  r = spike ? (int)P[X_REFN] : rr;
-  V_rel = spike ? (T)P[X_VRESET_REL] : V_rel;
-  s[0] = V_rel + EL;
+  const bool spike = V_rel >= p.theta;
+  r = spike ? p.refn : rr;
+  V_rel = spike ? p.Vreset_rel : V_rel;
+  s[0] = V_rel + p.EL;
   s[1] = iex;
   s[2] = iin;
   s[3] = (T)0;  // next-step latched current: x + sum_current_inputs = 0 on this path
@@ -87,29 +129,60 @@ __device__ __forceinline__ bool step_exp(const double* __restrict__ P, T* s, int
 
 // ---- iaf_psc_alpha -------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ bool step_alpha(const double* __restrict__ P, T* s, int& r, T w_all) {
+struct AlphaP {
+  T EL, theta, Vreset_rel, P11ex, P21ex, P22ex, P11in, P21in, P22in, expm1, P30, P31ex, P32ex,
+      P31in, P32in, epsc, ipsc, Ie, Vmin_rel;
+  int refn, has_vmin;
+};
+template <typename T>
+__device__ __forceinline__ AlphaP<T> load_alpha(const double* P) {
+  AlphaP<T> p;
+  p.EL = (T)P[A_EL];
+  p.theta = (T)P[A_THETA];
+  p.Vreset_rel = (T)P[A_VRESET_REL];
+  p.P11ex = (T)P[A_P11EX];
+  p.P21ex = (T)P[A_P21EX];
+  p.P22ex = (T)P[A_P22EX];
+  p.P11in = (T)P[A_P11IN];
+  p.P21in = (T)P[A_P21IN];
+  p.P22in = (T)P[A_P22IN];
+  p.expm1 = (T)P[A_EXPM1];
+  p.P30 = (T)P[A_P30];
+  p.P31ex = (T)P[A_P31EX];
+  p.P32ex = (T)P[A_P32EX];
+  p.P31in = (T)P[A_P31IN];
+  p.P32in = (T)P[A_P32IN];
+  p.epsc = (T)P[A_EPSC];
+  p.ipsc = (T)P[A_IPSC];
+  p.Ie = (T)P[A_IE];
+  p.Vmin_rel = (T)P[A_VMIN_REL];
+  p.refn = (int)P[A_REFN];
+  p.has_vmin = P[A_VMIN_REL] > -1e300;
+  return p;
+}
+template <typename T>
+__device__ __forceinline__ bool step_alpha(const AlphaP<T>& p, T* s, int& r, T w_all) {
   const T wex = w_all > (T)0 ? w_all : (T)0;
   const T win = w_all < (T)0 ? w_all : (T)0;
-  const T EL = (T)P[A_EL];
-  T y3 = s[0] - EL;
+  T y3 = s[0] - p.EL;
   T Iex = s[1], Iin = s[2], dIex = s[3], dIin = s[4];
   const T y0 = s[5];
   const bool not_ref = (r == 0);
-  T y3c = (T)P[A_P30] * (y0 + (T)P[A_IE]) + (T)P[A_P31EX] * dIex + (T)P[A_P32EX] * Iex +
-          (T)P[A_P31IN] * dIin + (T)P[A_P32IN] * Iin + (T)P[A_EXPM1] * y3 + y3;
-  if (P[A_VMIN_REL] > -1e300) y3c = y3c > (T)P[A_VMIN_REL] ? y3c : (T)P[A_VMIN_REL];
+  T y3c = p.P30 * (y0 + p.Ie) + p.P31ex * dIex + p.P32ex * Iex + p.P31in * dIin + p.P32in * Iin +
+          p.expm1 * y3 + y3;
+  if (p.has_vmin) y3c = y3c > p.Vmin_rel ? y3c : p.Vmin_rel;
   y3 = not_ref ? y3c : y3;
   int rr = not_ref ? r : r - 1;
-  Iex = (T)P[A_P21EX] * dIex + (T)P[A_P22EX] * Iex;
-  dIex = dIex * (T)P[A_P11EX];
-  dIex = dIex + (T)P[A_EPSC] * wex;
-  Iin = (T)P[A_P21IN] * dIin + (T)P[A_P22IN] * Iin;
-  dIin = dIin * (T)P[A_P11IN];
-  dIin = dIin + (T)P[A_IPSC] * win;
-  const bool spike = y3 >= (T)P[A_THETA];
-  r = spike ? (int)P[A_REFN] : rr;
-  y3 = spike ? (T)P[A_VRESET_REL] : y3;
-  s[0] = y3 + EL;
+  Iex = p.P21ex * dIex + p.P22ex * Iex;
+  dIex = dIex * p.P11ex;
+  dIex = dIex + p.epsc * wex;
+  Iin = p.P21in * dIin + p.P22in * Iin;
+  dIin = dIin * p.P11in;
+  dIin = dIin + p.ipsc * win;
+  const bool spike = y3 >= p.theta;
+  r = spike ? p.refn : rr;
+  y3 = spike ? p.Vreset_rel : y3;
+  s[0] = y3 + p.EL;
   s[1] = Iex;
   s[2] = Iin;
   s[3] = dIex;

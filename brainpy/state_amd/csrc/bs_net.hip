@@ -59,8 +59,10 @@ struct DevPop {
   int n_cw;          // packed count words per neuron (two 16-bit counters each)
   int n_fx;          // fixed-point channels
   int n_state;       // state variables used by the model
-  long long cw_base; // first count word of this population inside a ring slot
-  long long fx_base; // first fixed-point word inside a ring64 slot
+  int gp_begin, gp_count;  // incoming static projections (gather list)
+  int tile0;         // first tile of this population
+  int pad;
+  long long tab_bytes;  // bytes of tables staged into LDS by k_window
   double P[BS_NPARAM];
 };
 
@@ -70,31 +72,50 @@ struct DevChan {
   int max_count; // COUNT: table holds S[0..max_count]
   int fx, shift; // FIXED: index and F
   long long table_off;
-  double weight, L;
-  int pieces, delay;
+  long long lds_off;   // offset of the staged table in the LDS table area, -1 if not staged
+  double weight;
+  int parts, delay;    // POISSON: lam split into parts; delay in steps
+  int cdf_len, pad;
   long long tmin, tmax;
-  unsigned seed;
+  unsigned seed, pad2;
   const int* gen_index;
+  const double* cdf;             // POISSON: CDF table of Poisson(lam / parts)
+  const unsigned short* guide;   // POISSON: 64-entry guide table
 };
 
 struct Tile {
   int pop, start, count, pad;
 };
 
-struct UpdArgs {
+// An incoming static projection of a post population. Its edges are grouped
+// by (pre neuron, post tile): seg[i * (nt + 1) + t] .. seg[i * (nt + 1) + t + 1]
+// are the edges from pre neuron i into tile t; an edge word holds the post
+// lane within the tile (bits 0..7) and the delay in steps (bits 8..31).
+struct GProj {
+  int pre_pop, chan, d_min, d_max, nt;
+  int pad;
+  const uint32_t* seg;
+  const uint32_t* edges;
+  const long long* q;  // FIXED channels: 2^F fixed-point weights per edge
+};
+
+struct WinArgs {
   const Tile* tiles;
   const DevPop* pops;
   const DevChan* chans;
   const void* tables;
+  const GProj* gproj;
   void* state;  // [BS_NSTATE][Ntot]
   int* ref;
   long long Ntot;
-  uint32_t* ring;
-  long long CW;
-  long long* ring64;
-  long long FX;
-  int L;
-  int n0, W;
+  uint32_t* sb;        // spike buckets [Ls][Ntot]: local indices of the spikes of (step, pop)
+  uint32_t* sb_count;  // [Ls][n_pops]
+  int Ls, n_pops;
+  int n0, W, Wmax;
+  const long long* base_ptr;  // graph replays: n0 = *base_ptr + widx * Wmax
+  int widx;
+  int max_span;        // longest gathered step range, LDS prefix size
+  int TS;              // neurons per tile
   int2* rec;
   unsigned long long* rec_count;
   long long rec_cap;
@@ -103,173 +124,452 @@ struct UpdArgs {
   int n_vm;
   double* vm_buf;
   long long vm_step0;
-  unsigned char* spk_hist;  // [L][Ntot] last-L-steps spike flags (plastic nets)
 };
 
-// ------------------------------------------------------------ update kernel
-template <typename T>
-__global__ __launch_bounds__(256) void k_update(UpdArgs a) {
-  const Tile tl = a.tiles[blockIdx.x];
-  const DevPop& pop = a.pops[tl.pop];
-  const int model = pop.model;
-  const int j = tl.start + (int)threadIdx.x;
-  const bool active = (int)threadIdx.x < tl.count;
-  const long long gid = (long long)pop.gid0 + j;
-  const int n = pop.n;
-  T* st = (T*)a.state;
-  const T* tables = (const T*)a.tables;
+// ------------------------------------------------------------ window kernel
+// One workgroup (256 threads) per tile of TS <= 256 neurons of one population,
+// one launch per window of W steps:
+//  0. stage: the population's parameters, channel descriptors, count tables
+//     and Poisson CDF tables are copied into LDS;
+//  1. gather: for each incoming static projection, walk the spikes emitted in
+//     the steps whose deliveries can land in this window (read from the spike
+//     buckets), read only the segment of each spike's row that targets this
+//     tile, and add each event whose arrival step s + 1 + d falls in the
+//     window into an LDS counter (integer, order free) or an LDS 2^-F
+//     fixed-point accumulator. Loads are issued in phases (bucket entries,
+//     then segment bounds, then edges) so each phase's loads are independent;
+//  2. update: lane j < TS advances its neuron through the W steps, reading its
+//     inputs from LDS, drawing the Poisson drive, applying the model step;
+//     spikes are buffered in LDS;
+//  3. emit: once per window, each step's spikes are appended to the
+//     (step, population) bucket read by later windows and to the record list.
+struct GInfo {
+  const uint32_t* seg;
+  const uint32_t* edges;
+  const long long* q;
+  int pre_gid0, nt1, kind, cw, half, fx, s_lo, pad;
+};
 
+struct WinLayout {
+  size_t cnt, fxa, spl, spn, base, pref, gtot, gslo, ginfo, P, ch, inp, tab, end;
+};
+
+__host__ __device__ inline size_t al8(size_t x) { return (x + 7) & ~(size_t)7; }
+
+// LDS layout of k_window for one population (tab_bytes: staged tables).
+__host__ __device__ inline WinLayout win_layout(int Wm, int TS, int ncw, int nfx, int ngp, int SP,
+                                                int nch, int nrec, int tsz, size_t tab_bytes) {
+  WinLayout L;
+  L.cnt = 0;
+  L.fxa = al8(L.cnt + (size_t)Wm * ncw * TS * 4);
+  L.spl = al8(L.fxa + (size_t)Wm * nfx * TS * 8);
+  L.spn = al8(L.spl + (size_t)Wm * TS * 2);
+  L.base = al8(L.spn + (size_t)Wm * 4);
+  L.pref = al8(L.base + (size_t)Wm * 16);
+  L.gtot = al8(L.pref + (size_t)ngp * SP * 4);
+  L.gslo = L.gtot + (size_t)(ngp + 1) * 4;
+  L.ginfo = al8(L.gslo + (size_t)ngp * 4);
+  L.P = al8(L.ginfo + (size_t)ngp * sizeof(GInfo));
+  L.ch = L.P + (size_t)BS_NPARAM * 8;
+  L.inp = al8(L.ch + (size_t)nch * sizeof(DevChan));
+  L.tab = al8(L.inp + (size_t)Wm * TS * nrec * tsz);
+  L.end = L.tab + tab_bytes;
+  return L;
+}
+
+// Channel descriptor held in registers by the update loop.
+template <typename T>
+struct ChReg {
+  int kind, receptor, cw, half, max_count, fx, shift, delay, parts, cdf_len;
+  unsigned seed;
+  long long tmin, tmax;
+  T weight;
+  const T* tab;                 // count table S[0..max_count] (LDS or global)
+  const double* cdf;            // Poisson CDF (LDS or global)
+  const unsigned short* guide;  // Poisson guide table
+  const int* gen_index;
+};
+
+template <typename T>
+__device__ __forceinline__ ChReg<T> load_ch(const DevChan& ch, const unsigned char* tab,
+                                            const T* tables) {
+  ChReg<T> c;
+  c.kind = ch.kind;
+  c.receptor = ch.receptor;
+  c.cw = ch.cw;
+  c.half = ch.half;
+  c.max_count = ch.max_count;
+  c.fx = ch.fx;
+  c.shift = ch.shift;
+  c.delay = ch.delay;
+  c.parts = ch.parts;
+  c.cdf_len = ch.cdf_len;
+  c.seed = ch.seed;
+  c.tmin = ch.tmin;
+  c.tmax = ch.tmax;
+  c.weight = (T)ch.weight;
+  c.gen_index = ch.gen_index;
+  const bool staged = ch.lds_off >= 0;
+  c.tab = staged ? (const T*)(tab + ch.lds_off) : tables + ch.table_off;
+  c.cdf = staged ? (const double*)(tab + ch.lds_off) : ch.cdf;
+  c.guide = staged ? (const unsigned short*)((const double*)(tab + ch.lds_off) + ch.cdf_len)
+                   : ch.guide;
+  return c;
+}
+
+template <typename T>
+struct UpdCtx {
+  const WinArgs& a;
+  const uint32_t* cnt;
+  const long long* fxa;
+  unsigned short* spl;
+  int* spn;
+  const DevChan* chs;
+  const unsigned char* tab;
+  const T* tables;
+  int TS, ncw, nfx, nch, start, tid;
+  bool active;
+  long long gid;
+  const T* inp;
+  int nrec, Wm;
+};
+
+// Input of one channel at window step w for lane tid (neuron j).
+template <typename T>
+__device__ __forceinline__ T chan_value(const UpdCtx<T>& u, const ChReg<T>& c, int w, int step,
+                                        int lane) {
+  if (c.kind == BS_CH_COUNT) {
+    uint32_t k = (u.cnt[(w * u.ncw + c.cw) * u.TS + lane] >> (16 * c.half)) & 0xFFFFu;
+    if (k > (uint32_t)c.max_count) {  // cannot happen for consistent edges
+      atomicOr(u.a.flags, FLAG_COUNT);
+      k = (uint32_t)c.max_count;
+    }
+    return c.tab[k];
+  }
+  if (c.kind == BS_CH_POISSON) {
+    const int j = u.start + lane;
+    const int e = c.gen_index ? c.gen_index[j] : j;
+    const long long sgs = (long long)step - 1 - c.delay;
+    uint32_t cn = 0;
+    if (e >= 0 && sgs >= 0 && sgs > c.tmin && sgs <= c.tmax)
+      cn = poisson_inv(c.seed, (uint32_t)sgs, (uint32_t)e, c.cdf, c.guide, c.cdf_len, c.parts);
+    return (T)cn * c.weight;
+  }
+  const long long q = u.fxa[(w * u.nfx + c.fx) * u.TS + lane];  // BS_CH_FIXED
+  return (T)__builtin_ldexp((double)q, -c.shift);
+}
+
+// The W-step loop of one lane: inputs summed in channel order, model step,
+// voltmeter tap, spike buffered in LDS.
+template <typename T, typename F>
+__device__ __forceinline__ void update_loop(const UpdCtx<T>& u, const ChReg<T>* cr, int ns, F stepf) {
+  const WinArgs& a = u.a;
+  T* st = (T*)a.state;
   T s[BS_NSTATE];
   int r = 0;
-  const int ns = pop.n_state;
 #pragma unroll
   for (int k = 0; k < BS_NSTATE; ++k) s[k] = (T)0;
-  int vmc = -1;
-  if (active) {
-    for (int k = 0; k < ns; ++k) s[k] = st[(long long)k * a.Ntot + gid];
-    r = a.ref[gid];
-    if (a.n_vm > 0) vmc = a.vm_col[gid];
-  }
-  const int lane = __lane_id();
-
+  if (!u.active) return;
+  for (int k = 0; k < ns; ++k) s[k] = st[(long long)k * a.Ntot + u.gid];
+  r = a.ref[u.gid];
+  const int vmc = a.n_vm > 0 ? a.vm_col[u.gid] : -1;
   for (int w = 0; w < a.W; ++w) {
     const int step = a.n0 + w;
-    const int slot = step % a.L;
-    T acc0 = (T)0, acc1 = (T)0;
-    if (active) {
-      uint32_t words[8];
-      const long long wbase = (long long)slot * a.CW + pop.cw_base + j;
-      for (int q = 0; q < pop.n_cw; ++q) {
-        words[q] = a.ring[wbase + (long long)q * n];
-        a.ring[wbase + (long long)q * n] = 0u;
-      }
-      for (int c = 0; c < pop.ch_count; ++c) {
-        const DevChan& ch = a.chans[pop.ch_begin + c];
-        T val = (T)0;
-        if (ch.kind == BS_CH_COUNT) {
-          uint32_t k = (words[ch.cw] >> (16 * ch.half)) & 0xFFFFu;
-          if (k > (uint32_t)ch.max_count) {  // cannot happen for a consistent ring
-            atomicOr(a.flags, FLAG_COUNT);
-            k = (uint32_t)ch.max_count;
-          }
-          val = tables[ch.table_off + k];
-        } else if (ch.kind == BS_CH_POISSON) {
-          const int e = ch.gen_index ? ch.gen_index[j] : j;
-          const long long sg = (long long)step - 1 - ch.delay;
-          uint32_t cnt = 0;
-          if (e >= 0 && sg >= 0 && sg > ch.tmin && sg <= ch.tmax)
-            cnt = poisson_knuth(ch.seed, (uint32_t)sg, (uint32_t)e, ch.L, ch.pieces);
-          val = (T)cnt * (T)ch.weight;
-        } else {  // BS_CH_FIXED
-          const long long idx = (long long)slot * a.FX + pop.fx_base + (long long)ch.fx * n + j;
-          const long long q = a.ring64[idx];
-          a.ring64[idx] = 0;
-          val = (T)__builtin_ldexp((double)q, -ch.shift);
-        }
-        if (ch.receptor == 0)
-          acc0 = acc0 + val;
-        else
-          acc1 = acc1 + val;
-      }
-    }
-    bool spike = false;
-    if (active) {
-      if (model == BS_IAF_PSC_DELTA) {
-        spike = step_delta<T>(pop.P, s, r, acc0);
-      } else if (model == BS_IAF_PSC_EXP) {
-        spike = step_exp<T>(pop.P, s, r, acc0);
-      } else if (model == BS_IAF_PSC_ALPHA) {
-        spike = step_alpha<T>(pop.P, s, r, acc0);
-      } else if (model == BS_AEIF_COND_ALPHA) {
-        if constexpr (std::is_same<T, double>::value) {
-          int unstable = 0;
-          spike = step_aeif(pop.P, s, r, acc0, acc1, &unstable);
-          if (unstable) atomicOr(a.flags, FLAG_UNSTABLE);
-        }
-      }
-      if (vmc >= 0) a.vm_buf[(long long)(step - a.vm_step0) * a.n_vm + vmc] = (double)s[0];
-      if (a.spk_hist) a.spk_hist[(long long)slot * a.Ntot + gid] = spike ? 1 : 0;
-    }
-    // wave-aggregated append of (gid, step)
-    const unsigned long long mask = __ballot(spike);
-    if (mask) {
-      const int leader = __ffsll((long long)mask) - 1;
-      unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd(a.rec_count, (unsigned long long)__popcll(mask));
-      base = __shfl(base, leader);
-      if (spike) {
-        const unsigned long long idx = base + __popcll(mask & ((1ull << lane) - 1ull));
-        if ((long long)idx < a.rec_cap)
-          a.rec[idx] = make_int2((int)gid, step);
-        else
-          atomicOr(a.flags, FLAG_OVERFLOW);
-      }
+    const T acc0 = u.inp[w * u.TS + u.tid];
+    const T acc1 = u.nrec > 1 ? u.inp[(u.Wm + w) * u.TS + u.tid] : (T)0;
+    const bool spike = stepf(s, r, acc0, acc1);
+    if (vmc >= 0) a.vm_buf[(long long)(step - a.vm_step0) * a.n_vm + vmc] = (double)s[0];
+    if (spike) {
+      const int p = atomicAdd(&u.spn[w], 1);
+      u.spl[w * u.TS + p] = (unsigned short)u.tid;
     }
   }
-  if (active) {
-    for (int k = 0; k < ns; ++k) st[(long long)k * a.Ntot + gid] = s[k];
-    a.ref[gid] = r;
-  }
+  for (int k = 0; k < ns; ++k) st[(long long)k * a.Ntot + u.gid] = s[k];
+  a.ref[u.gid] = r;
 }
 
-// ---------------------------------------------------------- delivery kernel
-// Edge word: bits 0..23 count word inside a ring slot, bit 24 counter half,
-// bits 25..31 delay in steps.
-__global__ __launch_bounds__(256) void k_deliver(const int2* __restrict__ rec,
-                                                 const unsigned long long* __restrict__ rec_count,
-                                                 long long rec_cap, long long* win_begin,
-                                                 unsigned int* done,
-                                                 const long long* __restrict__ rowptr,
-                                                 const uint32_t* __restrict__ edges,
-                                                 uint32_t* ring, int L, long long CW) {
-  const long long begin = *win_begin;
-  long long end = (long long)*rec_count;
-  if (end > rec_cap) end = rec_cap;
-  for (long long i = begin + blockIdx.x; i < end; i += gridDim.x) {
-    const int2 sp = rec[i];
-    const long long e0 = rowptr[sp.x], e1 = rowptr[sp.x + 1];
-    for (long long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-      const uint32_t wd = edges[e];
-      const uint32_t cw = wd & 0xFFFFFFu;
-      const uint32_t half = (wd >> 24) & 1u;
-      const int d = (int)(wd >> 25);
-      const int slot = (sp.y + 1 + d) % L;
-      atomicAdd(&ring[(long long)slot * CW + cw], 1u << (16u * half));
+template <typename T>
+__global__ __launch_bounds__(256) void k_window(WinArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  if (a.base_ptr) {
+    const long long b = *a.base_ptr;
+    a.n0 = (int)b + a.widx * a.Wmax;
+    a.vm_step0 = b;
+  }
+  const Tile tl = a.tiles[blockIdx.x];
+  const DevPop& gpop = a.pops[tl.pop];
+  const int tid = (int)threadIdx.x;
+  const int TS = a.TS;
+  const int tile_in_pop = (int)blockIdx.x - gpop.tile0;
+  const int W = a.W, Wm = a.Wmax;
+  const int ncw = gpop.n_cw, nfx = gpop.n_fx, ngp = gpop.gp_count, nch = gpop.ch_count;
+  const int SP = a.max_span + 1;
+  const int nrec = gpop.model == BS_AEIF_COND_ALPHA ? 2 : 1;
+  const WinLayout Lo = win_layout(Wm, TS, ncw, nfx, ngp, SP, nch, nrec, (int)sizeof(T), gpop.tab_bytes);
+  uint32_t* cnt = (uint32_t*)(lds + Lo.cnt);
+  long long* fxa = (long long*)(lds + Lo.fxa);
+  unsigned short* spl = (unsigned short*)(lds + Lo.spl);
+  int* spn = (int*)(lds + Lo.spn);
+  unsigned long long* base = (unsigned long long*)(lds + Lo.base);
+  int* pref = (int*)(lds + Lo.pref);
+  int* gtot = (int*)(lds + Lo.gtot);
+  int* gslo = (int*)(lds + Lo.gslo);
+  GInfo* gi = (GInfo*)(lds + Lo.ginfo);
+  double* P = (double*)(lds + Lo.P);
+  DevChan* chs = (DevChan*)(lds + Lo.ch);
+  unsigned char* tab = lds + Lo.tab;
+  T* inp = (T*)(lds + Lo.inp);  // [nrec][Wm][TS] summed inputs per step and lane
+
+  // ---- 0. stage
+  if (blockIdx.x == 0) {  // clear the bucket counts of the next window's steps
+    for (int t = tid; t < Wm * a.n_pops; t += 256) {
+      const int st = a.n0 + W + t / a.n_pops;
+      a.sb_count[(st % a.Ls) * a.n_pops + t % a.n_pops] = 0u;
     }
   }
-  // the last block to finish publishes the next window's first spike index
+  for (int k = tid; k < W * ncw * TS; k += 256) cnt[k] = 0u;
+  for (int k = tid; k < W * nfx * TS; k += 256) fxa[k] = 0;
+  for (int k = tid; k < Wm; k += 256) spn[k] = 0;
+  for (int k = tid; k < BS_NPARAM; k += 256) P[k] = gpop.P[k];
+  {
+    const int words = nch * (int)(sizeof(DevChan) / 8);
+    const unsigned long long* src = (const unsigned long long*)(a.chans + gpop.ch_begin);
+    for (int k = tid; k < words; k += 256) ((unsigned long long*)chs)[k] = src[k];
+  }
+  for (int c = 0; c < nch; ++c) {  // staged count tables and Poisson CDFs
+    const DevChan& ch = a.chans[gpop.ch_begin + c];
+    if (ch.lds_off < 0) continue;
+    if (ch.kind == BS_CH_COUNT) {
+      const T* src = (const T*)a.tables + ch.table_off;
+      T* dst = (T*)(tab + ch.lds_off);
+      for (int k = tid; k <= ch.max_count; k += 256) dst[k] = src[k];
+    } else if (ch.kind == BS_CH_POISSON) {
+      double* dc = (double*)(tab + ch.lds_off);
+      for (int k = tid; k < ch.cdf_len; k += 256) dc[k] = ch.cdf[k];
+      unsigned short* dg = (unsigned short*)(dc + ch.cdf_len);
+      for (int k = tid; k < 64; k += 256) dg[k] = ch.guide[k];
+    }
+  }
+  // bucket sizes of every (projection, step) the gather reads, loaded in parallel
+  for (int k = tid; k < ngp * SP; k += 256) {
+    const int g = k / SP, t = k % SP;
+    const GProj& G = a.gproj[gpop.gp_begin + g];
+    int s_lo = a.n0 - 1 - G.d_max;
+    if (s_lo < 0) s_lo = 0;
+    const int s_hi = a.n0 + W - 2 - G.d_min;
+    const int s = s_lo + t;
+    pref[k] = (t > 0 && s - 1 <= s_hi) ? (int)a.sb_count[((s - 1) % a.Ls) * a.n_pops + G.pre_pop] : 0;
+    if (t == 0) {
+      const DevChan& ch = a.chans[G.chan];
+      GInfo I;
+      I.seg = G.seg;
+      I.edges = G.edges;
+      I.q = G.q;
+      I.pre_gid0 = a.pops[G.pre_pop].gid0;
+      I.nt1 = G.nt + 1;
+      I.kind = ch.kind;
+      I.cw = ch.cw;
+      I.half = ch.half;
+      I.fx = ch.fx;
+      I.s_lo = s_lo;
+      I.pad = 0;
+      gi[g] = I;
+      gslo[g] = s_lo;
+    }
+  }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned int prev = atomicAdd(done, 1u);
-    if (prev == gridDim.x - 1) {
-      *win_begin = end;
-      *done = 0u;
+  if (tid < ngp) {  // per-projection inclusive prefix over steps
+    int acc = 0;
+    for (int t = 0; t < SP; ++t) {
+      acc += pref[tid * SP + t];
+      pref[tid * SP + t] = acc;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    gtot[0] = 0;
+    for (int g = 0; g < ngp; ++g) {
+      acc += pref[g * SP + SP - 1];
+      gtot[g + 1] = acc;
+    }
+  }
+  __syncthreads();
+
+  // ---- 1. gather: up to four spikes per thread per pass, loads in phases
+  const int total = gtot[ngp];
+  for (int b0 = 0; b0 < total; b0 += 1024) {
+    uint32_t si[4], e0[4], e1[4];
+    int ss[4], sg[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int idx = b0 + q * 256 + tid;
+      sg[q] = -1;
+      ss[q] = 0;
+      si[q] = 0;
+      if (idx < total) {
+        int g = 0;
+        while (gtot[g + 1] <= idx) ++g;
+        const int r = idx - gtot[g];
+        const int* pg = pref + g * SP;
+        int lo = 0, hi = SP - 1;  // pg[lo] <= r < pg[lo + 1]
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (pg[mid] <= r) lo = mid; else hi = mid;
+        }
+        const int s = gslo[g] + lo;
+        sg[q] = g;
+        ss[q] = s;
+        si[q] = a.sb[(long long)(s % a.Ls) * a.Ntot + gi[g].pre_gid0 + (r - pg[lo])];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      e0[q] = e1[q] = 0;
+      if (sg[q] >= 0) {
+        const uint32_t* sp = gi[sg[q]].seg + (long long)si[q] * gi[sg[q]].nt1 + tile_in_pop;
+        e0[q] = sp[0];
+        e1[q] = sp[1];
+      }
+    }
+    // edges: rounds of eight per spike, 32 independent loads in flight
+    const uint32_t* ep[4];
+    const long long* qp[4];
+    int dn[4], kc[4], cwq[4], fxq[4];
+    uint32_t incq[4];
+    int rounds = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int g = sg[q] < 0 ? 0 : sg[q];
+      ep[q] = gi[g].edges;
+      qp[q] = gi[g].q;
+      kc[q] = gi[g].kind == BS_CH_COUNT;
+      cwq[q] = gi[g].cw;
+      fxq[q] = gi[g].fx;
+      incq[q] = 1u << (16 * gi[g].half);
+      dn[q] = ss[q] + 1 - a.n0;
+      const int len = (int)(e1[q] - e0[q]);
+      rounds = max(rounds, (len + 7) >> 3);
+    }
+    for (int rd = 0; rd < rounds; ++rd) {
+      uint32_t wd[4][4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const uint32_t e = e0[q] + (uint32_t)(rd * 8 + u);
+          wd[q][u] = e < e1[q] ? ep[q][e] : 0u;
+        }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const uint32_t e = e0[q] + (uint32_t)(rd * 8 + u);
+          if (e < e1[q]) {
+            const int w = dn[q] + (int)(wd[q][u] >> 8);
+            if (w >= 0 && w < W) {
+              const int ln = (int)(wd[q][u] & 255u);
+              if (kc[q])
+                atomicAdd(&cnt[(w * ncw + cwq[q]) * TS + ln], incq[q]);
+              else
+                atomicAdd((unsigned long long*)&fxa[(w * nfx + fxq[q]) * TS + ln],
+                          (unsigned long long)qp[q][e]);
+            }
+          }
+        }
+    }
+  }
+  __syncthreads();
+
+  // ---- 2. inputs of every (step, lane), channels summed in order, all threads
+  {
+    const T* tables = (const T*)a.tables;
+    ChReg<T> cr[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (c < nch) cr[c] = load_ch<T>(chs[c], tab, tables);
+    UpdCtx<T> u{a, cnt, fxa, spl, spn, chs, tab, tables, TS, ncw, nfx, nch, tl.start, tid,
+                tid < tl.count, (long long)gpop.gid0 + tl.start + tid, inp, nrec, Wm};
+    for (int idx = tid; idx < W * TS; idx += 256) {
+      const int w = idx / TS, l = idx - w * TS;
+      if (l >= tl.count) continue;
+      const int step = a.n0 + w;
+      T acc0 = (T)0, acc1 = (T)0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (c < nch) {
+          const T v = chan_value<T>(u, cr[c], w, step, l);
+          if (cr[c].receptor == 0) acc0 = acc0 + v; else acc1 = acc1 + v;
+        }
+      }
+      for (int c = 4; c < nch; ++c) {
+        const ChReg<T> cc = load_ch<T>(chs[c], tab, tables);
+        const T v = chan_value<T>(u, cc, w, step, l);
+        if (cc.receptor == 0) acc0 = acc0 + v; else acc1 = acc1 + v;
+      }
+      inp[w * TS + l] = acc0;
+      if (nrec > 1) inp[(Wm + w) * TS + l] = acc1;
+    }
+  }
+  __syncthreads();
+
+  // ---- 3. update: the W-step loop of each neuron
+  if (tid < TS) {
+    const T* tables = (const T*)a.tables;
+    ChReg<T> cr[4];
+    UpdCtx<T> u{a, cnt, fxa, spl, spn, chs, tab, tables, TS, ncw, nfx, nch, tl.start, tid,
+                tid < tl.count, (long long)gpop.gid0 + tl.start + tid, inp, nrec, Wm};
+    switch (gpop.model) {
+      case BS_IAF_PSC_DELTA: {
+        const DeltaP<T> mp = load_delta<T>(P);
+        update_loop<T>(u, cr, gpop.n_state,
+                       [&](T* s, int& r, T a0, T) { return step_delta<T>(mp, s, r, a0); });
+      } break;
+      case BS_IAF_PSC_EXP: {
+        const ExpP<T> mp = load_exp<T>(P);
+        update_loop<T>(u, cr, gpop.n_state,
+                       [&](T* s, int& r, T a0, T) { return step_exp<T>(mp, s, r, a0); });
+      } break;
+      case BS_IAF_PSC_ALPHA: {
+        const AlphaP<T> mp = load_alpha<T>(P);
+        update_loop<T>(u, cr, gpop.n_state,
+                       [&](T* s, int& r, T a0, T) { return step_alpha<T>(mp, s, r, a0); });
+      } break;
+      case BS_AEIF_COND_ALPHA: {
+        if constexpr (std::is_same<T, double>::value) {
+          update_loop<T>(u, cr, gpop.n_state, [&](T* s, int& r, T a0, T a1) {
+            int unstable = 0;
+            const bool sp = step_aeif(P, s, r, a0, a1, &unstable);
+            if (unstable) atomicOr(a.flags, FLAG_UNSTABLE);
+            return sp;
+          });
+        }
+      } break;
+    }
+  }
+  __syncthreads();
+
+  // ---- 4. emit: one bucket and one record reservation per step
+  if (tid < W && spn[tid] > 0) {
+    const int step = a.n0 + tid;
+    base[2 * tid] = atomicAdd(&a.sb_count[(step % a.Ls) * a.n_pops + tl.pop], (unsigned)spn[tid]);
+    base[2 * tid + 1] = atomicAdd(a.rec_count, (unsigned long long)spn[tid]);
+  }
+  __syncthreads();
+  for (int w = 0; w < W; ++w) {
+    const int m = spn[w];
+    for (int k = tid; k < m; k += 256) {
+      const int jj = tl.start + (int)spl[w * TS + k];
+      const int step = a.n0 + w;
+      a.sb[(long long)(step % a.Ls) * a.Ntot + gpop.gid0 + (long long)base[2 * w] + k] = (uint32_t)jj;
+      const unsigned long long idx = base[2 * w + 1] + (unsigned long long)k;
+      if ((long long)idx < a.rec_cap)
+        a.rec[idx] = make_int2(gpop.gid0 + jj, step);
+      else
+        atomicOr(a.flags, FLAG_OVERFLOW);
     }
   }
 }
-
-// Heterogeneous static weights: the same walk, adding the exact fixed-point
-// image q_e = rint(w_e * 2^F) into 64-bit ring words.
-__global__ __launch_bounds__(256) void k_deliver_fixed(
-    const int2* __restrict__ rec, const unsigned long long* __restrict__ rec_count,
-    long long rec_cap, const long long* win_begin, const long long* __restrict__ rowptr,
-    const uint32_t* __restrict__ tgt, const int* __restrict__ dly, const long long* __restrict__ q,
-    long long* ring64, int L, long long FX) {
-  const long long begin = *win_begin;
-  long long end = (long long)*rec_count;
-  if (end > rec_cap) end = rec_cap;
-  for (long long i = begin + blockIdx.x; i < end; i += gridDim.x) {
-    const int2 sp = rec[i];
-    const long long e0 = rowptr[sp.x], e1 = rowptr[sp.x + 1];
-    for (long long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-      const int slot = (sp.y + 1 + dly[e]) % L;
-      atomicAdd((unsigned long long*)&ring64[(long long)slot * FX + tgt[e]],
-                (unsigned long long)q[e]);
-    }
-  }
-}
-
 
 // ============================================================== host side
 struct bs_net {
@@ -279,8 +579,9 @@ struct bs_net {
   hipStream_t stream = nullptr;
   int device = 0;
   // layout
-  long long Ntot = 0, CW = 0, FX = 0;
-  int L = 1, W = 1, d_min = 0, d_max = 0;
+  long long Ntot = 0;
+  int W = 1, d_min = 0, d_max = 0, Ls = 1, max_span = 1, TS = 256;
+  size_t lds_bytes = 0;
   std::vector<DevPop> pops;
   std::vector<DevChan> chans;
   std::vector<Tile> tiles;
@@ -289,35 +590,30 @@ struct bs_net {
   Tile* d_tiles = nullptr;
   DevPop* d_pops = nullptr;
   DevChan* d_chans = nullptr;
+  GProj* d_gproj = nullptr;
   void* d_tables = nullptr;
   void* d_state = nullptr;
   int* d_ref = nullptr;
-  uint32_t* d_ring = nullptr;
-  long long* d_ring64 = nullptr;
+  uint32_t* d_sb = nullptr;
+  uint32_t* d_sb_count = nullptr;
   int2* d_rec = nullptr;
   unsigned long long* d_rec_count = nullptr;
   long long rec_cap = 0;
-  long long* d_win_begin = nullptr;
-  unsigned int* d_done = nullptr;
   int* d_flags = nullptr;
   int* d_vm_col = nullptr;
   int n_vm_dev = 0;
   double* d_vm = nullptr;
   long long vm_cap_steps = 0;
-  long long* d_rowptr = nullptr;
-  uint32_t* d_edges = nullptr;
-  long long n_edges = 0;
-  long long* d_frowptr = nullptr;
-  uint32_t* d_ftgt = nullptr;
-  int* d_fdly = nullptr;
-  long long* d_fq = nullptr;
-  long long n_fedges = 0;
-  std::vector<int*> d_gen;
+  long long* d_base = nullptr;     // first step of the chunk a graph replay advances
+  long long h_base = 0;
+  hipGraphExec_t graph = nullptr;  // one full chunk of window launches
+  std::vector<void*> owned;  // per-projection gather arrays, generator maps
   // host copies
   std::vector<double> state0;  // [BS_NSTATE][Ntot]
   std::vector<int> ref0;
-  std::vector<long long> rowptr_h, frowptr_h;
-  std::vector<int> vm_tap_col;  // per tap (in caller order) -> device column
+  std::vector<long long> outdeg;  // static out-degree per gid (event accounting)
+  std::vector<int> gid_pop;       // population of each gid
+  std::vector<int> vm_tap_col;    // per tap (in caller order) -> device column
   long long step = 0;
   std::vector<int32_t> out_gid, out_step;
   // timing / accounting of the last run
@@ -342,12 +638,12 @@ static void free_net(bs_net* net) {
   auto F = [](void* p) {
     if (p) (void)hipFree(p);
   };
-  F(net->d_tiles); F(net->d_pops); F(net->d_chans); F(net->d_tables); F(net->d_state);
-  F(net->d_ref); F(net->d_ring); F(net->d_ring64); F(net->d_rec); F(net->d_rec_count);
-  F(net->d_win_begin); F(net->d_done); F(net->d_flags); F(net->d_vm_col); F(net->d_vm);
-  F(net->d_rowptr); F(net->d_edges); F(net->d_frowptr); F(net->d_ftgt); F(net->d_fdly);
-  F(net->d_fq);
-  for (int* p : net->d_gen) F(p);
+  F(net->d_tiles); F(net->d_pops); F(net->d_chans); F(net->d_gproj); F(net->d_tables);
+  F(net->d_state); F(net->d_ref); F(net->d_sb); F(net->d_sb_count); F(net->d_rec);
+  F(net->d_rec_count); F(net->d_flags); F(net->d_vm_col); F(net->d_vm);
+  for (void* p : net->owned) F(p);
+  F(net->d_base);
+  if (net->graph) (void)hipGraphExecDestroy(net->graph);
   if (net->stream) (void)hipStreamDestroy(net->stream);
   delete net;
 }
@@ -370,12 +666,21 @@ static int model_nstate(int model, const double* P) {
 template <typename X>
 static int dmalloc(X** p, size_t count) {
   *p = nullptr;
-  if (count == 0) return BS_OK;
+  if (count == 0) count = 1;
   hipError_t e = hipMalloc((void**)p, count * sizeof(X));
   if (e != hipSuccess) {
     *p = nullptr;
     return fail(BS_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
   }
+  return BS_OK;
+}
+
+template <typename X>
+static int upload(bs_net* net, X** d, const X* h, size_t n) {
+  int rc = dmalloc(d, n);
+  if (rc) return rc;
+  net->owned.push_back((void*)*d);
+  if (n) HIPCHK(hipMemcpy(*d, h, sizeof(X) * n, hipMemcpyHostToDevice));
   return BS_OK;
 }
 
@@ -402,6 +707,33 @@ static int dmalloc(X** p, size_t count) {
   } while (0)
 
 static int upload_state(bs_net* net);
+static const size_t kLdsBudget = 64 * 1024;
+
+// CDF table of Poisson(mu), mu = lam / parts, parts = max(1, ceil(lam / 32)):
+// pmf_0 = exp(-mu), pmf_k = pmf_{k-1} * mu / k, cdf_k = cdf_{k-1} + pmf_k,
+// until k > mu and 1 - cdf_k < 1e-15 (at most 512 entries); guide[g] is the
+// first k with cdf_k >= g / 64. poisson_inv samples by inversion of it.
+static void poisson_table(double lam, std::vector<double>& cdf, std::vector<unsigned short>& guide,
+                          int& parts) {
+  parts = std::max(1, (int)std::ceil(lam / 32.0));
+  const double mu = lam / (double)parts;
+  double pmf = std::exp(-mu);
+  cdf.assign(1, pmf);
+  for (int k = 1; k < 512; ++k) {
+    pmf = pmf * mu / (double)k;
+    const double c = cdf.back() + pmf;
+    cdf.push_back(c);
+    if ((double)k > mu && 1.0 - c < 1e-15) break;
+  }
+  const int len = (int)cdf.size();
+  guide.assign(64, 0);
+  for (int g = 0; g < 64; ++g) {
+    const double t = (double)g / 64.0;
+    int k = 0;
+    while (k < len - 1 && cdf[k] < t) ++k;
+    guide[g] = (unsigned short)k;
+  }
+}
 
 int bs_net_create(const bs_net_desc* d, bs_net** out) {
   if (!d || !out) return fail(BS_EINVAL, "bs_net_create: null argument");
@@ -423,8 +755,21 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   HCHK(hipGetDevice(&net->device));
   HCHK(hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking));
 
-  // ---- populations and channels
-  long long gid = 0, cw = 0, fx = 0;
+  // ---- tile size: the largest of 256/128/64 neurons that still gives >= 1024
+  // tiles (four per CU), so small networks spread over the whole GPU
+  int TS = 256;
+  {
+    auto ntiles = [&](int ts) {
+      long long t = 0;
+      for (int p = 0; p < d->n_pops; ++p) t += (d->pops[p].n + ts - 1) / ts;
+      return t;
+    };
+    while (TS > 64 && ntiles(TS) < 1024) TS /= 2;
+  }
+  net->TS = TS;
+
+  // ---- populations, channels, tiles
+  long long gid = 0;
   net->pops.resize(d->n_pops);
   net->chans.resize(d->n_channels);
   for (int c = 0; c < d->n_channels; ++c) memset(&net->chans[c], 0, sizeof(DevChan));
@@ -451,6 +796,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     for (int c = sp.ch_begin; c < sp.ch_begin + sp.ch_count; ++c) {
       const bs_channel& ch = d->channels[c];
       DevChan& dc = net->chans[c];
+      if (chan_pop[c] >= 0) BAD("a channel belongs to two populations");
       chan_pop[c] = p;
       dc.kind = ch.kind;
       dc.receptor = ch.receptor;
@@ -475,10 +821,16 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
         if (ch.fixed_shift < 0 || ch.fixed_shift > 1000) BAD("bad fixed_shift");
       } else if (ch.kind == BS_CH_POISSON) {
         if (!(ch.lam >= 0)) BAD("Poisson lambda must be >= 0");
-        int pieces = (int)std::ceil(ch.lam / 16.0);
-        if (pieces < 1) pieces = 1;
-        dc.pieces = pieces;
-        dc.L = std::exp(-(ch.lam / (double)pieces));
+        std::vector<double> cdf;
+        std::vector<unsigned short> guide;
+        poisson_table(ch.lam, cdf, guide, dc.parts);
+        dc.cdf_len = (int)cdf.size();
+        double* dcdf = nullptr;
+        unsigned short* dgd = nullptr;
+        CHK(upload(net, &dcdf, cdf.data(), cdf.size()));
+        CHK(upload(net, &dgd, guide.data(), guide.size()));
+        dc.cdf = dcdf;
+        dc.guide = dgd;
         dc.tmin = ch.t_min_step;
         dc.tmax = ch.t_max_step;
         dc.seed = ch.seed;
@@ -487,9 +839,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
         if (ch.lam == 0.0) dc.tmax = -1;  // rate <= 0: all-zero output
         if (ch.gen_index) {
           int* g = nullptr;
-          CHK(dmalloc(&g, (size_t)sp.n));
-          net->d_gen.push_back(g);
-          HCHK(hipMemcpy(g, ch.gen_index, sizeof(int) * sp.n, hipMemcpyHostToDevice));
+          CHK(upload(net, &g, ch.gen_index, (size_t)sp.n));
           dc.gen_index = g;
         }
       } else {
@@ -497,102 +847,164 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
       }
     }
     dp.n_cw = (ncount + 1) / 2;
-    if (dp.n_cw > 8) BAD("at most 16 COUNT channels per population");
     dp.n_fx = nfx;
-    dp.cw_base = cw;
-    dp.fx_base = fx;
-    cw += (long long)dp.n_cw * sp.n;
-    fx += (long long)nfx * sp.n;
+    dp.tile0 = (int)net->tiles.size();
+    for (int s0 = 0; s0 < sp.n; s0 += TS)
+      net->tiles.push_back(Tile{p, s0, std::min(TS, sp.n - s0), 0});
     gid += sp.n;
     net->pop_record.push_back(sp.record_spikes);
   }
   net->Ntot = gid;
-  net->CW = cw;
-  net->FX = fx;
   if (gid >= (1ll << 31)) BAD("more than 2^31 neurons");
-  if (cw >= (1ll << 24)) BAD("count ring words exceed 2^24 (too many neurons x channels)");
+  net->gid_pop.assign(gid, 0);
+  for (int p = 0; p < d->n_pops; ++p)
+    for (int i = 0; i < net->pops[p].n; ++i) net->gid_pop[net->pops[p].gid0 + i] = p;
 
-  // ---- projections -> CSR rows by pre gid
-  std::vector<long long> cnt(gid + 1, 0), fcnt(gid + 1, 0);
+  // ---- static projections -> per-(pre, post tile) gather segments
+  net->outdeg.assign(gid, 0);
+  std::vector<GProj> gp_h;
+  std::vector<std::vector<int>> pop_gp(d->n_pops);
+  std::vector<int> chan_proj(d->n_channels, -1);
   int dmin = 1 << 30, dmax = 0;
-  bool have_plastic = false;
   for (int q = 0; q < d->n_projs; ++q) {
     const bs_projection& pj = d->projs[q];
     if (pj.pre_pop < 0 || pj.pre_pop >= d->n_pops || pj.post_pop < 0 || pj.post_pop >= d->n_pops)
       BAD("projection population index out of range");
     if (pj.channel < 0 || pj.channel >= d->n_channels || chan_pop[pj.channel] != pj.post_pop)
       BAD("projection channel does not belong to the post population");
-    if (pj.plastic) have_plastic = true;
+    if (chan_proj[pj.channel] >= 0) BAD("two projections share one channel");
+    chan_proj[pj.channel] = q;
     if (pj.plastic) BAD("plastic projections are not supported by this engine build");
     const int kind = d->channels[pj.channel].kind;
     if (kind != BS_CH_COUNT && kind != BS_CH_FIXED) BAD("projection channel must be COUNT or FIXED");
     if (kind == BS_CH_FIXED && !pj.weight) BAD("FIXED projection needs per-edge weights");
+    if (pj.n_edges < 0 || pj.n_edges >= (1ll << 32)) BAD("projection edge count out of range");
     const int npre = d->pops[pj.pre_pop].n, npost = d->pops[pj.post_pop].n;
+    const DevPop& post = net->pops[pj.post_pop];
+    const int nt = (npost + TS - 1) / TS;
+    int pdmin = 1 << 30, pdmax = 0;
+    std::vector<uint32_t> cnt((size_t)npre * (nt + 1) + 1, 0);
     for (long long e = 0; e < pj.n_edges; ++e) {
       const int a = pj.pre[e], b = pj.post[e];
       if (a < 0 || a >= npre || b < 0 || b >= npost) BAD("edge index out of range");
       const int dd = pj.delay ? pj.delay[e] : pj.delay_steps;
-      if (dd < 0) BAD("delay must be >= 0 steps");
-      if (kind == BS_CH_COUNT && dd > 127) BAD("delay above 127 steps is not supported");
-      dmin = std::min(dmin, dd);
-      dmax = std::max(dmax, dd);
-      const long long g = net->pops[pj.pre_pop].gid0 + a;
-      if (kind == BS_CH_COUNT)
-        cnt[g + 1]++;
-      else
-        fcnt[g + 1]++;
+      if (dd < 0 || dd >= (1 << 24)) BAD("delay must be in [0, 2^24) steps");
+      pdmin = std::min(pdmin, dd);
+      pdmax = std::max(pdmax, dd);
+      cnt[(size_t)a * (nt + 1) + b / TS + 1]++;
+      net->outdeg[net->pops[pj.pre_pop].gid0 + a]++;
     }
-  }
-  (void)have_plastic;
-  if (dmin > dmax) dmin = dmax = 0;
-  for (long long g = 0; g < gid; ++g) {
-    cnt[g + 1] += cnt[g];
-    fcnt[g + 1] += fcnt[g];
-  }
-  net->n_edges = cnt[gid];
-  net->n_fedges = fcnt[gid];
-  std::vector<uint32_t> edges(net->n_edges);
-  std::vector<uint32_t> ftgt(net->n_fedges);
-  std::vector<int> fdly(net->n_fedges);
-  std::vector<long long> fq(net->n_fedges);
-  {
-    std::vector<long long> pos(cnt.begin(), cnt.end() - 1), fpos(fcnt.begin(), fcnt.end() - 1);
-    for (int q = 0; q < d->n_projs; ++q) {
-      const bs_projection& pj = d->projs[q];
-      const bs_channel& ch = d->channels[pj.channel];
-      const DevPop& post = net->pops[pj.post_pop];
-      const DevChan& dc = net->chans[pj.channel];
-      for (long long e = 0; e < pj.n_edges; ++e) {
-        const long long g = net->pops[pj.pre_pop].gid0 + pj.pre[e];
-        const int dd = pj.delay ? pj.delay[e] : pj.delay_steps;
-        if (ch.kind == BS_CH_COUNT) {
-          const long long w = post.cw_base + (long long)dc.cw * post.n + pj.post[e];
-          edges[pos[g]++] = (uint32_t)w | ((uint32_t)dc.half << 24) | ((uint32_t)dd << 25);
-        } else {
-          const long long f = post.fx_base + (long long)dc.fx * post.n + pj.post[e];
-          const long long k = fpos[g]++;
-          ftgt[k] = (uint32_t)f;
-          fdly[k] = dd;
-          fq[k] = (long long)std::llrint(std::ldexp(pj.weight[e], ch.fixed_shift));
+    if (pj.n_edges == 0) pdmin = pdmax = 0;
+    // exclusive prefix over (pre, tile) in row-major order; entry (a, nt) is the row end
+    std::vector<uint32_t> seg((size_t)npre * (nt + 1), 0);
+    {
+      uint32_t acc = 0;
+      for (int a = 0; a < npre; ++a) {
+        for (int t = 0; t < nt; ++t) {
+          seg[(size_t)a * (nt + 1) + t] = acc;
+          acc += cnt[(size_t)a * (nt + 1) + t + 1];
         }
+        seg[(size_t)a * (nt + 1) + nt] = acc;
       }
     }
+    std::vector<uint32_t> pos(seg);
+    std::vector<uint32_t> ew(pj.n_edges);
+    std::vector<long long> eq(kind == BS_CH_FIXED ? pj.n_edges : 0);
+    const int F = d->channels[pj.channel].fixed_shift;
+    for (long long e = 0; e < pj.n_edges; ++e) {
+      const int a = pj.pre[e], b = pj.post[e];
+      const int dd = pj.delay ? pj.delay[e] : pj.delay_steps;
+      const uint32_t k = pos[(size_t)a * (nt + 1) + b / TS]++;
+      ew[k] = (uint32_t)(b % TS) | ((uint32_t)dd << 8);
+      if (kind == BS_CH_FIXED) eq[k] = (long long)std::llrint(std::ldexp(pj.weight[e], F));
+    }
+    GProj G;
+    memset(&G, 0, sizeof(G));
+    G.pre_pop = pj.pre_pop;
+    G.chan = pj.channel;
+    G.d_min = pdmin;
+    G.d_max = pdmax;
+    G.nt = nt;
+    uint32_t *dseg = nullptr, *dew = nullptr;
+    long long* deq = nullptr;
+    CHK(upload(net, &dseg, seg.data(), seg.size()));
+    CHK(upload(net, &dew, ew.data(), ew.size()));
+    if (kind == BS_CH_FIXED) CHK(upload(net, &deq, eq.data(), eq.size()));
+    G.seg = dseg;
+    G.edges = dew;
+    G.q = deq;
+    pop_gp[pj.post_pop].push_back((int)gp_h.size());
+    gp_h.push_back(G);
+    if (pj.n_edges) {
+      dmin = std::min(dmin, pdmin);
+      dmax = std::max(dmax, pdmax);
+    }
+    (void)post;
+  }
+  if (dmin > dmax) dmin = dmax = 0;
+  for (int c = 0; c < d->n_channels; ++c) {
+    const int k = d->channels[c].kind;
+    if ((k == BS_CH_COUNT || k == BS_CH_FIXED) && chan_proj[c] < 0)
+      BAD("COUNT/FIXED channel without a projection");
+  }
+  // gather list in post-population order
+  std::vector<GProj> gp_sorted;
+  for (int p = 0; p < d->n_pops; ++p) {
+    net->pops[p].gp_begin = (int)gp_sorted.size();
+    net->pops[p].gp_count = (int)pop_gp[p].size();
+    for (int k : pop_gp[p]) gp_sorted.push_back(gp_h[k]);
   }
   net->d_min = dmin;
   net->d_max = dmax;
+
+  // ---- window length: W = d_min + 1 (holder lag) capped by the LDS budget
   int W = dmin + 1;
   const int maxw = d->max_window > 0 ? d->max_window : 32;
   if (W > maxw) W = maxw;
   if (W < 1) W = 1;
+  // LDS layout of k_window (win_layout), first without staged tables
+  auto span_of = [&](int w) {
+    int span = 1;
+    for (const GProj& G : gp_sorted) span = std::max(span, G.d_max - G.d_min + w);
+    return span;
+  };
+  auto lds_of = [&](int w) {
+    size_t best = 0;
+    const int SP = span_of(w) + 1;
+    for (const DevPop& p : net->pops)
+      best = std::max(best, win_layout(w, TS, p.n_cw, p.n_fx, p.gp_count, SP, p.ch_count,
+                                       p.model == BS_AEIF_COND_ALPHA ? 2 : 1, (int)net->tsize, 0).end);
+    return best;
+  };
+  while (W > 1 && lds_of(W) > kLdsBudget) --W;
+  if (lds_of(W) > kLdsBudget) BAD("too many input channels per population for the LDS budget");
   net->W = W;
-  net->L = dmax + W + 1;
-  net->rowptr_h = cnt;
-  net->frowptr_h = fcnt;
-
-  // ---- tiles (<=256 neurons of one population each)
-  for (int p = 0; p < d->n_pops; ++p)
-    for (int s0 = 0; s0 < net->pops[p].n; s0 += 256)
-      net->tiles.push_back(Tile{p, s0, std::min(256, net->pops[p].n - s0), 0});
+  net->max_span = span_of(W);
+  // stage count tables and Poisson CDFs into LDS while they fit
+  {
+    const int SP = net->max_span + 1;
+    size_t need = 0;
+    for (DevPop& p : net->pops) {
+      const size_t base_end = win_layout(W, TS, p.n_cw, p.n_fx, p.gp_count, SP, p.ch_count,
+                                         p.model == BS_AEIF_COND_ALPHA ? 2 : 1, (int)net->tsize, 0).end;
+      size_t off = 0;
+      for (int c = p.ch_begin; c < p.ch_begin + p.ch_count; ++c) {
+        DevChan& dc = net->chans[c];
+        dc.lds_off = -1;
+        size_t sz = 0;
+        if (dc.kind == BS_CH_COUNT) sz = al8((size_t)(dc.max_count + 1) * net->tsize);
+        else if (dc.kind == BS_CH_POISSON) sz = al8((size_t)dc.cdf_len * 8 + 64 * 2);
+        if (sz && base_end + off + sz <= kLdsBudget) {
+          dc.lds_off = (long long)off;
+          off += sz;
+        }
+      }
+      p.tab_bytes = (long long)off;
+      need = std::max(need, base_end + off);
+    }
+    net->lds_bytes = need;
+  }
+  net->Ls = dmax + 2 * W + 2;
 
   // ---- voltmeter columns
   std::vector<int> vm_col(gid, -1);
@@ -623,29 +1035,22 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   const size_t ts = net->tsize;
   CHK(dmalloc(&net->d_tiles, net->tiles.size()));
   CHK(dmalloc(&net->d_pops, net->pops.size()));
-  CHK(dmalloc(&net->d_chans, std::max<size_t>(1, net->chans.size())));
+  CHK(dmalloc(&net->d_chans, net->chans.size()));
+  CHK(dmalloc(&net->d_gproj, gp_sorted.size()));
   CHK(dmalloc((char**)&net->d_tables, std::max<long long>(1, d->n_tables) * ts));
   CHK(dmalloc((char**)&net->d_state, (size_t)BS_NSTATE * gid * ts));
   CHK(dmalloc(&net->d_ref, gid));
-  CHK(dmalloc(&net->d_ring, (size_t)net->L * std::max<long long>(1, cw)));
-  CHK(dmalloc(&net->d_ring64, (size_t)net->L * std::max<long long>(1, fx)));
-  CHK(dmalloc(&net->d_rowptr, gid + 1));
-  CHK(dmalloc(&net->d_edges, std::max<long long>(1, net->n_edges)));
-  CHK(dmalloc(&net->d_frowptr, gid + 1));
-  CHK(dmalloc(&net->d_ftgt, std::max<long long>(1, net->n_fedges)));
-  CHK(dmalloc(&net->d_fdly, std::max<long long>(1, net->n_fedges)));
-  CHK(dmalloc(&net->d_fq, std::max<long long>(1, net->n_fedges)));
+  CHK(dmalloc(&net->d_sb, (size_t)net->Ls * gid));
+  CHK(dmalloc(&net->d_sb_count, (size_t)net->Ls * d->n_pops));
   CHK(dmalloc(&net->d_rec_count, 1));
-  CHK(dmalloc(&net->d_win_begin, 1));
-  CHK(dmalloc(&net->d_done, 1));
   CHK(dmalloc(&net->d_flags, 1));
-  CHK(dmalloc(&net->d_vm_col, std::max<long long>(1, gid)));
-  // spike list: worst case every neuron fires every step of K windows
-  {
+  CHK(dmalloc(&net->d_base, 1));
+  CHK(dmalloc(&net->d_vm_col, gid));
+  {  // spike record list: worst case every neuron fires every step of K windows
     const long long per_win = std::max<long long>(1, gid) * W;
     long long k = (64ll << 20) / per_win;
     if (k < 1) k = 1;
-    if (k > 256) k = 256;
+    if (k > 512) k = 512;
     net->rec_cap = per_win * k;
     CHK(dmalloc(&net->d_rec, net->rec_cap));
     net->vm_cap_steps = k * W;
@@ -655,6 +1060,8 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   HCHK(hipMemcpy(net->d_pops, net->pops.data(), sizeof(DevPop) * net->pops.size(), hipMemcpyHostToDevice));
   if (!net->chans.empty())
     HCHK(hipMemcpy(net->d_chans, net->chans.data(), sizeof(DevChan) * net->chans.size(), hipMemcpyHostToDevice));
+  if (!gp_sorted.empty())
+    HCHK(hipMemcpy(net->d_gproj, gp_sorted.data(), sizeof(GProj) * gp_sorted.size(), hipMemcpyHostToDevice));
   if (d->n_tables > 0) {
     if (ts == 8) {
       HCHK(hipMemcpy(net->d_tables, d->tables, 8 * d->n_tables, hipMemcpyHostToDevice));
@@ -664,16 +1071,11 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
       HCHK(hipMemcpy(net->d_tables, tf.data(), 4 * d->n_tables, hipMemcpyHostToDevice));
     }
   }
-  HCHK(hipMemcpy(net->d_rowptr, cnt.data(), sizeof(long long) * (gid + 1), hipMemcpyHostToDevice));
-  if (net->n_edges)
-    HCHK(hipMemcpy(net->d_edges, edges.data(), 4 * net->n_edges, hipMemcpyHostToDevice));
-  HCHK(hipMemcpy(net->d_frowptr, fcnt.data(), sizeof(long long) * (gid + 1), hipMemcpyHostToDevice));
-  if (net->n_fedges) {
-    HCHK(hipMemcpy(net->d_ftgt, ftgt.data(), 4 * net->n_fedges, hipMemcpyHostToDevice));
-    HCHK(hipMemcpy(net->d_fdly, fdly.data(), 4 * net->n_fedges, hipMemcpyHostToDevice));
-    HCHK(hipMemcpy(net->d_fq, fq.data(), 8 * net->n_fedges, hipMemcpyHostToDevice));
-  }
   if (gid) HCHK(hipMemcpy(net->d_vm_col, vm_col.data(), 4 * gid, hipMemcpyHostToDevice));
+  HCHK(hipFuncSetAttribute((const void*)k_window<double>,
+                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget));
+  HCHK(hipFuncSetAttribute((const void*)k_window<float>,
+                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget));
   {
     int rc = upload_state(net);
     if (rc != BS_OK) {
@@ -695,9 +1097,7 @@ static int upload_state(bs_net* net) {
     HIPCHK(hipMemcpy(net->d_state, f.data(), 4 * f.size(), hipMemcpyHostToDevice));
   }
   if (gid) HIPCHK(hipMemcpy(net->d_ref, net->ref0.data(), 4 * gid, hipMemcpyHostToDevice));
-  HIPCHK(hipMemset(net->d_ring, 0, 4 * (size_t)net->L * std::max<long long>(1, net->CW)));
-  HIPCHK(hipMemset(net->d_ring64, 0, 8 * (size_t)net->L * std::max<long long>(1, net->FX)));
-  HIPCHK(hipMemset(net->d_done, 0, sizeof(unsigned int)));
+  HIPCHK(hipMemset(net->d_sb_count, 0, 4 * (size_t)net->Ls * net->pops.size()));
   HIPCHK(hipMemset(net->d_flags, 0, sizeof(int)));
   HIPCHK(hipDeviceSynchronize());
   net->step = 0;
@@ -718,22 +1118,28 @@ int bs_net_step(bs_net* net, int64_t* step) {
 }
 
 template <typename T>
-static void launch_update(bs_net* net, int n0, int W, long long vm_step0) {
-  UpdArgs a;
+static void launch_window(bs_net* net, int n0, int W, long long vm_step0,
+                          const long long* base_ptr = nullptr, int widx = 0) {
+  WinArgs a;
+  a.base_ptr = base_ptr;
+  a.widx = widx;
   a.tiles = net->d_tiles;
   a.pops = net->d_pops;
   a.chans = net->d_chans;
   a.tables = net->d_tables;
+  a.gproj = net->d_gproj;
   a.state = net->d_state;
   a.ref = net->d_ref;
   a.Ntot = net->Ntot;
-  a.ring = net->d_ring;
-  a.CW = net->CW;
-  a.ring64 = net->d_ring64;
-  a.FX = net->FX;
-  a.L = net->L;
+  a.sb = net->d_sb;
+  a.sb_count = net->d_sb_count;
+  a.Ls = net->Ls;
+  a.n_pops = (int)net->pops.size();
   a.n0 = n0;
   a.W = W;
+  a.Wmax = net->W;
+  a.max_span = net->max_span;
+  a.TS = net->TS;
   a.rec = net->d_rec;
   a.rec_count = net->d_rec_count;
   a.rec_cap = net->rec_cap;
@@ -742,20 +1148,8 @@ static void launch_update(bs_net* net, int n0, int W, long long vm_step0) {
   a.n_vm = net->n_vm_dev;
   a.vm_buf = net->d_vm;
   a.vm_step0 = vm_step0;
-  a.spk_hist = nullptr;
-  hipLaunchKernelGGL(k_update<T>, dim3((unsigned)net->tiles.size()), dim3(256), 0, net->stream, a);
-}
-
-static const unsigned kDeliverBlocks = 1024;
-
-static void launch_deliver(bs_net* net) {
-  if (net->n_fedges)
-    hipLaunchKernelGGL(k_deliver_fixed, dim3(kDeliverBlocks), dim3(256), 0, net->stream, net->d_rec,
-                       net->d_rec_count, net->rec_cap, net->d_win_begin, net->d_frowptr, net->d_ftgt,
-                       net->d_fdly, net->d_fq, net->d_ring64, net->L, net->FX);
-  hipLaunchKernelGGL(k_deliver, dim3(kDeliverBlocks), dim3(256), 0, net->stream, net->d_rec,
-                     net->d_rec_count, net->rec_cap, net->d_win_begin, net->d_done, net->d_rowptr,
-                     net->d_edges, net->d_ring, net->L, net->CW);
+  hipLaunchKernelGGL(k_window<T>, dim3((unsigned)net->tiles.size()), dim3(256), net->lds_bytes,
+                     net->stream, a);
 }
 
 int bs_net_set_timing(bs_net* net, int on) {
@@ -783,41 +1177,59 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
   HIPCHK(hipEventCreate(&ev_all0));
   HIPCHK(hipEventCreate(&ev_all1));
   std::vector<hipEvent_t> evs;
-  auto t_wall0 = std::chrono::steady_clock::now();
   HIPCHK(hipEventRecord(ev_all0, net->stream));
   long long done = 0;
   int rc = BS_OK;
+  double state_bytes = 0;
+  for (const DevPop& p : net->pops) state_bytes += (double)p.n * (2.0 * p.n_state * net->tsize + 8.0);
   while (done < n_steps) {
     const long long chunk = std::min<long long>(chunk_max, n_steps - done);
     const long long c0 = net->step;
     HIPCHK(hipMemsetAsync(net->d_rec_count, 0, sizeof(unsigned long long), net->stream));
-    HIPCHK(hipMemsetAsync(net->d_win_begin, 0, sizeof(long long), net->stream));
     const int nwin = (int)((chunk + net->W - 1) / net->W);
     if (net->timing) {
-      for (size_t i = evs.size(); i < (size_t)nwin * 3; ++i) {
+      for (size_t i = evs.size(); i < (size_t)nwin * 2; ++i) {
         hipEvent_t e;
         HIPCHK(hipEventCreate(&e));
         evs.push_back(e);
       }
     }
-    for (int w = 0; w < nwin; ++w) {
-      const long long n0 = c0 + (long long)w * net->W;
-      const int Wc = (int)std::min<long long>(net->W, c0 + chunk - n0);
-      if (net->timing) HIPCHK(hipEventRecord(evs[3 * w], net->stream));
-      if (net->tsize == 8)
-        launch_update<double>(net, (int)n0, Wc, c0);
-      else
-        launch_update<float>(net, (int)n0, Wc, c0);
-      if (net->timing) HIPCHK(hipEventRecord(evs[3 * w + 1], net->stream));
-      launch_deliver(net);
-      if (net->timing) HIPCHK(hipEventRecord(evs[3 * w + 2], net->stream));
-      HIPCHK(hipGetLastError());
-      net->n_update++;
-      net->n_deliver++;
-      // algorithmic update bytes: state in+out, ref in+out, ring words read+clear
-      for (const DevPop& p : net->pops) {
-        net->b_update += (double)p.n * (2.0 * p.n_state * net->tsize + 8.0 +
-                                        (double)Wc * (8.0 * p.n_cw + 16.0 * p.n_fx));
+    const bool full = chunk == chunk_max;
+    if (full && !net->timing) {
+      // replay the captured chunk: K window launches reading the base step
+      if (!net->graph) {
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(net->stream, hipStreamCaptureModeThreadLocal));
+        for (int w = 0; w < nwin; ++w) {
+          if (net->tsize == 8)
+            launch_window<double>(net, 0, net->W, 0, net->d_base, w);
+          else
+            launch_window<float>(net, 0, net->W, 0, net->d_base, w);
+        }
+        HIPCHK(hipStreamEndCapture(net->stream, &g));
+        hipError_t e = hipGraphInstantiate(&net->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(BS_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      }
+      net->h_base = c0;  // stays valid until the chunk's synchronisation below
+      HIPCHK(hipMemcpyAsync(net->d_base, &net->h_base, sizeof(long long), hipMemcpyHostToDevice,
+                            net->stream));
+      HIPCHK(hipGraphLaunch(net->graph, net->stream));
+      net->n_update += nwin;
+      net->b_update += state_bytes * nwin;
+    } else {
+      for (int w = 0; w < nwin; ++w) {
+        const long long n0 = c0 + (long long)w * net->W;
+        const int Wc = (int)std::min<long long>(net->W, c0 + chunk - n0);
+        if (net->timing) HIPCHK(hipEventRecord(evs[2 * w], net->stream));
+        if (net->tsize == 8)
+          launch_window<double>(net, (int)n0, Wc, c0);
+        else
+          launch_window<float>(net, (int)n0, Wc, c0);
+        if (net->timing) HIPCHK(hipEventRecord(evs[2 * w + 1], net->stream));
+        HIPCHK(hipGetLastError());
+        net->n_update++;
+        net->b_update += state_bytes;
       }
     }
     unsigned long long cnt = 0;
@@ -830,7 +1242,7 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
       break;
     }
     if (flags & FLAG_COUNT) {
-      rc = fail(BS_ECAPACITY, "arrival-ring event count exceeded the channel's in-degree");
+      rc = fail(BS_ECAPACITY, "arrival event count exceeded the channel's in-degree");
       break;
     }
     if (flags & FLAG_UNSTABLE) {
@@ -839,30 +1251,25 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
     }
     if (net->timing) {
       for (int w = 0; w < nwin; ++w) {
-        float a = 0, b = 0;
-        HIPCHK(hipEventElapsedTime(&a, evs[3 * w], evs[3 * w + 1]));
-        HIPCHK(hipEventElapsedTime(&b, evs[3 * w + 1], evs[3 * w + 2]));
+        float a = 0;
+        HIPCHK(hipEventElapsedTime(&a, evs[2 * w], evs[2 * w + 1]));
         net->t_update += a;
-        net->t_deliver += b;
       }
     }
     hrec.resize(cnt);
     if (cnt) HIPCHK(hipMemcpy(hrec.data(), net->d_rec, sizeof(int2) * cnt, hipMemcpyDeviceToHost));
     for (unsigned long long i = 0; i < cnt; ++i) {
       const int g = hrec[i].x;
-      const long long row = net->rowptr_h[g + 1] - net->rowptr_h[g];
-      const long long frow = net->frowptr_h[g + 1] - net->frowptr_h[g];
-      net->n_events += row + frow;
-      net->b_deliver += 8.0 + 16.0 + 4.0 * row + (frow ? 16.0 + 16.0 * frow : 0.0);
-      // population of g (few populations: linear scan)
-      int p = 0;
-      while (p + 1 < (int)net->pops.size() && g >= net->pops[p + 1].gid0) ++p;
-      if (net->pop_record[p]) {
+      const long long row = net->outdeg[g];
+      net->n_events += row;
+      // gather: the spike's bucket entry, its row's segment bounds, 4 B per edge
+      net->b_deliver += 8.0 + 4.0 * row;
+      net->b_update += 8.0;  // record entry
+      if (net->pop_record[net->gid_pop[g]]) {
         net->out_gid.push_back(g);
         net->out_step.push_back(hrec[i].y);
       }
     }
-    net->b_update += 8.0 * cnt;
     if (vm_out && ntaps > 0) {
       hvm.resize((size_t)chunk * std::max(1, net->n_vm_dev));
       HIPCHK(hipMemcpy(hvm.data(), net->d_vm, 8 * hvm.size(), hipMemcpyDeviceToHost));
@@ -877,13 +1284,13 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
   HIPCHK(hipEventSynchronize(ev_all1));
   float tot = 0;
   HIPCHK(hipEventElapsedTime(&tot, ev_all0, ev_all1));
-  (void)t_wall0;
   net->t_total = tot;
+  net->n_deliver = net->n_update;
+  net->t_deliver = net->t_update;
   for (hipEvent_t e : evs) (void)hipEventDestroy(e);
   (void)hipEventDestroy(ev_all0);
   (void)hipEventDestroy(ev_all1);
-  // sort recorded spikes by (step, gid)
-  {
+  {  // sort recorded spikes by (step, gid)
     std::vector<long long> key(net->out_gid.size());
     for (size_t i = 0; i < key.size(); ++i) key[i] = ((long long)net->out_step[i] << 32) | (uint32_t)net->out_gid[i];
     std::sort(key.begin(), key.end());
@@ -973,10 +1380,11 @@ __global__ void k_dbg_rng(long long n, const uint32_t* k0, const uint32_t* k1, c
   threefry2x32(k0[i], k1[i], c0[i], c1[i], o0[i], o1[i]);
 }
 __global__ void k_dbg_poisson(long long n, const uint32_t* seed, const uint32_t* step,
-                              const uint32_t* elem, double L, int pieces, uint32_t* out) {
+                              const uint32_t* elem, const double* cdf, const unsigned short* guide,
+                              int len, int parts, uint32_t* out) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  out[i] = poisson_knuth(seed[i], step[i], elem[i], L, pieces);
+  out[i] = poisson_inv(seed[i], step[i], elem[i], cdf, guide, len, parts);
 }
 
 template <typename X>
@@ -1026,14 +1434,19 @@ int bs_debug_poisson(int64_t n, const uint32_t* seed, const uint32_t* step, cons
                      double lam, uint32_t* out) {
   if (n < 0 || !seed || !step || !elem || !out || !(lam >= 0)) return fail(BS_EINVAL, "bs_debug_poisson: bad args");
   if (n == 0) return BS_OK;
-  int pieces = (int)std::ceil(lam / 16.0);
-  if (pieces < 1) pieces = 1;
-  const double L = std::exp(-(lam / (double)pieces));
+  std::vector<double> cdf;
+  std::vector<unsigned short> guide;
+  int parts = 1;
+  poisson_table(lam, cdf, guide, parts);
   uint32_t *a, *b, *c, *o;
+  double* dc;
+  unsigned short* dg;
   to_dev(&a, seed, n); to_dev(&b, step, n); to_dev(&c, elem, n); to_dev(&o, (const uint32_t*)nullptr, n);
-  hipLaunchKernelGGL(k_dbg_poisson, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, (long long)n, a, b, c, L, pieces, o);
+  to_dev(&dc, cdf.data(), (long long)cdf.size()); to_dev(&dg, guide.data(), (long long)guide.size());
+  hipLaunchKernelGGL(k_dbg_poisson, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, (long long)n, a, b, c,
+                     (const double*)dc, (const unsigned short*)dg, (int)cdf.size(), parts, o);
   hipError_t e = hipMemcpy(out, o, 4 * n, hipMemcpyDeviceToHost);
-  (void)hipFree(a); (void)hipFree(b); (void)hipFree(c); (void)hipFree(o);
+  (void)hipFree(a); (void)hipFree(b); (void)hipFree(c); (void)hipFree(o); (void)hipFree(dc); (void)hipFree(dg);
   if (e != hipSuccess) return fail(BS_EHIP, hipGetErrorString(e));
   return BS_OK;
 }

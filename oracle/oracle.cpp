@@ -47,33 +47,57 @@ extern "C" void orc_threefry2x32(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t
   *o1 = x1;
 }
 
-// Poisson by Knuth's product of uniforms; the definition the device samples
-// with (lam split into ceil(lam/16) equal parts, uniforms u = (x + 1/2) 2^-32
-// taken from threefry(key=(seed, step), ctr=(elem, j)) two per counter value,
-// at most 1024 uniforms per part).
-extern "C" uint32_t orc_poisson(uint32_t seed, uint32_t step, uint32_t elem, double lam) {
-  int parts = (int)ceil(lam / 16.0);
-  if (parts < 1) parts = 1;
-  const double L = exp(-(lam / (double)parts));
-  uint32_t count = 0, ctr = 0;
-  std::vector<uint32_t> pool;
-  size_t next = 0;
-  for (int p = 0; p < parts; ++p) {
-    double prod = 1.0;
-    for (int it = 0; it < 1024; ++it) {
-      if (next == pool.size()) {
-        uint32_t a, b;
-        orc_threefry2x32(seed, step, elem, ctr++, &a, &b);
-        pool.push_back(a);
-        pool.push_back(b);
-      }
-      const double u = ((double)pool[next++] + 0.5) * 2.3283064365386963e-10;
-      prod = prod * u;
-      if (prod <= L) break;
-      count += 1;
-    }
+// Poisson sampling by inversion of a tabulated CDF -- the definition the
+// device samples with. Table of Poisson(mu), mu = lam / parts with
+// parts = max(1, ceil(lam / 32)): pmf_0 = exp(-mu), pmf_k = pmf_{k-1} * mu / k,
+// cdf_k = cdf_{k-1} + pmf_k, stopping once k > mu and 1 - cdf_k < 1e-15 (at
+// most 512 entries); guide[g] = min{k : cdf_k >= g / 64}. Part p takes uniform
+// number p of threefry(key=(seed, step), ctr=(elem, p / 2)) (word p % 2),
+// u = (x + 1/2) 2^-32, and adds min{k : u <= cdf_k} searched upward from
+// guide[floor(64 u)] and capped at the last entry.
+struct PoissonTable {
+  std::vector<double> cdf;
+  std::vector<int> guide;
+  int parts = 1;
+};
+
+static PoissonTable make_table(double lam) {
+  PoissonTable T;
+  T.parts = std::max(1, (int)ceil(lam / 32.0));
+  const double mu = lam / (double)T.parts;
+  double pmf = exp(-mu);
+  T.cdf.push_back(pmf);
+  for (int k = 1; k < 512; ++k) {
+    pmf = pmf * mu / (double)k;
+    const double c = T.cdf.back() + pmf;
+    T.cdf.push_back(c);
+    if ((double)k > mu && 1.0 - c < 1e-15) break;
+  }
+  const int len = (int)T.cdf.size();
+  for (int g = 0; g < 64; ++g) {
+    const double t = (double)g / 64.0;
+    int k = 0;
+    while (k < len - 1 && T.cdf[k] < t) ++k;
+    T.guide.push_back(k);
+  }
+  return T;
+}
+
+static uint32_t draw_poisson(const PoissonTable& T, uint32_t seed, uint32_t step, uint32_t elem) {
+  uint32_t count = 0, w[2] = {0, 0};
+  const int len = (int)T.cdf.size();
+  for (int p = 0; p < T.parts; ++p) {
+    if (p % 2 == 0) orc_threefry2x32(seed, step, elem, (uint32_t)(p / 2), &w[0], &w[1]);
+    const double u = ((double)w[p % 2] + 0.5) * 2.3283064365386963e-10;
+    int k = T.guide[(int)(u * 64.0)];
+    while (k < len - 1 && u > T.cdf[k]) ++k;
+    count += (uint32_t)k;
   }
   return count;
+}
+
+extern "C" uint32_t orc_poisson(uint32_t seed, uint32_t step, uint32_t elem, double lam) {
+  return draw_poisson(make_table(lam), seed, step, elem);
 }
 
 // exp and log from first principles: Cody-Waite reduction by ln 2 (split
@@ -153,6 +177,7 @@ struct orc_net {
   std::vector<std::vector<double>> params;  // per pop
   std::vector<bs_channel> chans;
   std::vector<std::vector<int>> gen_index;  // per channel (POISSON)
+  std::vector<PoissonTable> ptab;           // per channel (POISSON)
   std::vector<OrcProj> projs;
   std::vector<int> chan_proj;               // channel -> projection (COUNT/FIXED)
   std::vector<long long> gid0;
@@ -191,6 +216,7 @@ extern "C" int orc_create(const bs_net_desc* d, orc_net** out) {
   net->chans.assign(d->channels, d->channels + d->n_channels);
   net->chan_proj.assign(d->n_channels, -1);
   net->gen_index.resize(d->n_channels);
+  net->ptab.resize(d->n_channels);
   long long g = 0;
   for (int p = 0; p < d->n_pops; ++p) {
     const bs_population& sp = d->pops[p];
@@ -208,6 +234,7 @@ extern "C" int orc_create(const bs_net_desc* d, orc_net** out) {
         std::vector<int> gi(sp.n);
         for (int i = 0; i < sp.n; ++i) gi[i] = ch.gen_index ? ch.gen_index[i] : i;
         net->gen_index[c] = gi;
+        net->ptab[c] = make_table(ch.lam);
       }
     }
   }
@@ -584,7 +611,7 @@ static int run_t(orc_net* net, int64_t n_steps, double* vm_out) {
       const std::vector<int>& gi = net->gen_index[c];
       const bool active = (ch.t_min_step < n) && (n <= ch.t_max_step) && ch.lam > 0.0;
       for (size_t j = 0; j < h.size(); ++j)
-        h[j] = (active && gi[j] >= 0) ? (double)orc_poisson(ch.seed, (uint32_t)n, (uint32_t)gi[j], ch.lam) : 0.0;
+        h[j] = (active && gi[j] >= 0) ? (double)draw_poisson(net->ptab[c], ch.seed, (uint32_t)n, (uint32_t)gi[j]) : 0.0;
     }
     for (int p = 0; p < np; ++p) {
       const bs_population& sp = net->pops[p];
