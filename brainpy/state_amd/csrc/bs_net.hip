@@ -452,7 +452,7 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
       rounds = max(rounds, (len + 7) >> 3);
     }
     for (int rd = 0; rd < rounds; ++rd) {
-      uint32_t wd[4][4];
+      uint32_t wd[4][8];
 #pragma unroll
       for (int q = 0; q < 4; ++q)
 #pragma unroll
