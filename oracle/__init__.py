@@ -49,6 +49,8 @@ def lib():
             'orc_get_state': (C.c_int, [P, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
             'orc_get_weights': (C.c_int, [P, C.c_int32, C.POINTER(C.c_double)]),
             'orc_threefry2x32': (None, [C.c_uint32] * 4 + [C.POINTER(C.c_uint32)] * 2),
+            'orc_neuron_step': (C.c_int, [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_int), C.c_double, C.c_double, C.c_int]),
             'orc_poisson': (C.c_uint32, [C.c_uint32] * 3 + [C.c_double]),
             'orc_exp': (C.c_double, [C.c_double]),
             'orc_log': (C.c_double, [C.c_double]),
@@ -87,6 +89,29 @@ def log(x):
 
 def pow(x, y):  # noqa: A001
     return lib().orc_pow(float(x), float(y))
+
+
+class Neuron:
+    """One neuron of a model spec driven step by step through the oracle's update."""
+
+    def __init__(self, model, dt, dtype='float64', index=0):
+        self.model = model
+        self.P = np.ascontiguousarray(model.param_block(dt), dtype=np.float64)
+        self.s = np.ascontiguousarray(model.state0(dt)[:, index], dtype=np.float64)
+        self.r = C.c_int(0)
+        self.dtype = 0 if dtype == 'float64' else 1
+
+    def step(self, in0=0.0, in1=0.0):
+        rc = lib().orc_neuron_step(self.model.model_id, self.P.ctypes.data_as(C.POINTER(C.c_double)),
+                                   self.s.ctypes.data_as(C.POINTER(C.c_double)), C.byref(self.r),
+                                   float(in0), float(in1), self.dtype)
+        if rc < 0:
+            raise RuntimeError(f'oracle neuron step failed ({rc})')
+        return bool(rc)
+
+    @property
+    def V(self):
+        return float(self.s[0])
 
 
 class OracleNet:

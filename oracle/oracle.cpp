@@ -696,3 +696,35 @@ extern "C" int orc_get_weights(orc_net* net, int32_t proj, double* out) {
   for (size_t e = 0; e < P.w.size(); ++e) out[e] = P.w[e];
   return BS_OK;
 }
+
+// One model update of a single neuron with explicit inputs (the reference's
+// unit tests drive neurons this way: add_delta_input + update()). in0 is the
+// summed delta input (mV for iaf_psc_delta, pA for iaf_psc_exp/alpha, nS into
+// g_ex for aeif_cond_alpha), in1 the aeif g_in input. state: BS_NSTATE values.
+extern "C" int orc_neuron_step(int model, const double* params, double* state, int* ref,
+                               double in0, double in1, int dtype) {
+  std::vector<double> P(params, params + BS_NPARAM);
+  if (dtype == BS_F32 && model != BS_AEIF_COND_ALPHA) {
+    float s[BS_NSTATE];
+    for (int k = 0; k < BS_NSTATE; ++k) s[k] = (float)state[k];
+    bool spk = false;
+    if (model == BS_IAF_PSC_DELTA) spk = upd_delta<float>(P, s[0], s[1], *ref, (float)in0);
+    else if (model == BS_IAF_PSC_EXP) spk = upd_exp<float>(P, s, *ref, (float)in0);
+    else if (model == BS_IAF_PSC_ALPHA) spk = upd_alpha<float>(P, s, *ref, (float)in0);
+    else return -1;
+    for (int k = 0; k < BS_NSTATE; ++k) state[k] = s[k];
+    return spk ? 1 : 0;
+  }
+  bool spk = false;
+  if (model == BS_IAF_PSC_DELTA) spk = upd_delta<double>(P, state[0], state[1], *ref, in0);
+  else if (model == BS_IAF_PSC_EXP) spk = upd_exp<double>(P, state, *ref, in0);
+  else if (model == BS_IAF_PSC_ALPHA) spk = upd_alpha<double>(P, state, *ref, in0);
+  else if (model == BS_AEIF_COND_ALPHA) {
+    bool unstable = false;
+    spk = aeif::update(P, state, *ref, in0, in1, &unstable);
+    if (unstable) return -5;
+  } else {
+    return -1;
+  }
+  return spk ? 1 : 0;
+}

@@ -39,6 +39,11 @@ int orc_copy_spikes(orc_net* net, int32_t* gid, int32_t* step, int64_t cap);
 int orc_get_state(orc_net* net, int32_t pop, int32_t var, double* out);
 int orc_get_weights(orc_net* net, int32_t proj, double* out);
 
+/* One update of a single neuron with explicit summed inputs; returns the
+ * spike flag (0/1) or a negative error. */
+int orc_neuron_step(int model, const double* params, double* state, int* ref, double in0,
+                    double in1, int dtype);
+
 /* Elementwise primitives (the definitions the device must reproduce). */
 void orc_threefry2x32(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t c1, uint32_t* o0,
                       uint32_t* o1);

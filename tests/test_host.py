@@ -1,0 +1,175 @@
+"""CPU tests: the C-ABI library loads and exports its header, and the host-side
+Simulator logic (rules, seeds, descriptor layout, validation) is right."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from brainpy.state_amd import (Simulator, all_to_all, explicit_edges, fixed_indegree,
+                               fixed_total_number, iaf_psc_delta, iaf_psc_exp, one_to_one,
+                               pairwise_bernoulli, poisson_generator, spike_recorder, units as u,
+                               voltmeter, aeif_cond_alpha)
+from brainpy.state_amd import _lib as L
+from brainpy.state_amd.simulator import delay_to_steps, seq_sum_table
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_functions():
+    text = open(os.path.join(ROOT, 'include', 'brainspike_hip.h')).read()
+    return sorted(set(re.findall(r'^\s*(?:int|const char\*)\s+(bs_\w+)\(', text, re.M)))
+
+
+def test_library_exports_every_header_function():
+    if not os.path.exists(L.LIB_PATH):
+        pytest.skip('engine not built')
+    lib = C.CDLL(L.LIB_PATH)
+    names = _header_functions()
+    assert len(names) >= 18
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_abi_version_and_error_path_without_device():
+    lib = L.lib()
+    assert lib.bs_abi_version() == L.BS_ABI_VERSION
+    assert lib.bs_device_count() >= 0
+    h = C.c_void_p()
+    rc = lib.bs_net_create(None, C.byref(h))
+    assert rc == -1
+    assert b'null' in lib.bs_last_error()
+
+
+def test_descriptor_abi_rejected_on_version_mismatch():
+    lib = L.lib()
+    d = L.NetDesc()
+    d.abi_version = 999
+    h = C.c_void_p()
+    assert lib.bs_net_create(C.byref(d), C.byref(h)) == -1
+    assert b'ABI' in lib.bs_last_error()
+
+
+def test_derive_seed_matches_reference_formula():
+    # simulator.py:926-935
+    assert Simulator._derive_seed(None, 0) == 1
+    assert Simulator._derive_seed(1, 2) == (1_000_003 + 3) & 0x7FFFFFFF
+    assert Simulator._derive_seed(2**40, 5) == (2**40 * 1_000_003 + 6) & 0x7FFFFFFF
+
+
+def test_fixed_indegree_properties():
+    spec = fixed_indegree(7).sample(50, 40, seed=3, pre_is_post=True, allow_autapses=False,
+                                    allow_multapses=True)
+    assert spec.n_edges == 280
+    assert np.all(np.bincount(spec.post_idx, minlength=40) == 7)
+    assert not np.any(spec.pre_idx == spec.post_idx)
+    assert spec.pre_idx.min() >= 0 and spec.pre_idx.max() < 50
+    spec2 = fixed_indegree(7).sample(50, 40, seed=3, pre_is_post=False, allow_autapses=True,
+                                     allow_multapses=False)
+    for j in range(40):
+        pre = spec2.pre_idx[spec2.post_idx == j]
+        assert len(set(pre.tolist())) == 7
+
+
+def test_pairwise_bernoulli_density_and_order():
+    spec = pairwise_bernoulli(0.1).sample(300, 200, seed=1, pre_is_post=False,
+                                          allow_autapses=True, allow_multapses=True)
+    assert abs(spec.n_edges / 60000 - 0.1) < 0.01
+    flat = spec.pre_idx.astype(np.int64) * 200 + spec.post_idx
+    assert np.all(np.diff(flat) > 0)      # row-major order, no duplicates
+
+
+def test_rules_misc():
+    a = all_to_all.sample(3, 3, seed=0, pre_is_post=True, allow_autapses=False, allow_multapses=True)
+    assert a.n_edges == 6
+    o = one_to_one.sample(4, 4, seed=0, pre_is_post=False, allow_autapses=True, allow_multapses=True)
+    assert np.array_equal(o.pre_idx, o.post_idx)
+    f = fixed_total_number(25).sample(10, 10, seed=0, pre_is_post=False, allow_autapses=True,
+                                      allow_multapses=True)
+    assert f.n_edges == 25
+    e = explicit_edges(np.array([0, 0, 2]), np.array([1, 4, 3]))
+    assert e.sample(5, 5, seed=0, pre_is_post=True, allow_autapses=True, allow_multapses=True).n_edges == 3
+    with pytest.raises(ValueError):
+        explicit_edges(np.array([0, 1]), np.array([1]))
+    with pytest.raises(ValueError):
+        pairwise_bernoulli(1.5)
+    with pytest.raises(ValueError):
+        fixed_indegree(-1)
+
+
+def test_delay_conversion():
+    assert delay_to_steps(1.5 * u.ms, 0.1) == 15
+    assert delay_to_steps(None, 0.1) == 0
+    assert np.array_equal(delay_to_steps(np.array([1.0, 2.0]) * u.ms, 0.1), [10, 20])
+    with pytest.raises(ValueError):
+        delay_to_steps(0.15 * u.ms, 0.1)
+
+
+def test_seq_sum_table_is_sequential_accumulation():
+    t = seq_sum_table(0.1, 30, np.float64)
+    acc = 0.0
+    for k in range(31):
+        assert t[k] == acc
+        acc = acc + 0.1
+    t32 = seq_sum_table(0.1, 30, np.float32)
+    acc = np.float32(0.0)
+    for k in range(31):
+        assert t32[k] == float(acc)
+        acc = np.float32(acc + np.float32(0.1))
+
+
+def test_descriptor_channel_order_and_generator_fanout():
+    sim = Simulator(dt=0.1 * u.ms)
+    ne = sim.create(iaf_psc_delta, 20)
+    ni = sim.create(iaf_psc_delta, 5)
+    noise = sim.create(poisson_generator, rate=8000. * u.Hz)
+    sim.connect(noise, ne + ni, weight=0.1 * u.mV, delay=1.5 * u.ms)     # ordinals 0, 1
+    sim.connect(ne, ne + ni, weight=0.1, delay=1.5, rule=fixed_indegree(4), seed=1)  # 2, 3
+    sim.connect(ni, ne + ni, weight=-0.5, delay=1.5, rule=fixed_indegree(1), seed=2)  # 4, 5
+    d = sim.build_desc()
+    c = d.c
+    assert c.n_pops == 2 and c.n_channels == 6 and c.n_projs == 4
+    kinds = [c.channels[k].kind for k in range(6)]
+    assert kinds == [L.BS_CH_POISSON, L.BS_CH_COUNT, L.BS_CH_COUNT] * 2
+    # generators realised per target segment with derived seeds (simulator.py:1437-1463)
+    assert c.channels[0].seed == Simulator._derive_seed(0, 0)
+    assert c.channels[3].seed == Simulator._derive_seed(0, 1)
+    assert c.channels[0].lam == pytest.approx(0.8)
+    assert c.channels[0].t_min_step == 0
+    assert c.channels[1].weight == 0.1 and c.channels[5].weight == -0.5
+    assert c.projs[0].delay_steps == 15
+
+
+def test_validation_errors():
+    sim = Simulator(dt=0.1)
+    a = sim.create(aeif_cond_alpha, 3)
+    with pytest.raises(ValueError):
+        sim.connect(a, a, weight=1.0, delay=1.0)        # aeif needs a receptor port
+    with pytest.raises(ValueError):
+        iaf_psc_exp(1, V_reset=-55., V_th=-55.)
+    with pytest.raises(ValueError):
+        Simulator(dt=0.1, dtype='float16')
+    with pytest.raises(NotImplementedError):
+        iaf_psc_exp(1, delta=1.0)
+
+
+def test_simulate_without_device_fails_loudly():
+    if L.lib().bs_device_count() > 0:
+        pytest.skip('a device is present')
+    sim = Simulator(dt=0.1)
+    p = sim.create(iaf_psc_delta, 3)
+    with pytest.raises(L.EngineUnavailable):
+        sim.simulate(1.0)
+
+
+def test_recorder_taps_registered():
+    sim = Simulator(dt=0.1)
+    p = sim.create(iaf_psc_delta, 10)
+    sr = sim.create(spike_recorder)
+    vm = sim.create(voltmeter)
+    sim.connect(p[:4], sr)
+    sim.connect(vm, p[2:5])
+    d = sim.build_desc()
+    assert d.c.pops[0].record_spikes == 1
+    assert d.c.pops[0].n_vm == 3 and d.n_taps == 3
