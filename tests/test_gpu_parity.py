@@ -113,3 +113,81 @@ def test_simulate_api_raster_and_trace():
     res2 = sim.cont(100. * u.ms)
     assert res2.spikes(h['esr']).shape == (1000, 200)
     assert float(res2.times.to_decimal(u.ms)[0]) == pytest.approx(200.0)
+
+
+# ---------------------------------------------------------------- other models
+def _mixed_net(model, dtype='float64', hetero=False, n=300, delay=1.0, gen_subset=False):
+    from brainpy.state_amd import (Simulator, aeif_cond_alpha, dist, fixed_indegree,
+                                   poisson_generator, spike_recorder, voltmeter)
+    from brainpy.state_amd.init import Uniform
+    sim = Simulator(dt=0.1 * u.ms, dtype=dtype)
+    cond = model is aeif_cond_alpha
+    kw = {} if cond else dict(V_initializer=Uniform(-70., -56., seed=5))
+    a = sim.create(model, n, **kw)
+    b = sim.create(model, n // 4, **kw)
+    noise = sim.create(poisson_generator, rate=12000. * u.Hz)
+    rt = dict(receptor_type=1) if cond else {}
+    rti = dict(receptor_type=2) if cond else {}
+    wE = 3.0 if cond else 60.0
+    wI = 4.0 if cond else -240.0
+    if model.__name__ == 'iaf_psc_delta':
+        wE, wI = 0.5, -2.0
+    if gen_subset:
+        sim.connect(noise, a[:n // 2], weight=wE, delay=delay, **rt)
+        sim.connect(noise, a[n // 2:], weight=wE, delay=2 * delay, **rt)
+    else:
+        sim.connect(noise, a, weight=wE, delay=delay, **rt)
+    sim.connect(noise, b, weight=wE, delay=delay, **rt)
+    w = dist.Normal(wE, 0.2 * wE) if hetero else wE
+    sim.connect(a, a + b, weight=w, delay=delay, rule=fixed_indegree(30), seed=4, **rt)
+    wi = dist.Uniform(0.5 * wI, 1.5 * wI) if hetero else wI
+    sim.connect(b, a + b, weight=wi, delay=2 * delay, rule=fixed_indegree(10), seed=5, **rti)
+    ra = sim.create(spike_recorder)
+    sim.connect(a, ra)
+    rb = sim.create(spike_recorder)
+    sim.connect(b, rb)
+    vm = sim.create(voltmeter)
+    sim.connect(vm, a[:5])
+    return sim
+
+
+@pytest.mark.parametrize('dtype', ['float64', 'float32'])
+def test_iaf_psc_exp_network(dtype):
+    from brainpy.state_amd import iaf_psc_exp
+    sim = _mixed_net(iaf_psc_exp, dtype=dtype)
+    gpu, orc = _both(sim, 3000)
+    assert len(orc[1]) > 100
+    _assert_same(gpu, orc, rtol=1e-12 if dtype == 'float64' else 0)
+
+
+def test_iaf_psc_alpha_network_fp64():
+    from brainpy.state_amd import iaf_psc_alpha
+    sim = _mixed_net(iaf_psc_alpha)
+    gpu, orc = _both(sim, 3000)
+    assert len(orc[1]) > 100
+    _assert_same(gpu, orc)
+
+
+def test_aeif_cond_alpha_network_fp64():
+    from brainpy.state_amd import aeif_cond_alpha
+    sim = _mixed_net(aeif_cond_alpha, n=200)
+    gpu, orc = _both(sim, 1500)
+    assert len(orc[1]) > 50
+    _assert_same(gpu, orc)
+
+
+def test_heterogeneous_weights_fixed_point_sums():
+    from brainpy.state_amd import iaf_psc_exp
+    sim = _mixed_net(iaf_psc_exp, hetero=True)
+    gpu, orc = _both(sim, 3000)
+    assert len(orc[1]) > 100
+    _assert_same(gpu, orc)
+
+
+def test_generator_subsets_and_zero_delay_window():
+    """Generators on sub-segments (gen_index) and delay 0 (window of one step)."""
+    from brainpy.state_amd import iaf_psc_delta
+    sim = _mixed_net(iaf_psc_delta, gen_subset=True, delay=0.0)
+    gpu, orc = _both(sim, 1000)
+    assert len(orc[1]) > 100
+    _assert_same(gpu, orc)
