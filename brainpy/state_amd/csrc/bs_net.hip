@@ -81,6 +81,7 @@ struct DevChan {
   const int* gen_index;
   const double* cdf;             // POISSON: CDF table of Poisson(lam / parts)
   const unsigned short* guide;   // POISSON: 64-entry guide table
+  long long* pacc;               // FIXED fed by a plastic projection: deposits per post
 };
 
 struct Tile {
@@ -114,6 +115,7 @@ struct WinArgs {
   int n0, W, Wmax;
   const long long* base_ptr;  // graph replays: n0 = *base_ptr + widx * Wmax
   int widx;
+  unsigned char* spk_hist;  // [Ls][Ntot] spike flags (plastic networks), or null
   int max_span;        // longest gathered step range, LDS prefix size
   int TS;              // neurons per tile
   int2* rec;
@@ -188,6 +190,7 @@ struct ChReg {
   const double* cdf;            // Poisson CDF (LDS or global)
   const unsigned short* guide;  // Poisson guide table
   const int* gen_index;
+  long long* pacc;              // plastic deposits (FIXED channel of a plastic projection)
 };
 
 template <typename T>
@@ -209,6 +212,7 @@ __device__ __forceinline__ ChReg<T> load_ch(const DevChan& ch, const unsigned ch
   c.tmax = ch.tmax;
   c.weight = (T)ch.weight;
   c.gen_index = ch.gen_index;
+  c.pacc = ch.pacc;
   const bool staged = ch.lds_off >= 0;
   c.tab = staged ? (const T*)(tab + ch.lds_off) : tables + ch.table_off;
   c.cdf = staged ? (const double*)(tab + ch.lds_off) : ch.cdf;
@@ -255,7 +259,14 @@ __device__ __forceinline__ T chan_value(const UpdCtx<T>& u, const ChReg<T>& c, i
       cn = poisson_inv(c.seed, (uint32_t)sgs, (uint32_t)e, c.cdf, c.guide, c.cdf_len, c.parts);
     return (T)cn * c.weight;
   }
-  const long long q = u.fxa[(w * u.nfx + c.fx) * u.TS + lane];  // BS_CH_FIXED
+  long long q;  // BS_CH_FIXED
+  if (c.pacc) {  // plastic: this step's deposits (windows of one step), consumed here
+    const int j = u.start + lane;
+    q = c.pacc[j];
+    c.pacc[j] = 0;
+  } else {
+    q = u.fxa[(w * u.nfx + c.fx) * u.TS + lane];
+  }
   return (T)__builtin_ldexp((double)q, -c.shift);
 }
 
@@ -279,6 +290,7 @@ __device__ __forceinline__ void update_loop(const UpdCtx<T>& u, const ChReg<T>* 
     const T acc1 = u.nrec > 1 ? u.inp[(u.Wm + w) * u.TS + u.tid] : (T)0;
     const bool spike = stepf(s, r, acc0, acc1);
     if (vmc >= 0) a.vm_buf[(long long)(step - a.vm_step0) * a.n_vm + vmc] = (double)s[0];
+    if (a.spk_hist) a.spk_hist[(long long)(step % a.Ls) * a.Ntot + u.gid] = spike ? 1 : 0;
     if (spike) {
       const int p = atomicAdd(&u.spn[w], 1);
       u.spl[w * u.TS + p] = (unsigned short)u.tid;
@@ -571,6 +583,134 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   }
 }
 
+// ------------------------------------------------------------ STDP kernels
+// Plastic projection state (stdp_synapse.py:163-179 on event_plastic.py:454-536),
+// per-edge delays. At arrival step n, for edge e (pre i, post j, delay d):
+//   pre trace   tr_e(n) = U_i(n - d),  U_i(m) = U_i(m-1) P+ + s_i(m-1)
+//   post trace  V_j(n)  = V_j(n-1) P- + s_j(n-1)
+//   pre_fired   s_i(n - 1 - d), post_fired s_j(n - 1)
+//   w <- facilitate(w, tr - pre_fired) where post_fired, then
+//   w <- depress(w, V - post_fired) where pre_fired; the new w of arriving
+//   edges is delivered to j at step n.
+// Edges are stored grouped by (pre, delay) with row offsets per delay, and
+// indexed by post (CSC) for the facilitation pass.
+struct StdpDev {
+  int pre_pop, post_pop, pre_gid0, post_gid0, n_pre, n_post, d_min, d_max, Lu, shift;
+  const int* e_post;
+  const int* e_pre;
+  const int* e_dly;
+  const unsigned* roff;     // [n_pre][d_max - d_min + 2]
+  const unsigned* csc_ptr;  // [n_post + 1]
+  const unsigned* csc_eid;  // edge ids grouped by post
+  double* w;
+  double* U;                // [Lu][n_pre]
+  double* V;                // [n_post]
+  long long* pacc;          // [n_post] deposits for the current step
+  double decay_plus, decay_minus, lam, alpha, mup, mum, Wmax;
+};
+
+struct StdpArgs {
+  const StdpDev* P;
+  int n_proj;
+  const unsigned char* spk_hist;  // [Ls][Ntot]
+  const uint32_t* sb;
+  const uint32_t* sb_count;
+  long long Ntot;
+  int Ls, n_pops, n;
+  const long long* base_ptr;      // graph replays: n = *base_ptr + widx
+  int widx;
+};
+
+__device__ __forceinline__ double stdp_facilitate(const StdpDev& p, double w, double kplus) {
+  const double nw = w / p.Wmax + p.lam * pow_p(1.0 - w / p.Wmax, p.mup) * kplus;
+  return nw < 1.0 ? nw * p.Wmax : p.Wmax;
+}
+__device__ __forceinline__ double stdp_depress(const StdpDev& p, double w, double kminus) {
+  const double nw = w / p.Wmax - p.alpha * p.lam * pow_p(w / p.Wmax, p.mum) * kminus;
+  return nw > 0.0 ? nw * p.Wmax : 0.0;
+}
+
+__device__ __forceinline__ int stdp_step(const StdpArgs& a) {
+  return a.base_ptr ? (int)(*a.base_ptr) + a.widx : a.n;
+}
+
+// Trace recurrences of step n from the spikes of step n-1.
+__global__ __launch_bounds__(256) void k_stdp_traces(StdpArgs a) {
+  const int n = stdp_step(a);
+  const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const int hs = ((n - 1) % a.Ls + a.Ls) % a.Ls;
+  for (int q = 0; q < a.n_proj; ++q) {
+    const StdpDev& p = a.P[q];
+    const int cur = n % p.Lu, prv = (n - 1 + p.Lu) % p.Lu;
+    for (long long i = t0; i < p.n_pre; i += stride) {
+      const double s = n >= 1 ? (double)a.spk_hist[(long long)hs * a.Ntot + p.pre_gid0 + i] : 0.0;
+      const double u = n >= 1 ? p.U[(long long)prv * p.n_pre + i] : 0.0;
+      p.U[(long long)cur * p.n_pre + i] = u * p.decay_plus + s;
+    }
+    for (long long j = t0; j < p.n_post; j += stride) {
+      const double s = n >= 1 ? (double)a.spk_hist[(long long)hs * a.Ntot + p.post_gid0 + j] : 0.0;
+      p.V[j] = p.V[j] * p.decay_minus + s;
+    }
+  }
+}
+
+// Facilitation of post-spike edges and depression + delivery of arriving edges.
+// Work items: for every projection, (a) each post spike of step n-1, then
+// (b) each pre spike of steps n-1-d_max .. n-1-d_min; one workgroup per item.
+__global__ __launch_bounds__(256) void k_stdp_events(StdpArgs a) {
+  const int n = stdp_step(a);
+  if (n < 1) return;
+  const int hs1 = (n - 1) % a.Ls;
+  long long item = blockIdx.x;
+  for (int q = 0; q < a.n_proj; ++q) {
+    const StdpDev& p = a.P[q];
+    // (a) post spikes of step n-1
+    const long long na = a.sb_count[hs1 * a.n_pops + p.post_pop];
+    for (; item < na; item += gridDim.x) {
+      const int j = (int)a.sb[(long long)hs1 * a.Ntot + p.post_gid0 + item];
+      for (unsigned c = p.csc_ptr[j] + threadIdx.x; c < p.csc_ptr[j + 1]; c += blockDim.x) {
+        const unsigned e = p.csc_eid[c];
+        const int d = p.e_dly[e];
+        const int i = p.e_pre[e];
+        const int ms = n - 1 - d;
+        const bool pre_fired =
+            ms >= 0 && a.spk_hist[(long long)(ms % a.Ls) * a.Ntot + p.pre_gid0 + i] != 0;
+        if (pre_fired) continue;  // handled with the arrival below, facilitation first
+        const int mu = n - d;
+        const double tr = mu >= 0 ? p.U[(long long)(mu % p.Lu) * p.n_pre + i] : 0.0;
+        p.w[e] = stdp_facilitate(p, p.w[e], tr - 0.0);
+      }
+    }
+    item -= na;
+    // (b) pre spikes whose edges of delay n-1-s arrive now
+    int s_lo = n - 1 - p.d_max;
+    if (s_lo < 0) s_lo = 0;
+    const int s_hi = n - 1 - p.d_min;
+    const int nd1 = p.d_max - p.d_min + 2;
+    for (int s = s_lo; s <= s_hi; ++s) {
+      const long long ns = a.sb_count[(s % a.Ls) * a.n_pops + p.pre_pop];
+      for (; item < ns; item += gridDim.x) {
+        const int i = (int)a.sb[(long long)(s % a.Ls) * a.Ntot + p.pre_gid0 + item];
+        const int d = n - 1 - s;
+        const unsigned* ro = p.roff + (long long)i * nd1 + (d - p.d_min);
+        const double tr = p.U[(long long)((n - d) % p.Lu) * p.n_pre + i];
+        for (unsigned e = ro[0] + threadIdx.x; e < ro[1]; e += blockDim.x) {
+          const int j = p.e_post[e];
+          const bool post_fired = a.spk_hist[(long long)hs1 * a.Ntot + p.post_gid0 + j] != 0;
+          double w = p.w[e];
+          if (post_fired) w = stdp_facilitate(p, w, tr - 1.0);
+          w = stdp_depress(p, w, p.V[j] - (post_fired ? 1.0 : 0.0));
+          p.w[e] = w;
+          atomicAdd((unsigned long long*)&p.pacc[j],
+                    (unsigned long long)(long long)__builtin_rint(__builtin_ldexp(w, p.shift)));
+        }
+      }
+      item -= ns;
+    }
+  }
+}
+
 // ============================================================== host side
 struct bs_net {
   int dtype = BS_F64;
@@ -604,6 +744,13 @@ struct bs_net {
   int n_vm_dev = 0;
   double* d_vm = nullptr;
   long long vm_cap_steps = 0;
+  // plastic projections
+  std::vector<StdpDev> stdp;         // device pointers inside
+  StdpDev* d_stdp = nullptr;
+  unsigned char* d_spk_hist = nullptr;
+  std::vector<int> proj_stdp;        // descriptor projection -> stdp index (-1: static)
+  std::vector<std::vector<double>> w0;        // initial weights, (pre, delay) order
+  std::vector<std::vector<long long>> w_ref;  // reference CSR position -> stored position
   long long* d_base = nullptr;     // first step of the chunk a graph replay advances
   long long h_base = 0;
   hipGraphExec_t graph = nullptr;  // one full chunk of window launches
@@ -643,6 +790,8 @@ static void free_net(bs_net* net) {
   F(net->d_rec_count); F(net->d_flags); F(net->d_vm_col); F(net->d_vm);
   for (void* p : net->owned) F(p);
   F(net->d_base);
+  F(net->d_stdp);
+  F(net->d_spk_hist);
   if (net->graph) (void)hipGraphExecDestroy(net->graph);
   if (net->stream) (void)hipStreamDestroy(net->stream);
   delete net;
@@ -874,8 +1023,119 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
       BAD("projection channel does not belong to the post population");
     if (chan_proj[pj.channel] >= 0) BAD("two projections share one channel");
     chan_proj[pj.channel] = q;
-    if (pj.plastic) BAD("plastic projections are not supported by this engine build");
+    net->proj_stdp.push_back(-1);
     const int kind = d->channels[pj.channel].kind;
+    if (pj.plastic) {
+      if (kind != BS_CH_FIXED) BAD("a plastic projection needs a FIXED channel");
+      if (!pj.weight) BAD("a plastic projection needs per-edge initial weights");
+      if (!(pj.stdp.Wmax != 0.0) || !(pj.stdp.tau_plus > 0) || !(pj.stdp.tau_minus > 0))
+        BAD("invalid stdp_synapse parameters");
+      const int npre = d->pops[pj.pre_pop].n, npost = d->pops[pj.post_pop].n;
+      const long long E = pj.n_edges;
+      int pdmin = 1 << 30, pdmax = 0;
+      for (long long e = 0; e < E; ++e) {
+        const int a = pj.pre[e], b = pj.post[e];
+        if (a < 0 || a >= npre || b < 0 || b >= npost) BAD("edge index out of range");
+        const int dd = pj.delay ? pj.delay[e] : pj.delay_steps;
+        if (dd < 1) BAD("plastic projections need delays of at least one step");
+        pdmin = std::min(pdmin, dd);
+        pdmax = std::max(pdmax, dd);
+        net->outdeg[net->pops[pj.pre_pop].gid0 + a]++;
+      }
+      if (E == 0) pdmin = pdmax = 1;
+      const int nd1 = pdmax - pdmin + 2;
+      // stored order: stable by (pre, delay); reference order: stable by pre
+      std::vector<long long> ord(E), ref(E);
+      for (long long e = 0; e < E; ++e) ord[e] = ref[e] = e;
+      auto dly = [&](long long e) { return pj.delay ? pj.delay[e] : pj.delay_steps; };
+      std::stable_sort(ord.begin(), ord.end(), [&](long long x, long long y) {
+        if (pj.pre[x] != pj.pre[y]) return pj.pre[x] < pj.pre[y];
+        return dly(x) < dly(y);
+      });
+      std::stable_sort(ref.begin(), ref.end(), [&](long long x, long long y) { return pj.pre[x] < pj.pre[y]; });
+      std::vector<long long> spos(E);
+      for (long long k = 0; k < E; ++k) spos[ord[k]] = k;
+      std::vector<long long> wref(E);
+      for (long long k = 0; k < E; ++k) wref[k] = spos[ref[k]];
+      std::vector<int> epost(E), epre(E), edly(E);
+      std::vector<double> w(E);
+      std::vector<unsigned> roff((size_t)npre * nd1, 0), cptr(npost + 1, 0), ceid(E);
+      for (long long k = 0; k < E; ++k) {
+        const long long e = ord[k];
+        epost[k] = pj.post[e];
+        epre[k] = pj.pre[e];
+        edly[k] = dly(e);
+        w[k] = pj.weight[e];
+        roff[(size_t)epre[k] * nd1 + (edly[k] - pdmin) + 1]++;
+        cptr[epost[k] + 1]++;
+      }
+      {  // offsets: row i, delay slot t -> first edge; slot nd1-1 is the row end
+        unsigned acc = 0;
+        for (int a = 0; a < npre; ++a)
+          for (int t = 0; t < nd1; ++t) {
+            const unsigned c = t + 1 < nd1 ? roff[(size_t)a * nd1 + t + 1] : 0;
+            roff[(size_t)a * nd1 + t] = acc;
+            acc += c;
+          }
+        for (int b = 0; b < npost; ++b) cptr[b + 1] += cptr[b];
+        std::vector<unsigned> pos(cptr.begin(), cptr.end() - 1);
+        for (long long k = 0; k < E; ++k) ceid[pos[epost[k]]++] = (unsigned)k;
+      }
+      StdpDev S;
+      memset(&S, 0, sizeof(S));
+      S.pre_pop = pj.pre_pop;
+      S.post_pop = pj.post_pop;
+      S.pre_gid0 = net->pops[pj.pre_pop].gid0;
+      S.post_gid0 = net->pops[pj.post_pop].gid0;
+      S.n_pre = npre;
+      S.n_post = npost;
+      S.d_min = pdmin;
+      S.d_max = pdmax;
+      S.Lu = pdmax + 2;
+      S.shift = d->channels[pj.channel].fixed_shift;
+      S.decay_plus = pj.stdp.decay_plus;
+      S.decay_minus = pj.stdp.decay_minus;
+      S.lam = pj.stdp.lambda_;
+      S.alpha = pj.stdp.alpha;
+      S.mup = pj.stdp.mu_plus;
+      S.mum = pj.stdp.mu_minus;
+      S.Wmax = pj.stdp.Wmax;
+      int *dpost, *dpre, *ddly;
+      unsigned *droff, *dcptr, *dceid;
+      double *dw, *dU, *dV;
+      long long* dpacc;
+      CHK(upload(net, &dpost, epost.data(), epost.size()));
+      CHK(upload(net, &dpre, epre.data(), epre.size()));
+      CHK(upload(net, &ddly, edly.data(), edly.size()));
+      CHK(upload(net, &droff, roff.data(), roff.size()));
+      CHK(upload(net, &dcptr, cptr.data(), cptr.size()));
+      CHK(upload(net, &dceid, ceid.data(), ceid.size()));
+      CHK(upload(net, &dw, w.data(), w.size()));
+      CHK(dmalloc(&dU, (size_t)S.Lu * npre));
+      net->owned.push_back(dU);
+      CHK(dmalloc(&dV, (size_t)npost));
+      net->owned.push_back(dV);
+      CHK(dmalloc(&dpacc, (size_t)npost));
+      net->owned.push_back(dpacc);
+      S.e_post = dpost;
+      S.e_pre = dpre;
+      S.e_dly = ddly;
+      S.roff = droff;
+      S.csc_ptr = dcptr;
+      S.csc_eid = dceid;
+      S.w = dw;
+      S.U = dU;
+      S.V = dV;
+      S.pacc = dpacc;
+      net->chans[pj.channel].pacc = dpacc;
+      net->proj_stdp.back() = (int)net->stdp.size();
+      net->stdp.push_back(S);
+      net->w0.push_back(w);
+      net->w_ref.push_back(wref);
+      dmin = std::min(dmin, pdmin);
+      dmax = std::max(dmax, pdmax);
+      continue;
+    }
     if (kind != BS_CH_COUNT && kind != BS_CH_FIXED) BAD("projection channel must be COUNT or FIXED");
     if (kind == BS_CH_FIXED && !pj.weight) BAD("FIXED projection needs per-edge weights");
     if (pj.n_edges < 0 || pj.n_edges >= (1ll << 32)) BAD("projection edge count out of range");
@@ -959,6 +1219,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
 
   // ---- window length: W = d_min + 1 (holder lag) capped by the LDS budget
   int W = dmin + 1;
+  if (!net->stdp.empty()) W = 1;  // plastic updates interleave with every step
   const int maxw = d->max_window > 0 ? d->max_window : 32;
   if (W > maxw) W = maxw;
   if (W < 1) W = 1;
@@ -1045,6 +1306,12 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   CHK(dmalloc(&net->d_rec_count, 1));
   CHK(dmalloc(&net->d_flags, 1));
   CHK(dmalloc(&net->d_base, 1));
+  if (!net->stdp.empty()) {
+    CHK(dmalloc(&net->d_spk_hist, (size_t)net->Ls * gid));
+    CHK(dmalloc(&net->d_stdp, net->stdp.size()));
+    HCHK(hipMemcpy(net->d_stdp, net->stdp.data(), sizeof(StdpDev) * net->stdp.size(),
+                   hipMemcpyHostToDevice));
+  }
   CHK(dmalloc(&net->d_vm_col, gid));
   {  // spike record list: worst case every neuron fires every step of K windows
     const long long per_win = std::max<long long>(1, gid) * W;
@@ -1099,6 +1366,15 @@ static int upload_state(bs_net* net) {
   if (gid) HIPCHK(hipMemcpy(net->d_ref, net->ref0.data(), 4 * gid, hipMemcpyHostToDevice));
   HIPCHK(hipMemset(net->d_sb_count, 0, 4 * (size_t)net->Ls * net->pops.size()));
   HIPCHK(hipMemset(net->d_flags, 0, sizeof(int)));
+  for (size_t q = 0; q < net->stdp.size(); ++q) {  // traces 0, initial weights
+    const StdpDev& S = net->stdp[q];
+    HIPCHK(hipMemset(S.U, 0, 8 * (size_t)S.Lu * std::max(1, S.n_pre)));
+    HIPCHK(hipMemset(S.V, 0, 8 * (size_t)std::max(1, S.n_post)));
+    HIPCHK(hipMemset(S.pacc, 0, 8 * (size_t)std::max(1, S.n_post)));
+    if (!net->w0[q].empty())
+      HIPCHK(hipMemcpy(S.w, net->w0[q].data(), 8 * net->w0[q].size(), hipMemcpyHostToDevice));
+  }
+  if (net->d_spk_hist) HIPCHK(hipMemset(net->d_spk_hist, 0, (size_t)net->Ls * gid));
   HIPCHK(hipDeviceSynchronize());
   net->step = 0;
   return BS_OK;
@@ -1123,6 +1399,7 @@ static void launch_window(bs_net* net, int n0, int W, long long vm_step0,
   WinArgs a;
   a.base_ptr = base_ptr;
   a.widx = widx;
+  a.spk_hist = net->d_spk_hist;
   a.tiles = net->d_tiles;
   a.pops = net->d_pops;
   a.chans = net->d_chans;
@@ -1150,6 +1427,26 @@ static void launch_window(bs_net* net, int n0, int W, long long vm_step0,
   a.vm_step0 = vm_step0;
   hipLaunchKernelGGL(k_window<T>, dim3((unsigned)net->tiles.size()), dim3(256), net->lds_bytes,
                      net->stream, a);
+}
+
+static void launch_stdp(bs_net* net, int n, const long long* base_ptr, int widx) {
+  StdpArgs a;
+  a.P = net->d_stdp;
+  a.n_proj = (int)net->stdp.size();
+  a.spk_hist = net->d_spk_hist;
+  a.sb = net->d_sb;
+  a.sb_count = net->d_sb_count;
+  a.Ntot = net->Ntot;
+  a.Ls = net->Ls;
+  a.n_pops = (int)net->pops.size();
+  a.n = n;
+  a.base_ptr = base_ptr;
+  a.widx = widx;
+  long long nmax = 1;
+  for (const StdpDev& S : net->stdp) nmax = std::max<long long>(nmax, std::max(S.n_pre, S.n_post));
+  const unsigned tb = (unsigned)std::min<long long>((nmax + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_stdp_traces, dim3(tb), dim3(256), 0, net->stream, a);
+  hipLaunchKernelGGL(k_stdp_events, dim3(512), dim3(256), 0, net->stream, a);
 }
 
 int bs_net_set_timing(bs_net* net, int on) {
@@ -1201,6 +1498,7 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
         hipGraph_t g;
         HIPCHK(hipStreamBeginCapture(net->stream, hipStreamCaptureModeThreadLocal));
         for (int w = 0; w < nwin; ++w) {
+          if (!net->stdp.empty()) launch_stdp(net, 0, net->d_base, w);
           if (net->tsize == 8)
             launch_window<double>(net, 0, net->W, 0, net->d_base, w);
           else
@@ -1222,6 +1520,7 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
         const long long n0 = c0 + (long long)w * net->W;
         const int Wc = (int)std::min<long long>(net->W, c0 + chunk - n0);
         if (net->timing) HIPCHK(hipEventRecord(evs[2 * w], net->stream));
+        if (!net->stdp.empty()) launch_stdp(net, (int)n0, nullptr, 0);
         if (net->tsize == 8)
           launch_window<double>(net, (int)n0, Wc, c0);
         else
@@ -1340,10 +1639,16 @@ int bs_net_get_state(bs_net* net, int32_t pop, int32_t var, double* out) {
 }
 
 int bs_net_get_weights(bs_net* net, int32_t proj, double* out) {
-  (void)proj;
-  (void)out;
-  if (!net) return fail(BS_EINVAL, "bs_net_get_weights: null handle");
-  return fail(BS_EINVAL, "bs_net_get_weights: projection is not plastic");
+  if (!net || !out) return fail(BS_EINVAL, "bs_net_get_weights: null argument");
+  if (proj < 0 || proj >= (int)net->proj_stdp.size() || net->proj_stdp[proj] < 0)
+    return fail(BS_EINVAL, "bs_net_get_weights: projection is not plastic");
+  const int q = net->proj_stdp[proj];
+  const size_t E = net->w0[q].size();
+  std::vector<double> w(E);
+  HIPCHK(hipStreamSynchronize(net->stream));
+  if (E) HIPCHK(hipMemcpy(w.data(), net->stdp[q].w, 8 * E, hipMemcpyDeviceToHost));
+  for (size_t k = 0; k < E; ++k) out[k] = w[net->w_ref[q][k]];
+  return BS_OK;
 }
 
 int bs_net_last_timing(bs_net* net, double* deliver_ms, double* update_ms, double* total_ms,

@@ -444,12 +444,40 @@ class Simulator:
         T = u.scalar(duration, 'time')
         return int(math.ceil(T / self.dt))   # len(arange(0, T, dt))
 
+    def get_weights(self, proj):
+        """Current weights of a plastic projection, in the reference's CSR
+        (stable sorted-by-pre) edge order (event_plastic.py:293-296)."""
+        if not isinstance(proj, PlasticProjection):
+            raise TypeError('get_weights needs a plastic projection handle')
+        eng = self._ensure_engine()
+        return eng.weights(eng.desc.proj_index[id(proj)], proj.n_edges)
+
     def _run_window(self, duration) -> SimulationResult:
         eng = self._engine
         T = u.scalar(duration, 'time')
         n = self.n_steps_of(duration)
-        vm, (gids, steps) = eng.run(n)
-        return self._result(eng.desc, n, T, vm, gids, steps)
+        weights = {}
+        if self._weight_taps:
+            # weight taps read the post-update weights every step (simulator.py:1868)
+            taps = list(self._weight_taps.items())
+            traj = {rid: np.zeros((n, p.n_edges)) for rid, p in taps}
+            vms, gl, sl = [], [], []
+            for t in range(n):
+                vm1, (g1, s1) = eng.run(1)
+                vms.append(vm1)
+                gl.append(g1)
+                sl.append(s1)
+                for rid, p in taps:
+                    traj[rid][t] = eng.weights(eng.desc.proj_index[id(p)], p.n_edges)
+            vm = np.concatenate(vms) if vms else np.zeros((0, eng.desc.n_taps))
+            gids = np.concatenate(gl) if gl else np.zeros(0, np.int32)
+            steps = np.concatenate(sl) if sl else np.zeros(0, np.int32)
+            weights = {rid: u.Quantity(traj[rid], u.pA) for rid in traj}
+        else:
+            vm, (gids, steps) = eng.run(n)
+        res = self._result(eng.desc, n, T, vm, gids, steps)
+        res._weights = weights
+        return res
 
     def _result(self, desc, n, T, vm, gids, steps) -> SimulationResult:
         step0 = self._base_step

@@ -55,6 +55,7 @@ def lib():
             'orc_exp': (C.c_double, [C.c_double]),
             'orc_log': (C.c_double, [C.c_double]),
             'orc_pow': (C.c_double, [C.c_double, C.c_double]),
+            'orc_stdp_edge': (C.c_double, [C.c_double] * 5 + [C.c_void_p]),
         }
         for k, (res, args) in sig.items():
             f = getattr(h, k)
@@ -89,6 +90,16 @@ def log(x):
 
 def pow(x, y):  # noqa: A001
     return lib().orc_pow(float(x), float(y))
+
+
+def stdp_edge(w, pre_spike, post_spike, pre_trace, post_trace, Wmax=100., lambda_=0.01,
+              alpha=1., mu_plus=1., mu_minus=1.):
+    """One edge's stdp_synapse update with the given spikes and traces."""
+    from brainpy.state_amd import _lib as L
+    S = L.Stdp()
+    S.Wmax, S.lambda_, S.alpha, S.mu_plus, S.mu_minus = Wmax, lambda_, alpha, mu_plus, mu_minus
+    return lib().orc_stdp_edge(float(w), float(pre_spike), float(post_spike), float(pre_trace),
+                               float(post_trace), C.addressof(S))
 
 
 class Neuron:

@@ -168,6 +168,12 @@ struct OrcProj {
   std::vector<int> post, delay;      // per CSR edge
   std::vector<double> w;             // per CSR edge weight (COUNT: channel weight)
   std::vector<int64_t> q;            // FIXED: rint(w * 2^F)
+  // stdp_synapse (plastic projections)
+  bool plastic = false;
+  bs_stdp stdp;
+  std::vector<int> pre;              // per CSR edge: pre local index
+  std::vector<double> tr;            // per CSR edge: pre trace of its delayed spike train
+  std::vector<double> V;             // per post neuron: post trace
 };
 
 struct orc_net {
@@ -268,6 +274,13 @@ extern "C" int orc_create(const bs_net_desc* d, orc_net** out) {
       const double w = (P.kind == BS_CH_COUNT) ? ch.weight : pj.weight[e];
       P.w.push_back(w);
       if (P.kind == BS_CH_FIXED) P.q.push_back((int64_t)llrint(ldexp(w, ch.fixed_shift)));
+      P.pre.push_back(pj.pre[e]);
+    }
+    if (pj.plastic) {
+      P.plastic = true;
+      P.stdp = pj.stdp;
+      P.tr.assign(pj.n_edges, 0.0);
+      P.V.assign(d->pops[pj.post_pop].n, 0.0);
     }
     net->chan_proj[pj.channel] = q;
     net->projs.push_back(std::move(P));
@@ -537,6 +550,55 @@ static bool update(const std::vector<double>& P, double* st, int& r, double w_ex
 }
 }  // namespace aeif
 
+// One edge's stdp_synapse update (stdp_synapse.py:163-179): facilitation
+// gated on the post spike with K+ = pre trace minus this step's pre spike,
+// then depression gated on the pre spike with K- = post trace minus this
+// step's post spike; each clamps to [0, Wmax] in Wmax-normalised form.
+double orc_stdp_edge(double w, double pre_spike, double post_spike, double pre_trace,
+                     double post_trace, const bs_stdp* S) {
+  const double W = S->Wmax;
+  const double kplus = pre_trace - pre_spike;
+  const double kminus = post_trace - post_spike;
+  if (post_spike > 0) {
+    const double nw = w / W + S->lambda_ * orc_pow(1.0 - w / W, S->mu_plus) * kplus;
+    w = nw < 1.0 ? nw * W : W;
+  }
+  if (pre_spike > 0) {
+    const double nw = w / W - S->alpha * S->lambda_ * orc_pow(w / W, S->mu_minus) * kminus;
+    w = nw > 0.0 ? nw * W : 0.0;
+  }
+  return w;
+}
+
+// stdp_synapse on EventPlasticProj for one step n (event_plastic.py:454-536,
+// stdp_synapse.py:163-179): traces decay-then-add (the pre trace of each
+// edge follows that edge's delayed spike train), facilitation where the post
+// neuron fired in the previous step, then depression where the delayed pre
+// spike arrives, then the new weights of the arriving edges are summed per
+// post neuron (2^-F fixed point, the engine's order-free sum).
+template <typename T>
+static void stdp_step(orc_net* net, OrcProj& P, long long n, std::vector<T>& y) {
+  const bs_stdp& S = P.stdp;
+  const int Lh = net->Lh;
+  const int F = net->chans[P.channel].fixed_shift;
+  const std::vector<double>* post_prev =
+      n >= 1 ? &net->hist[P.post_pop][(int)((n - 1) % Lh)] : nullptr;
+  for (size_t j = 0; j < P.V.size(); ++j)
+    P.V[j] = P.V[j] * S.decay_minus + (post_prev ? (*post_prev)[j] : 0.0);
+  std::vector<int64_t> acc(y.size(), 0);
+  for (size_t e = 0; e < P.w.size(); ++e) {
+    const long long src = n - 1 - P.delay[e];
+    const double x = src >= 0 ? net->hist[P.pre_pop][(int)(src % Lh)][P.pre[e]] : 0.0;
+    P.tr[e] = P.tr[e] * S.decay_plus + x;
+    const int j = P.post[e];
+    const double post_fired = post_prev ? (*post_prev)[j] : 0.0;
+    const double w = orc_stdp_edge(P.w[e], x, post_fired, P.tr[e], P.V[j], &S);
+    if (x > 0) acc[j] += (int64_t)llrint(ldexp(w, F));
+    P.w[e] = w;
+  }
+  for (size_t j = 0; j < y.size(); ++j) y[j] = (T)ldexp((double)acc[j], -F);
+}
+
 // -------------------------------------------------------------------- run
 template <typename T>
 static int run_t(orc_net* net, int64_t n_steps, double* vm_out) {
@@ -568,10 +630,14 @@ static int run_t(orc_net* net, int64_t n_steps, double* vm_out) {
     // (n + d_e); the slot of step n is then read. For a homogeneous delay this
     // is InputDelay followed by the CSR event matmul (delay.py:114-128,
     // projections.py:56-59), accumulated in the same sequential order.
-    for (const OrcProj& P : net->projs) {
+    for (OrcProj& P : net->projs) {
       const int c = P.channel;
       std::vector<T>& y = yin[c];
       const int npre = net->pops[P.pre_pop].n;
+      if (P.plastic) {
+        stdp_step(net, P, n, y);
+        continue;
+      }
       if (n >= 1) {
         const std::vector<double>& x = net->hist[P.pre_pop][(int)((n - 1) % Lh)];
         for (int i = 0; i < npre; ++i) {

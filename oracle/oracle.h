@@ -44,6 +44,11 @@ int orc_get_weights(orc_net* net, int32_t proj, double* out);
 int orc_neuron_step(int model, const double* params, double* state, int* ref, double in0,
                     double in1, int dtype);
 
+/* One stdp_synapse edge update given this step's spikes and the traces
+ * (which include this step's own spikes); returns the new weight. */
+double orc_stdp_edge(double w, double pre_spike, double post_spike, double pre_trace,
+                     double post_trace, const bs_stdp* S);
+
 /* Elementwise primitives (the definitions the device must reproduce). */
 void orc_threefry2x32(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t c1, uint32_t* o0,
                       uint32_t* o1);

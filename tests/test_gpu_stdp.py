@@ -1,0 +1,91 @@
+"""GPU parity of stdp_synapse projections: raster, V_m and final weights
+against the oracle's literal per-step, per-edge restatement."""
+import numpy as np
+import pytest
+
+import oracle
+from brainpy.state_amd import (Simulator, aeif_cond_alpha, fixed_indegree, iaf_psc_exp,
+                               poisson_generator, spike_recorder, stdp_synapse, voltmeter)
+from brainpy.state_amd import units as u
+from brainpy.state_amd.init import Uniform
+
+pytestmark = pytest.mark.gpu
+
+
+def _plastic_net(model=iaf_psc_exp, n=400, delay=('uniform', 1.0, 3.0), dtype='float64',
+                 mu=(0.0, 1.0)):
+    sim = Simulator(dt=0.1 * u.ms, dtype=dtype)
+    cond = model is aeif_cond_alpha
+    kw = {} if cond else dict(V_initializer=Uniform(-70., -56., seed=5))
+    e = sim.create(model, n, **kw)
+    i = sim.create(model, n // 4, **kw)
+    noise = sim.create(poisson_generator, rate=14000. * u.Hz)
+    rt = dict(receptor_type=1) if cond else {}
+    rti = dict(receptor_type=2) if cond else {}
+    wE, wI = (3.0, 6.0) if cond else (40.0, -160.0)
+    sim.connect(noise, e + i, weight=wE, delay=1.0, **rt)
+    if isinstance(delay, tuple):
+        # axonal delays: one per pre neuron, whole steps in [lo, hi] ms
+        rng = np.random.default_rng(11)
+        delay = np.round(rng.uniform(delay[1], delay[2], n), 1) * u.ms
+    syn = stdp_synapse(weight=wE, tau_plus=20., tau_minus=20., lambda_=0.05,
+                       alpha=1.1, mu_plus=mu[0], mu_minus=mu[1], Wmax=3 * wE)
+    proj = sim.connect(e, e, rule=fixed_indegree(20), synapse=syn, delay=delay, seed=3, **rt)
+    sim.connect(e, i, weight=wE, delay=1.5, rule=fixed_indegree(20), seed=4, **rt)
+    sim.connect(i, e + i, weight=wI, delay=1.0, rule=fixed_indegree(5), seed=5, **rti)
+    sim.connect(e, sim.create(spike_recorder))
+    sim.connect(i, sim.create(spike_recorder))
+    vm = sim.create(voltmeter)
+    sim.connect(vm, e[:4])
+    return sim, proj
+
+
+def _compare(sim, proj, n_steps, rtol=1e-12):
+    desc = sim.build_desc()
+    eng = sim._ensure_engine()
+    eng.reset()
+    vm_g, (g_g, s_g) = eng.run(n_steps)
+    w_g = sim.get_weights(proj)
+    orc = oracle.OracleNet(desc)
+    vm_o, g_o, s_o = orc.run(n_steps)
+    w_o = orc.weights(desc.proj_index[id(proj)], proj.n_edges)
+    assert len(g_o) > 200
+    assert len(g_g) == len(g_o), f'spike count differs: gpu {len(g_g)} oracle {len(g_o)}'
+    assert np.array_equal(s_g, s_o) and np.array_equal(g_g, g_o)
+    np.testing.assert_allclose(vm_g, vm_o, rtol=rtol, atol=0)
+    w0 = np.asarray(proj.weight, np.float64)
+    assert np.any(w_o != w0), 'no plasticity happened'
+    np.testing.assert_allclose(w_g, w_o, rtol=rtol, atol=0)
+    return w_g
+
+
+def test_stdp_iaf_psc_exp_per_edge_delays():
+    sim, proj = _plastic_net()
+    _compare(sim, proj, 2000)
+
+
+@pytest.mark.parametrize('mu', [(1.0, 1.0), (0.4, 0.4)])
+def test_stdp_multiplicative_and_fractional_mu(mu):
+    sim, proj = _plastic_net(n=200, delay=1.0, mu=mu)
+    _compare(sim, proj, 1500)
+
+
+def test_stdp_aeif_cond_alpha():
+    sim, proj = _plastic_net(model=aeif_cond_alpha, n=200, delay=('uniform', 1.0, 5.0))
+    _compare(sim, proj, 1000)
+
+
+def test_stdp_chunks_and_continue():
+    """Two runs back to back equal one long run (traces and weights carry over)."""
+    sim, proj = _plastic_net(n=200, delay=1.0)
+    desc = sim.build_desc()
+    eng = sim._ensure_engine()
+    eng.reset()
+    eng.run(700)
+    vm_b, (g_b, s_b) = eng.run(800)
+    orc = oracle.OracleNet(desc)
+    orc.run(700)
+    vm_o, g_o, s_o = orc.run(800)
+    assert np.array_equal(g_b, g_o) and np.array_equal(s_b, s_o)
+    np.testing.assert_allclose(sim.get_weights(proj),
+                               orc.weights(desc.proj_index[id(proj)], proj.n_edges), rtol=1e-12)
