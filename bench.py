@@ -39,10 +39,12 @@ def main():
     ap.add_argument('--steps', type=int, default=100000)
     ap.add_argument('--warmup', type=int, default=2000)
     ap.add_argument('--dtype', default='float64', choices=['float64', 'float32'])
-    ap.add_argument('--cpu-steps', type=int, default=3000)
+    ap.add_argument('--cpu-steps', type=int, default=20000)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--profile-only', action='store_true',
                     help='run warmup + timed steps only (for rocprofv3)')
+    ap.add_argument('--direct', action='store_true',
+                    help='launch every window directly instead of replaying the captured graph')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported '
@@ -58,6 +60,8 @@ def main():
     eng = sim._ensure_engine()
     log(f'built Brunel-11250 in {time.time() - t0:.1f}s')
     eng.reset()
+    if args.direct:
+        eng.set_timing(True)
     eng.run(args.warmup)
     log(f'warmup {args.warmup} steps done')
 

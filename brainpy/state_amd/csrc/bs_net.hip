@@ -308,11 +308,20 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
     a.n0 = (int)b + a.widx * a.Wmax;
     a.vm_step0 = b;
   }
-  const Tile tl = a.tiles[blockIdx.x];
+  // XCD-aware tile order: the dispatcher deals blocks round-robin over the
+  // eight XCDs (block b runs on XCD b % 8), so each XCD is given one
+  // contiguous run of tiles. A spike's edge row is laid out tile-major, and
+  // every XCD then fetches only its own slice of the row into its L2.
+  const int tix = [] {
+    const int G = (int)gridDim.x, x = (int)(blockIdx.x & 7u), k = (int)(blockIdx.x >> 3);
+    const int q = G >> 3, rem = G & 7;
+    return x * q + min(x, rem) + k;
+  }();
+  const Tile tl = a.tiles[tix];
   const DevPop& gpop = a.pops[tl.pop];
   const int tid = (int)threadIdx.x;
   const int TS = a.TS;
-  const int tile_in_pop = (int)blockIdx.x - gpop.tile0;
+  const int tile_in_pop = tix - gpop.tile0;
   const int W = a.W, Wm = a.Wmax;
   const int ncw = gpop.n_cw, nfx = gpop.n_fx, ngp = gpop.gp_count, nch = gpop.ch_count;
   const int SP = a.max_span + 1;
@@ -333,7 +342,7 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   T* inp = (T*)(lds + Lo.inp);  // [nrec][Wm][TS] summed inputs per step and lane
 
   // ---- 0. stage
-  if (blockIdx.x == 0) {  // clear the bucket counts of the next window's steps
+  if (tix == 0) {  // clear the bucket counts of the next window's steps
     for (int t = tid; t < Wm * a.n_pops; t += 256) {
       const int st = a.n0 + W + t / a.n_pops;
       a.sb_count[(st % a.Ls) * a.n_pops + t % a.n_pops] = 0u;
@@ -1477,8 +1486,11 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
   HIPCHK(hipEventRecord(ev_all0, net->stream));
   long long done = 0;
   int rc = BS_OK;
-  double state_bytes = 0;
-  for (const DevPop& p : net->pops) state_bytes += (double)p.n * (2.0 * p.n_state * net->tsize + 8.0);
+  // Algorithmic update bytes per network step: every neuron reads and writes
+  // its state variables and its int32 refractory counter and reads one
+  // arrival slot (4 B count), plus 8 B per recorded V_m sample.
+  double state_bytes = 8.0 * net->n_vm_dev;
+  for (const DevPop& p : net->pops) state_bytes += (double)p.n * (2.0 * p.n_state * net->tsize + 8.0 + 4.0);
   while (done < n_steps) {
     const long long chunk = std::min<long long>(chunk_max, n_steps - done);
     const long long c0 = net->step;
@@ -1514,7 +1526,7 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
                             net->stream));
       HIPCHK(hipGraphLaunch(net->graph, net->stream));
       net->n_update += nwin;
-      net->b_update += state_bytes * nwin;
+      net->b_update += state_bytes * chunk;
     } else {
       for (int w = 0; w < nwin; ++w) {
         const long long n0 = c0 + (long long)w * net->W;
@@ -1528,7 +1540,7 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
         if (net->timing) HIPCHK(hipEventRecord(evs[2 * w + 1], net->stream));
         HIPCHK(hipGetLastError());
         net->n_update++;
-        net->b_update += state_bytes;
+        net->b_update += state_bytes * Wc;
       }
     }
     unsigned long long cnt = 0;
