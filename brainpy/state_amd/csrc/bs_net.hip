@@ -21,6 +21,7 @@
 //     fired in the previous step, k_stdp_pre applies depression to arriving
 //     edges and deposits their weights as exact 2^-F fixed-point integers.
 #include <hip/hip_runtime.h>
+#include <hipcub/device/device_radix_sort.hpp>
 #include <stdint.h>
 #include <stdio.h>
 #include <string.h>
@@ -118,7 +119,8 @@ struct WinArgs {
   unsigned char* spk_hist;  // [Ls][Ntot] spike flags (plastic networks), or null
   int max_span;        // longest gathered step range, LDS prefix size
   int TS;              // neurons per tile
-  int2* rec;
+  unsigned long long* rec;  // (step - vm_step0) << gid_bits | gid
+  int gid_bits;
   unsigned long long* rec_count;
   long long rec_cap;
   int* flags;
@@ -585,7 +587,8 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
       a.sb[(long long)(step % a.Ls) * a.Ntot + gpop.gid0 + (long long)base[2 * w] + k] = (uint32_t)jj;
       const unsigned long long idx = base[2 * w + 1] + (unsigned long long)k;
       if ((long long)idx < a.rec_cap)
-        a.rec[idx] = make_int2(gpop.gid0 + jj, step);
+        a.rec[idx] = ((unsigned long long)(step - a.vm_step0) << a.gid_bits) |
+                     (unsigned long long)(gpop.gid0 + jj);
       else
         atomicOr(a.flags, FLAG_OVERFLOW);
     }
@@ -745,7 +748,11 @@ struct bs_net {
   int* d_ref = nullptr;
   uint32_t* d_sb = nullptr;
   uint32_t* d_sb_count = nullptr;
-  int2* d_rec = nullptr;
+  unsigned long long* d_rec = nullptr;      // record keys of the chunk, emit order
+  unsigned long long* d_rec_sorted = nullptr;
+  void* d_sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  int gid_bits = 1, step_bits = 1;
   unsigned long long* d_rec_count = nullptr;
   long long rec_cap = 0;
   int* d_flags = nullptr;
@@ -795,7 +802,7 @@ static void free_net(bs_net* net) {
     if (p) (void)hipFree(p);
   };
   F(net->d_tiles); F(net->d_pops); F(net->d_chans); F(net->d_gproj); F(net->d_tables);
-  F(net->d_state); F(net->d_ref); F(net->d_sb); F(net->d_sb_count); F(net->d_rec);
+  F(net->d_state); F(net->d_ref); F(net->d_sb); F(net->d_sb_count); F(net->d_rec); F(net->d_rec_sorted); F(net->d_sort_tmp);
   F(net->d_rec_count); F(net->d_flags); F(net->d_vm_col); F(net->d_vm);
   for (void* p : net->owned) F(p);
   F(net->d_base);
@@ -1329,7 +1336,17 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     if (k > 512) k = 512;
     net->rec_cap = per_win * k;
     CHK(dmalloc(&net->d_rec, net->rec_cap));
+    CHK(dmalloc(&net->d_rec_sorted, net->rec_cap));
     net->vm_cap_steps = k * W;
+    while ((1ll << net->gid_bits) < gid) ++net->gid_bits;
+    while ((1ll << net->step_bits) < net->vm_cap_steps) ++net->step_bits;
+    HCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, net->sort_tmp_bytes, net->d_rec, net->d_rec_sorted,
+                                           (int)net->rec_cap, 0, net->gid_bits + net->step_bits));
+    {
+      void* t = nullptr;
+      HCHK(hipMalloc(&t, std::max<size_t>(net->sort_tmp_bytes, 256)));
+      net->d_sort_tmp = t;
+    }
     CHK(dmalloc(&net->d_vm, (size_t)std::max(1, ncol) * net->vm_cap_steps));
   }
   HCHK(hipMemcpy(net->d_tiles, net->tiles.data(), sizeof(Tile) * net->tiles.size(), hipMemcpyHostToDevice));
@@ -1427,6 +1444,7 @@ static void launch_window(bs_net* net, int n0, int W, long long vm_step0,
   a.max_span = net->max_span;
   a.TS = net->TS;
   a.rec = net->d_rec;
+  a.gid_bits = net->gid_bits;
   a.rec_count = net->d_rec_count;
   a.rec_cap = net->rec_cap;
   a.flags = net->d_flags;
@@ -1477,7 +1495,7 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
   net->n_events = 0;
   const int ntaps = (int)net->vm_tap_col.size();
   const long long chunk_max = net->vm_cap_steps;  // = K * W
-  std::vector<int2> hrec;
+  std::vector<unsigned long long> hrec;
   std::vector<double> hvm;
   hipEvent_t ev_all0, ev_all1;
   HIPCHK(hipEventCreate(&ev_all0));
@@ -1567,10 +1585,22 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
         net->t_update += a;
       }
     }
+    // the chunk's records sorted by (step, gid) on the device, then decoded
     hrec.resize(cnt);
-    if (cnt) HIPCHK(hipMemcpy(hrec.data(), net->d_rec, sizeof(int2) * cnt, hipMemcpyDeviceToHost));
+    if (cnt) {
+      size_t tb = net->sort_tmp_bytes;
+      HIPCHK(hipcub::DeviceRadixSort::SortKeys(net->d_sort_tmp, tb, net->d_rec, net->d_rec_sorted, (int)cnt,
+                                               0, net->gid_bits + net->step_bits, net->stream));
+      HIPCHK(hipMemcpyAsync(hrec.data(), net->d_rec_sorted, sizeof(unsigned long long) * cnt,
+                            hipMemcpyDeviceToHost, net->stream));
+      HIPCHK(hipStreamSynchronize(net->stream));
+    }
+    const unsigned long long gmask = (1ull << net->gid_bits) - 1;
+    net->out_gid.reserve(net->out_gid.size() + cnt);
+    net->out_step.reserve(net->out_step.size() + cnt);
     for (unsigned long long i = 0; i < cnt; ++i) {
-      const int g = hrec[i].x;
+      const int g = (int)(hrec[i] & gmask);
+      const int st = (int)(c0 + (long long)(hrec[i] >> net->gid_bits));
       const long long row = net->outdeg[g];
       net->n_events += row;
       // gather: the spike's bucket entry, its row's segment bounds, 4 B per edge
@@ -1578,7 +1608,7 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
       net->b_update += 8.0;  // record entry
       if (net->pop_record[net->gid_pop[g]]) {
         net->out_gid.push_back(g);
-        net->out_step.push_back(hrec[i].y);
+        net->out_step.push_back(st);
       }
     }
     if (vm_out && ntaps > 0) {
@@ -1601,15 +1631,6 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
   for (hipEvent_t e : evs) (void)hipEventDestroy(e);
   (void)hipEventDestroy(ev_all0);
   (void)hipEventDestroy(ev_all1);
-  {  // sort recorded spikes by (step, gid)
-    std::vector<long long> key(net->out_gid.size());
-    for (size_t i = 0; i < key.size(); ++i) key[i] = ((long long)net->out_step[i] << 32) | (uint32_t)net->out_gid[i];
-    std::sort(key.begin(), key.end());
-    for (size_t i = 0; i < key.size(); ++i) {
-      net->out_step[i] = (int32_t)(key[i] >> 32);
-      net->out_gid[i] = (int32_t)(key[i] & 0xFFFFFFFF);
-    }
-  }
   return rc;
 }
 
