@@ -10,7 +10,7 @@ import ctypes as C
 import os
 
 __all__ = ['lib', 'EngineUnavailable', 'EngineError', 'check', 'BS_NPARAM', 'BS_NSTATE',
-           'Population', 'Channel', 'Projection', 'Stdp', 'NetDesc', 'LIB_PATH']
+           'Population', 'Channel', 'Projection', 'Stdp', 'NetDesc', 'BpttDesc', 'LIB_PATH']
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libbrainspike_hip.so')
@@ -76,6 +76,16 @@ class NetDesc(C.Structure):
                 ('n_tables', C.c_int64)]
 
 
+class BpttDesc(C.Structure):
+    _fields_ = ([(k, C.c_int32) for k in ('abi_version', 'n_in', 'n_exc', 'n_inh', 'batch', 'n_steps',
+                                          'delay_steps', 'delay_in_steps', 'checkpoint', 'loss',
+                                          'ref_steps', '_pad')] +
+                [(k, C.c_double) for k in ('dt_ms', 'E_L', 'V_th', 'V_reset', 'I_e', 'P22', 'P21ex',
+                                           'P21in', 'P11ex', 'P11in', 'P20', 'surrogate_alpha')])
+
+
+BS_LOSS_RATE_MSE, BS_LOSS_MEMBRANE_MSE = 0, 1
+
 _P = C.c_void_p
 _SIGS = {
     'bs_abi_version': (C.c_int, []),
@@ -94,6 +104,11 @@ _SIGS = {
     'bs_net_last_timing': (C.c_int, [_P] + [C.POINTER(C.c_double)] * 3 + [C.POINTER(C.c_int64)] * 2),
     'bs_net_last_bytes': (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                     C.POINTER(C.c_int64)]),
+    'bs_bptt_create': (C.c_int, [C.POINTER(BpttDesc), C.POINTER(_P)]),
+    'bs_bptt_destroy': (C.c_int, [_P]),
+    'bs_bptt_forward': (C.c_int, [_P, _P, _P, _P, _P, _P, C.POINTER(C.c_double)]),
+    'bs_bptt_backward': (C.c_int, [_P, _P, _P]),
+    'bs_bptt_spikes': (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_float)]),
     'bs_debug_math': (C.c_int, [C.c_int32, C.c_int64, C.POINTER(C.c_double),
                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     'bs_debug_threefry': (C.c_int, [C.c_int64] + [C.POINTER(C.c_uint32)] * 6),

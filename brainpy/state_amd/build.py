@@ -5,7 +5,7 @@ import os
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = [os.path.join(HERE, 'csrc', 'bs_net.hip')]
+SRC = [os.path.join(HERE, 'csrc', 'bs_net.hip'), os.path.join(HERE, 'csrc', 'bs_bptt.hip')]
 OUT = os.path.join(HERE, 'libbrainspike_hip.so')
 
 # -ffp-contract=off: every arithmetic op is one IEEE-rounded op (no FMA
@@ -13,12 +13,14 @@ OUT = os.path.join(HERE, 'libbrainspike_hip.so')
 # -packed-fp32-ops off: no packed FP32 instructions are generated.
 FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared', '-ffp-contract=off',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
+# rocBLAS: the dense transposed-delivery GEMM of the BPTT backward pass
+LIBS = ['-L/opt/rocm/lib', '-lrocblas', '-Wl,-rpath,/opt/rocm/lib']
 
 
 def build(verbose: bool = False) -> str:
     hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
     tmp = OUT + '.tmp'
-    cmd = [hipcc] + FLAGS + ['-o', tmp] + SRC
+    cmd = [hipcc] + FLAGS + ['-o', tmp] + SRC + LIBS
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError('hipcc failed:\n' + res.stdout + res.stderr)

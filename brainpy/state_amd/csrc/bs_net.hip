@@ -45,6 +45,8 @@ static int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
+// shared with the other translation units of the library
+int bs_internal_fail(int code, const std::string& msg) { return fail(code, msg); }
 #define HIPCHK(expr)                                                                   \
   do {                                                                                 \
     hipError_t e_ = (expr);                                                            \

@@ -173,6 +173,55 @@ int bs_net_last_timing(bs_net* net, double* deliver_ms, double* update_ms,
 int bs_net_last_bytes(bs_net* net, double* deliver_bytes, double* update_bytes,
                       int64_t* n_events);
 
+/* ---- surrogate-gradient BPTT of a recurrent iaf_psc_exp network ----------
+ *
+ * Replaces the kernel learner's episode gradient: BPTT._value_and_grad over
+ * checkpointed_scan(Program.step) (_kernel/learn/bptt.py:437-491) for an
+ * iaf_psc_exp population (_kernel/models/iaf_psc_exp.py:120-151) with
+ * SpikeRule.check / count_ste (_kernel/spiking.py:65-79, 364-488),
+ * detach_reset = True, a fast-sigmoid surrogate on the margin
+ * (V - V_th) / (V_th - V_reset), and the rate_mse / membrane_mse readouts
+ * (bptt.py:170-215). Neurons 0..n_exc-1 are excitatory (their spikes and the
+ * input sources feed I_syn_ex), n_exc..N-1 inhibitory (they feed I_syn_in).
+ * An event emitted at step n reaches its target at step n + delay.
+ * Weights are dense row-major float32 matrices (pre, post) in pA; absent
+ * edges hold 0. Array arguments of forward/backward are DEVICE pointers; the
+ * calls return after the work has finished. */
+#define BS_LOSS_RATE_MSE 0
+#define BS_LOSS_MEMBRANE_MSE 1
+
+typedef struct bs_bptt bs_bptt;
+
+typedef struct bs_bptt_desc {
+  int32_t abi_version;
+  int32_t n_in, n_exc, n_inh;   /* input sources, excitatory, inhibitory     */
+  int32_t batch, n_steps;
+  int32_t delay_steps;          /* recurrent delay, >= 1                     */
+  int32_t delay_in_steps;       /* input delay, >= 1                         */
+  int32_t checkpoint;           /* steps between stored states, >= 1         */
+  int32_t loss;                 /* BS_LOSS_*                                 */
+  int32_t ref_steps;            /* refractory counter value set by a spike   */
+  int32_t _pad;
+  double dt_ms;
+  double E_L, V_th, V_reset, I_e;                 /* mV, mV, mV, pA          */
+  double P22, P21ex, P21in, P11ex, P11in, P20;    /* exact propagators       */
+  double surrogate_alpha;       /* slope of the fast sigmoid on the margin   */
+} bs_bptt_desc;
+
+int bs_bptt_create(const bs_bptt_desc* desc, bs_bptt** out);
+int bs_bptt_destroy(bs_bptt* h);
+/* w_in [n_in, N], w_rec [N, N], in_counts uint8 [n_steps, batch, n_in]
+ * (events of each source per step), v0 [batch, N] initial V_m or null (E_L),
+ * target [N] (Hz for rate_mse, mV for membrane_mse). Writes the loss. */
+int bs_bptt_forward(bs_bptt* h, const float* w_in, const float* w_rec, const uint8_t* in_counts,
+                    const float* v0, const float* target, double* loss);
+/* Gradients of the last forward's loss: g_in [n_in, N], g_rec [N, N]. */
+int bs_bptt_backward(bs_bptt* h, float* g_in, float* g_rec);
+/* Host copies from the last forward: spike bitmaps uint64 [n_steps, batch,
+ * ceil(N/64)] (bit j%64 of word j/64) and spike counts float [batch, N];
+ * either pointer may be null. */
+int bs_bptt_spikes(bs_bptt* h, uint64_t* bitmaps, float* counts);
+
 /* ---- test hooks: device arithmetic evaluated elementwise ------------------
  * No counterpart in the reference; they let tests compare the device's
  * exp/log/pow, Threefry-2x32 and Poisson sampler bit-for-bit with the C
