@@ -79,7 +79,7 @@ class NetDesc(C.Structure):
 class BpttDesc(C.Structure):
     _fields_ = ([(k, C.c_int32) for k in ('abi_version', 'n_in', 'n_exc', 'n_inh', 'batch', 'n_steps',
                                           'delay_steps', 'delay_in_steps', 'checkpoint', 'loss',
-                                          'ref_steps', '_pad')] +
+                                          'ref_steps', 'gemm')] +
                 [(k, C.c_double) for k in ('dt_ms', 'E_L', 'V_th', 'V_reset', 'I_e', 'P22', 'P21ex',
                                            'P21in', 'P11ex', 'P11in', 'P20', 'surrogate_alpha')])
 

@@ -99,9 +99,15 @@ class BPTT:
     loss : {'rate_mse', 'membrane_mse'}
     detach_reset : bool
         Only ``True`` (the learner's default, bptt.py:276) is implemented.
+    gemm : {'bf16x3', 'fp32'}
+        Arithmetic of the transposed delivery ``W_rec . ibar``: three bf16
+        products of the hi/lo split of both operands accumulated in fp32
+        (matrix-core rate, about 2^-16 relative error per product), or plain
+        fp32.
     """
 
-    def __init__(self, net, *, batch, n_steps, checkpoint=100, loss='rate_mse', detach_reset=True):
+    def __init__(self, net, *, batch, n_steps, checkpoint=100, loss='rate_mse', detach_reset=True,
+                 gemm='bf16x3'):
         if not detach_reset:
             raise NotImplementedError('only detach_reset=True is implemented')
         if loss not in ('rate_mse', 'membrane_mse'):
@@ -117,6 +123,9 @@ class BPTT:
         d.checkpoint = self.checkpoint
         d.loss = L.BS_LOSS_RATE_MSE if loss == 'rate_mse' else L.BS_LOSS_MEMBRANE_MSE
         d.ref_steps = k['ref_steps']
+        if gemm not in ('bf16x3', 'fp32'):
+            raise ValueError(f"gemm must be 'bf16x3' or 'fp32', got {gemm!r}")
+        d.gemm = 0 if gemm == 'bf16x3' else 1
         d.dt_ms = net.dt
         for f in ('E_L', 'V_th', 'V_reset', 'I_e', 'P22', 'P21ex', 'P21in', 'P11ex', 'P11in', 'P20',
                   'surrogate_alpha'):

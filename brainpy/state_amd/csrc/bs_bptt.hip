@@ -75,6 +75,7 @@ struct FwdArgs {
   const float* w_rec;
   const uint8_t* cnt;
   unsigned long long* bm;
+  unsigned long long* bmT;  // [NW][T][B]
   float *sv, *se, *si;
   int* sr;
   float *ck_v, *ck_e, *ck_i;
@@ -84,6 +85,7 @@ struct FwdArgs {
   const float* target;
   float* act;
   float* vbuf;
+  int* err;
 };
 
 // Fast sigmoid (SuperSpike) derivative on the margin m = (V - V_th) / (V_th - V_reset):
@@ -93,14 +95,44 @@ __device__ __forceinline__ float surrogate_dv(float m, const Consts& c) {
   return 1.0f / (q * q) / c.span;
 }
 
+constexpr int kChunk = 8;      // steps whose events are compacted together
+constexpr int kEvCap = 3072;   // events per chunk held in LDS (~28 KB of LDS per
+                               // block keeps 1024 blocks resident on 256 CUs)
+constexpr int kBatch = 16;     // event loads in flight per neuron
+
+__device__ __forceinline__ int wave_excl_scan(int v, int lane) {
+  int inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  return inc - v;
+}
+
+// Weight of event row r for neuron j: rows 0..n_in-1 are w_in, the rest w_rec.
+__device__ __forceinline__ float ev_weight(const FwdArgs& a, int row, int j) {
+  return row < a.n_in ? a.w_in[(size_t)row * a.N + j] : a.w_rec[(size_t)(row - a.n_in) * a.N + j];
+}
+
+// Forward steps of one (trial, neuron tile). The events arriving in a chunk
+// of steps (inputs with a non-zero count, then spiking presynaptic neurons,
+// both ascending) are compacted into LDS first; each step then sums them in
+// that order while the next step's weights are already being loaded.
 __global__ __launch_bounds__(kTile) void k_bptt_fwd(FwdArgs a) {
-  __shared__ unsigned long long bml[kMaxWords];
-  __shared__ unsigned char cl[kMaxIn];
+  __shared__ unsigned short ev_row[kEvCap];
+  __shared__ unsigned char ev_sc[kEvCap];
+  __shared__ unsigned char ev_k[kEvCap];
+  __shared__ float xs[2][kChunk][kTile];  // summed inputs (ex, in) per step of the chunk
+  __shared__ int ev_off[kChunk + 1];
+  __shared__ int st_cnt[kChunk];
   const int tid = (int)threadIdx.x, b = (int)blockIdx.y;
+  const int lane = tid & 63, wave = tid >> 6;
   const int j = (int)blockIdx.x * kTile + tid;
   const bool on = j < a.N;
+  const int jj = on ? j : 0;
   const size_t BN = (size_t)a.B * a.N;
-  const size_t bj = (size_t)b * a.N + (on ? j : 0);
+  const size_t bj = (size_t)b * a.N + jj;
   const Consts& c = a.c;
   float V = 0.f, ie = 0.f, ii = 0.f, cnt = 0.f;
   int r = 0;
@@ -122,66 +154,137 @@ __global__ __launch_bounds__(kTile) void k_bptt_fwd(FwdArgs a) {
     }
   }
   const float tgt = on ? a.target[j] : 0.f;
-  for (int k = 0; k < a.nsteps; ++k) {
-    const int n = a.n0 + k;
-    if (a.mode == 0 && on && n % a.C == 0) {
-      const size_t o = (size_t)(n / a.C) * BN + bj;
-      a.ck_v[o] = V;
-      a.ck_e[o] = ie;
-      a.ck_i[o] = ii;
-      a.ck_r[o] = r;
+  const int nex_rows = a.n_in + a.n_exc;  // rows below this feed I_syn_ex
+  for (int c0 = 0; c0 < a.nsteps; c0 += kChunk) {
+    const int nc = min(kChunk, a.nsteps - c0);
+    // ---- count the events of every step of the chunk (one wave per step)
+    __syncthreads();
+    for (int k = wave; k < nc; k += kTile / 64) {
+      const int n = a.n0 + c0 + k;
+      const int s_in = n - a.d_in, s_rec = n - a.d;
+      int m = 0;
+      if (s_in >= 0)
+        for (int i = lane; i < a.n_in; i += 64) m += a.cnt[((size_t)s_in * a.B + b) * a.n_in + i] != 0;
+      if (s_rec >= 0)
+        for (int w = lane; w < a.NW; w += 64) m += __popcll(a.bm[((size_t)s_rec * a.B + b) * a.NW + w]);
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) m += __shfl_xor(m, o, 64);
+      if (lane == 0) st_cnt[k] = m;
     }
-    const int s_in = n - a.d_in, s_rec = n - a.d;
     __syncthreads();
-    for (int t = tid; t < a.n_in; t += kTile)
-      cl[t] = s_in >= 0 ? a.cnt[((size_t)s_in * a.B + b) * a.n_in + t] : (unsigned char)0;
-    for (int t = tid; t < a.NW; t += kTile)
-      bml[t] = s_rec >= 0 ? a.bm[((size_t)s_rec * a.B + b) * a.NW + t] : 0ull;
-    __syncthreads();
-    // inputs arriving at n: sources ascending, then presynaptic neurons ascending
-    float xe = 0.f, xi = 0.f;
-    if (on) {
-      for (int i = 0; i < a.n_in; ++i) {
-        const unsigned q = cl[i];
-        if (q) xe = xe + (float)q * a.w_in[(size_t)i * a.N + j];
+    if (tid == 0) {
+      int t = 0;
+      for (int k = 0; k < nc; ++k) {
+        ev_off[k] = t;
+        t += st_cnt[k];
       }
-      for (int w = 0; w < a.NW; ++w) {
-        unsigned long long m = bml[w];
-        while (m) {
-          const int p = w * 64 + __builtin_ctzll(m);
-          m &= m - 1;
-          const float wv = a.w_rec[(size_t)p * a.N + j];
-          if (p < a.n_exc) xe = xe + wv; else xi = xi + wv;
+      ev_off[nc] = t;
+    }
+    __syncthreads();
+    if (ev_off[nc] > kEvCap) {  // more events than LDS holds: report, leave state untouched
+      if (tid == 0) atomicOr(a.err, 1);
+      return;
+    }
+    // ---- write the lists: inputs ascending, then presynaptic neurons ascending
+    for (int k = wave; k < nc; k += kTile / 64) {
+      const int n = a.n0 + c0 + k;
+      const int s_in = n - a.d_in, s_rec = n - a.d;
+      int pos = ev_off[k];
+      if (s_in >= 0)
+        for (int i0 = 0; i0 < a.n_in; i0 += 64) {
+          const int i = i0 + lane;
+          const unsigned q = i < a.n_in ? a.cnt[((size_t)s_in * a.B + b) * a.n_in + i] : 0u;
+          const unsigned long long bal = __ballot(q != 0);
+          if (q) {
+            const int slot = pos + __popcll(bal & ((1ull << lane) - 1ull));
+            ev_row[slot] = (unsigned short)i;
+            ev_sc[slot] = (unsigned char)q;
+            ev_k[slot] = (unsigned char)k;
+          }
+          pos += __popcll(bal);
+        }
+      if (s_rec >= 0)
+        for (int w0 = 0; w0 < a.NW; w0 += 64) {
+          const int w = w0 + lane;
+          unsigned long long word = w < a.NW ? a.bm[((size_t)s_rec * a.B + b) * a.NW + w] : 0ull;
+          const int pc = __popcll(word);
+          int slot = pos + wave_excl_scan(pc, lane);
+          while (word) {
+            const int bit = __builtin_ctzll(word);
+            word &= word - 1;
+            ev_row[slot] = (unsigned short)(a.n_in + w * 64 + bit);
+            ev_sc[slot] = 1;
+            ev_k[slot] = (unsigned char)k;
+            ++slot;
+          }
+          pos = __shfl(slot, 63, 64);  // end of lane 63's bits
+        }
+    }
+    __syncthreads();
+    // ---- gather: every event of the chunk in list order (per step: inputs,
+    // then presynaptic neurons, ascending), kBatch loads in flight, summed
+    // into this neuron's LDS slot of its step
+    for (int k = 0; k < nc; ++k) {
+      xs[0][k][tid] = 0.f;
+      xs[1][k][tid] = 0.f;
+    }
+    const int E = ev_off[nc];
+    for (int e0 = 0; e0 < E; e0 += kBatch) {
+      float v[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) v[u] = e0 + u < E ? ev_weight(a, ev_row[e0 + u], jj) : 0.f;
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        if (e0 + u < E) {
+          const int row = ev_row[e0 + u];
+          const float x = row < a.n_in ? (float)ev_sc[e0 + u] * v[u] : v[u];
+          float* acc = &xs[row < nex_rows ? 0 : 1][ev_k[e0 + u]][tid];
+          *acc = *acc + x;
         }
       }
     }
-    // the neuron update (iaf_psc_exp.py:136-151), then the spike rule
-    const bool ref = r > 0;
-    float y = V - c.E_L;
-    const float y_new = ((y * c.P22 + ie * c.P21ex) + ii * c.P21in) + c.I_e * c.P20;
-    if (!ref) y = y_new;
-    ie = ie * c.P11ex + xe;
-    ii = ii * c.P11in + xi;
-    V = y + c.E_L;
-    const float m = (V - c.V_th) / (c.V_th - c.V_reset);
-    const bool hard = on && !ref && m >= 0.f;
-    if (a.mode == 1 && on) {
-      const size_t o = (size_t)(n - a.seg0) * BN + bj;
-      a.act[o] = ref ? -1.0f : surrogate_dv(m, c);
-    }
-    if (hard) {
-      V = c.V_reset;
-      r = c.reset_to;
-    }
-    r = r > 0 ? r - 1 : 0;
-    if (a.mode == 1 && on && a.vbuf) a.vbuf[(size_t)(n - a.seg0) * BN + bj] = V;
-    if (a.mode == 0) {
-      const unsigned long long mask = __ballot(hard);
-      if ((tid & 63) == 0 && j < a.N) a.bm[((size_t)n * a.B + b) * a.NW + j / 64] = mask;
-      if (hard) cnt += 1.0f;
-      if (a.loss == BS_LOSS_MEMBRANE_MSE) {
-        const double e = (double)V - (double)tgt;
-        lp += e * e;
+    // ---- the steps
+    for (int k = 0; k < nc; ++k) {
+      const int n = a.n0 + c0 + k;
+      if (a.mode == 0 && on && n % a.C == 0) {
+        const size_t o = (size_t)(n / a.C) * BN + bj;
+        a.ck_v[o] = V;
+        a.ck_e[o] = ie;
+        a.ck_i[o] = ii;
+        a.ck_r[o] = r;
+      }
+      const float xe = xs[0][k][tid], xi = xs[1][k][tid];
+      // the neuron update (iaf_psc_exp.py:136-151), then the spike rule
+      const bool ref = r > 0;
+      float y = V - c.E_L;
+      const float y_new = ((y * c.P22 + ie * c.P21ex) + ii * c.P21in) + c.I_e * c.P20;
+      if (!ref) y = y_new;
+      ie = ie * c.P11ex + xe;
+      ii = ii * c.P11in + xi;
+      V = y + c.E_L;
+      const float mg = (V - c.V_th) / (c.V_th - c.V_reset);
+      const bool hard = on && !ref && mg >= 0.f;
+      if (a.mode == 1 && on) {
+        const size_t o = (size_t)(n - a.seg0) * BN + bj;
+        a.act[o] = ref ? -1.0f : surrogate_dv(mg, c);
+      }
+      if (hard) {
+        V = c.V_reset;
+        r = c.reset_to;
+      }
+      r = r > 0 ? r - 1 : 0;
+      if (a.mode == 1 && on && a.vbuf) a.vbuf[(size_t)(n - a.seg0) * BN + bj] = V;
+      if (a.mode == 0) {
+        const unsigned long long mask = __ballot(hard);
+        if (lane == 0 && j < a.N) {
+          a.bm[((size_t)n * a.B + b) * a.NW + j / 64] = mask;
+          a.bmT[((size_t)(j / 64) * a.T + n) * a.B + b] = mask;
+        }
+        if (hard) cnt += 1.0f;
+        if (a.loss == BS_LOSS_MEMBRANE_MSE) {
+          const double e = (double)V - (double)tgt;
+          lp += e * e;
+        }
       }
     }
   }
@@ -215,16 +318,22 @@ __global__ __launch_bounds__(kTile) void k_bptt_init(int B, int N, float E_L, co
   }
 }
 
-// Loss of the episode and, for rate_mse, d loss / d spike (constant over steps):
-// one block, fixed summation order.
-__global__ __launch_bounds__(kTile) void k_bptt_loss(int B, int N, int loss, double inv_T_s,
-                                                     double inv_count, const float* counts,
-                                                     const double* lpart, const float* target,
-                                                     float* grate, double* out) {
+// Loss of the episode and, for rate_mse, d loss / d spike (constant over
+// steps). Two fixed-order reduction levels: kLossBlocks blocks each sum one
+// contiguous range, then one block sums the partials in block order.
+constexpr int kLossBlocks = 256;
+
+__global__ __launch_bounds__(kTile) void k_bptt_loss_part(int B, int N, int loss, double inv_T_s,
+                                                          double inv_count, const float* counts,
+                                                          const double* lpart, const float* target,
+                                                          float* grate, double* part) {
   __shared__ double red[kTile];
   const int tid = (int)threadIdx.x;
+  const size_t total = (size_t)B * N;
+  const size_t per = (total + kLossBlocks - 1) / kLossBlocks;
+  const size_t lo = (size_t)blockIdx.x * per, hi = std::min(total, lo + per);
   double acc = 0.0;
-  for (size_t i = tid; i < (size_t)B * N; i += kTile) {
+  for (size_t i = lo + tid; i < hi; i += kTile) {
     if (loss == BS_LOSS_RATE_MSE) {
       const double rate = (double)counts[i] * inv_T_s;
       const double e = rate - (double)target[i % N];
@@ -240,7 +349,44 @@ __global__ __launch_bounds__(kTile) void k_bptt_loss(int B, int N, int loss, dou
     if (tid < s) red[tid] += red[tid + s];
     __syncthreads();
   }
+  if (tid == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kTile) void k_bptt_loss_final(double inv_count, const double* part,
+                                                           double* out) {
+  __shared__ double red[kTile];
+  const int tid = (int)threadIdx.x;
+  red[tid] = tid < kLossBlocks ? part[tid] : 0.0;
+  __syncthreads();
+  for (int s = kTile / 2; s > 0; s >>= 1) {
+    if (tid < s) red[tid] += red[tid + s];
+    __syncthreads();
+  }
   if (tid == 0) *out = red[0] * inv_count;
+}
+
+// bf16 round-to-nearest-even of a finite float, as raw bits.
+__device__ __forceinline__ unsigned short bf16_bits(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_float(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+// Row of the three-term split: [hi | lo | hi] for the ibar side, [hi | hi | lo]
+// for the weight side, so that one GEMM with K = 3N sums hi*hi + hi*lo + lo*hi.
+__device__ __forceinline__ void split3(float x, unsigned short* row, int N, int j, bool weight_side) {
+  const unsigned short hi = bf16_bits(x);
+  const unsigned short lo = bf16_bits(x - bf16_float(hi));
+  row[j] = hi;
+  row[N + j] = weight_side ? hi : lo;
+  row[2 * N + j] = weight_side ? lo : hi;
+}
+
+__global__ __launch_bounds__(kTile) void k_bptt_split_w(int N, const float* w, unsigned short* w3) {
+  const int p = (int)blockIdx.y;
+  const int j = (int)blockIdx.x * kTile + (int)threadIdx.x;
+  if (j < N) split3(w[(size_t)p * N + j], w3 + (size_t)p * 3 * N, N, j, true);
 }
 
 struct BwdArgs {
@@ -256,6 +402,7 @@ struct BwdArgs {
   float mscale;
   float *ay, *ae, *ai;
   float *ibex, *ibin;
+  unsigned short *ib3e, *ib3i;  // bf16 split rows [T][B][3N], or null for fp32 GEMMs
 };
 
 // Reverse-time adjoint of one window for one (trial, neuron tile). The carry
@@ -282,6 +429,11 @@ __global__ __launch_bounds__(kTile) void k_bptt_bwd(BwdArgs a) {
     const float sb = a.sbar[((size_t)k * a.B + b) * a.N + j] + g_rate;
     a.ibex[(size_t)n * BN + bj] = ge;
     a.ibin[(size_t)n * BN + bj] = gi;
+    if (a.ib3e) {
+      const size_t row = ((size_t)n * a.B + b) * 3 * a.N;
+      split3(ge, a.ib3e + row, a.N, j, false);
+      split3(gi, a.ib3i + row, a.N, j, false);
+    }
     const float gp = (hard ? 0.f : gv) + sb * ds;  // d loss / d V before the reset
     if (ref) {
       gv = gp;
@@ -298,93 +450,48 @@ __global__ __launch_bounds__(kTile) void k_bptt_bwd(BwdArgs a) {
   a.ai[bj] = gi;
 }
 
-// d loss / d w_rec[p][:] = sum over the spikes of p at step s of
-// ibar_{ex|in}(s + d)[:], in ascending (step, trial) order; one block per p.
-__global__ __launch_bounds__(kTile) void k_bptt_wgrad_rec(int N, int NW, int B, int T, int d, int n_exc,
-                                                          const unsigned long long* bm,
-                                                          const float* ibex, const float* ibin,
-                                                          float* g) {
-  __shared__ int ev[kTile];
-  __shared__ int wcount[kTile / 64];
-  __shared__ int nev;
-  const int p = (int)blockIdx.x, tid = (int)threadIdx.x;
-  const float* ib = p < n_exc ? ibex : ibin;
-  const int wi = p / 64, bit = p & 63;
-  const long long npairs = (long long)(T - d) * B;  // arrival steps d..T-1
-  for (int c0 = 0; c0 < N; c0 += 16 * kTile) {
-    float acc[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) acc[q] = 0.f;
-    for (long long base = 0; base < npairs; base += kTile) {
-      const long long e = base + tid;
-      bool hit = false;
-      if (e < npairs) {
-        const long long s = e / B;  // spike step (arrival s + d)
-        const int bb = (int)(e % B);
-        hit = (bm[((size_t)s * B + bb) * NW + wi] >> bit) & 1ull;
-      }
-      __syncthreads();
-      // compact in ascending pair order: wave prefix via ballot, waves in order
-      const unsigned long long bal = __ballot(hit);
-      const int lane = tid & 63, wv = tid >> 6;
-      if (lane == 0) wcount[wv] = __popcll(bal);
-      __syncthreads();
-      int off = 0;
-      for (int w = 0; w < wv; ++w) off += wcount[w];
-      if (hit) ev[off + __popcll(bal & ((1ull << lane) - 1ull))] = (int)(e - base);
-      if (tid == 0) {
-        int t = 0;
-        for (int w = 0; w < kTile / 64; ++w) t += wcount[w];
-        nev = t;
-      }
-      __syncthreads();
-      const int m = nev;
-      for (int k = 0; k < m; ++k) {
-        const long long e2 = base + ev[k];
-        const long long s = e2 / B;
-        const int bb = (int)(e2 % B);
-        const float* row = ib + ((size_t)(s + d) * B + bb) * N;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          const int col = c0 + q * kTile + tid;
-          if (col < N) acc[q] = acc[q] + row[col];
-        }
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int col = c0 + q * kTile + tid;
-      if (col < N) g[(size_t)p * N + col] = acc[q];
-    }
-  }
-}
+// Weight gradients, one block per presynaptic row (a recurrent neuron p or an
+// input source i): d loss / d w[row][:] = sum over the row's events at step s
+// (trial b) of scale * ibar(s + delay)[b][:], in ascending (s, b) order. The
+// events are found by scanning the row's column of the transposed spike
+// bitmaps (or input counts), contiguous in (s, b).
+struct WgArgs {
+  int N, B, T, d, kind;  // kind 0: recurrent rows, 1: input rows
+  int n_exc;
+  const unsigned long long* bmT;  // [NW][T][B]
+  const uint8_t* cntT;            // [n_in][T][B]
+  const float* ibex;
+  const float* ibin;
+  float* g;
+};
 
-// d loss / d w_in[i][:] = sum over (step, trial) of count_i(s) * ibar_ex(s + d_in)[:].
-__global__ __launch_bounds__(kTile) void k_bptt_wgrad_in(int N, int B, int T, int d_in, int n_in,
-                                                         const uint8_t* cnt, const float* ibex,
-                                                         float* g) {
+__global__ __launch_bounds__(kTile) void k_bptt_wgrad(WgArgs a) {
   __shared__ int ev[kTile];
-  __shared__ unsigned char evc[kTile];
+  __shared__ float evs[kTile];
   __shared__ int wcount[kTile / 64];
   __shared__ int nev;
-  const int i = (int)blockIdx.x, tid = (int)threadIdx.x;
-  const long long npairs = (long long)(T - d_in) * B;
-  for (int c0 = 0; c0 < N; c0 += 16 * kTile) {
+  const int row = (int)blockIdx.x, tid = (int)threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  const size_t TB = (size_t)a.T * a.B;
+  const float* ib = (a.kind == 0 && row >= a.n_exc) ? a.ibin : a.ibex;
+  const unsigned long long* wcol = a.kind == 0 ? a.bmT + (size_t)(row / 64) * TB : nullptr;
+  const uint8_t* ccol = a.kind == 1 ? a.cntT + (size_t)row * TB : nullptr;
+  const int bit = row & 63;
+  const long long npairs = (long long)(a.T - a.d) * a.B;
+  for (int c0 = 0; c0 < a.N; c0 += 16 * kTile) {
     float acc[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) acc[q] = 0.f;
     for (long long base = 0; base < npairs; base += kTile) {
       const long long e = base + tid;
-      unsigned char cv = 0;
+      float sc = 0.f;
       if (e < npairs) {
-        const long long s = e / B;
-        const int bb = (int)(e % B);
-        cv = cnt[((size_t)s * B + bb) * n_in + i];
+        if (a.kind == 0) sc = (float)((wcol[e] >> bit) & 1ull);
+        else sc = (float)ccol[e];
       }
-      const bool hit = cv != 0;
+      const bool hit = sc != 0.f;
       __syncthreads();
       const unsigned long long bal = __ballot(hit);
-      const int lane = tid & 63, wv = tid >> 6;
       if (lane == 0) wcount[wv] = __popcll(bal);
       __syncthreads();
       int off = 0;
@@ -392,7 +499,7 @@ __global__ __launch_bounds__(kTile) void k_bptt_wgrad_in(int N, int B, int T, in
       if (hit) {
         const int slot = off + __popcll(bal & ((1ull << lane) - 1ull));
         ev[slot] = (int)(e - base);
-        evc[slot] = cv;
+        evs[slot] = sc;
       }
       if (tid == 0) {
         int t = 0;
@@ -401,25 +508,49 @@ __global__ __launch_bounds__(kTile) void k_bptt_wgrad_in(int N, int B, int T, in
       }
       __syncthreads();
       const int m = nev;
-      for (int k = 0; k < m; ++k) {
-        const long long e2 = base + ev[k];
-        const long long s = e2 / B;
-        const int bb = (int)(e2 % B);
-        const float cf = (float)evc[k];
-        const float* row = ibex + ((size_t)(s + d_in) * B + bb) * N;
+      for (int k = 0; k < m; k += 4) {
+        float v[4][16];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          const int col = c0 + q * kTile + tid;
-          if (col < N) acc[q] = acc[q] + cf * row[col];
+        for (int u = 0; u < 4; ++u) {
+          const float* rp = nullptr;
+          if (k + u < m) {
+            const long long e2 = base + ev[k + u];
+            const long long st = e2 / a.B;
+            const int bb = (int)(e2 % a.B);
+            rp = ib + ((size_t)(st + a.d) * a.B + bb) * a.N;
+          }
+#pragma unroll
+          for (int q = 0; q < 16; ++q) {
+            const int col = c0 + q * kTile + tid;
+            v[u][q] = (rp && col < a.N) ? rp[col] : 0.f;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (k + u < m) {
+            const float f = evs[k + u];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[q] = acc[q] + f * v[u][q];
+          }
         }
       }
     }
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int col = c0 + q * kTile + tid;
-      if (col < N) g[(size_t)i * N + col] = acc[q];
+      if (col < a.N) a.g[(size_t)row * a.N + col] = acc[q];
     }
   }
+}
+
+// counts [T][B][n_in] -> [n_in][T][B]
+__global__ __launch_bounds__(kTile) void k_bptt_transpose_counts(int n_in, size_t TB, const uint8_t* cnt,
+                                                                 uint8_t* cntT) {
+  const size_t idx = (size_t)blockIdx.x * kTile + threadIdx.x;
+  if (idx >= TB * n_in) return;
+  const size_t sb = idx / n_in;
+  const int i = (int)(idx % n_in);
+  cntT[(size_t)i * TB + sb] = cnt[idx];
 }
 
 }  // namespace
@@ -432,6 +563,8 @@ struct bs_bptt {
   rocblas_handle blas = nullptr;
   std::vector<void*> owned;
   unsigned long long* bm = nullptr;
+  unsigned long long* bmT = nullptr;
+  uint8_t* cntT = nullptr;
   float *sv = nullptr, *se = nullptr, *si = nullptr;
   int* sr = nullptr;
   float *ck_v = nullptr, *ck_e = nullptr, *ck_i = nullptr;
@@ -443,8 +576,11 @@ struct bs_bptt {
   float* act = nullptr;
   float* vbuf = nullptr;
   float *ibex = nullptr, *ibin = nullptr;
+  unsigned short *w3 = nullptr, *ib3e = nullptr, *ib3i = nullptr;
+  double* lpart_blocks = nullptr;
   float* sbar = nullptr;
   float *ay = nullptr, *ae = nullptr, *ai = nullptr;
+  int* err = nullptr;
   // arrays of the last forward (owned by the caller)
   const float* w_in = nullptr;
   const float* w_rec = nullptr;
@@ -494,6 +630,7 @@ FwdArgs fwd_args(bs_bptt* h) {
   a.w_rec = h->w_rec;
   a.cnt = h->cnt;
   a.bm = h->bm;
+  a.bmT = h->bmT;
   a.sv = h->sv;
   a.se = h->se;
   a.si = h->si;
@@ -507,6 +644,7 @@ FwdArgs fwd_args(bs_bptt* h) {
   a.target = h->target;
   a.act = h->act;
   a.vbuf = h->vbuf;
+  a.err = h->err;
   return a;
 }
 
@@ -568,6 +706,8 @@ int bs_bptt_create(const bs_bptt_desc* d, bs_bptt** out) {
     }                                  \
   } while (0)
   BALLOC(h->bm, (size_t)T * B * h->NW);
+  BALLOC(h->bmT, (size_t)T * B * h->NW);
+  BALLOC(h->cntT, (size_t)T * B * std::max(1, d->n_in));
   BALLOC(h->sv, BN);
   BALLOC(h->se, BN);
   BALLOC(h->si, BN);
@@ -584,17 +724,25 @@ int bs_bptt_create(const bs_bptt_desc* d, bs_bptt** out) {
   if (d->loss == BS_LOSS_MEMBRANE_MSE) BALLOC(h->vbuf, (size_t)std::min(C, T) * BN);
   BALLOC(h->ibex, (size_t)T * BN);
   BALLOC(h->ibin, (size_t)T * BN);
+  if (d->gemm == 0) {
+    BALLOC(h->w3, (size_t)N * 3 * N);
+    BALLOC(h->ib3e, (size_t)T * BN * 3);
+    BALLOC(h->ib3i, (size_t)T * BN * 3);
+  }
+  BALLOC(h->lpart_blocks, kLossBlocks);
   BALLOC(h->sbar, (size_t)h->W * BN);
   BALLOC(h->ay, BN);
   BALLOC(h->ae, BN);
   BALLOC(h->ai, BN);
+  BALLOC(h->err, 1);
 #undef BALLOC
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     free_bptt(h);
     return bs_internal_fail(BS_EHIP, "bs_bptt_create: hipStreamCreate failed");
   }
   if (rocblas_create_handle(&h->blas) != rocblas_status_success ||
-      rocblas_set_stream(h->blas, h->stream) != rocblas_status_success) {
+      rocblas_set_stream(h->blas, h->stream) != rocblas_status_success ||
+      rocblas_set_atomics_mode(h->blas, rocblas_atomics_not_allowed) != rocblas_status_success) {
     free_bptt(h);
     return bs_internal_fail(BS_EHIP, "bs_bptt_create: rocblas handle");
   }
@@ -621,6 +769,7 @@ int bs_bptt_forward(bs_bptt* h, const float* w_in, const float* w_rec, const uin
   hipLaunchKernelGGL(k_bptt_init, dim3(nb), dim3(kTile), 0, h->stream, B, N, h->c.E_L, v0, h->sv,
                      h->se, h->si, h->sr, h->counts, h->lpart, (float*)nullptr, (float*)nullptr,
                      (float*)nullptr);
+  BCHK(hipMemsetAsync(h->err, 0, sizeof(int), h->stream));
   FwdArgs a = fwd_args(h);
   a.mode = 0;
   const dim3 grid((unsigned)((N + kTile - 1) / kTile), (unsigned)B);
@@ -630,12 +779,22 @@ int bs_bptt_forward(bs_bptt* h, const float* w_in, const float* w_rec, const uin
     hipLaunchKernelGGL(k_bptt_fwd, grid, dim3(kTile), 0, h->stream, a);
   }
   const double T_s = T * h->d.dt_ms * 1e-3;
-  hipLaunchKernelGGL(k_bptt_loss, dim3(1), dim3(kTile), 0, h->stream, B, N, h->d.loss, 1.0 / T_s,
-                     h->d.loss == BS_LOSS_RATE_MSE ? 1.0 / (double)BN : 1.0 / ((double)BN * T),
-                     h->counts, h->lpart, target, h->grate, h->d_loss);
+  const double inv_count = h->d.loss == BS_LOSS_RATE_MSE ? 1.0 / (double)BN : 1.0 / ((double)BN * T);
+  hipLaunchKernelGGL(k_bptt_loss_part, dim3(kLossBlocks), dim3(kTile), 0, h->stream, B, N, h->d.loss,
+                     1.0 / T_s, inv_count, h->counts, h->lpart, target, h->grate, h->lpart_blocks);
+  hipLaunchKernelGGL(k_bptt_loss_final, dim3(1), dim3(kTile), 0, h->stream, inv_count,
+                     (const double*)h->lpart_blocks, h->d_loss);
+  if (h->w3)  // the weight side of the three-term split, once per forward
+    hipLaunchKernelGGL(k_bptt_split_w, dim3((unsigned)((N + kTile - 1) / kTile), (unsigned)N), dim3(kTile),
+                       0, h->stream, N, w_rec, h->w3);
   BCHK(hipGetLastError());
   BCHK(hipMemcpyAsync(loss, h->d_loss, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  int err = 0;
+  BCHK(hipMemcpyAsync(&err, h->err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   BCHK(hipStreamSynchronize(h->stream));
+  if (err)
+    return bs_internal_fail(BS_ECAPACITY, "bs_bptt_forward: more than 3072 arriving events in "
+                                          "eight steps of one trial");
   h->have_fwd = true;
   return BS_OK;
 }
@@ -673,6 +832,8 @@ int bs_bptt_backward(bs_bptt* h, float* g_in, float* g_rec) {
   ba.ai = h->ai;
   ba.ibex = h->ibex;
   ba.ibin = h->ibin;
+  ba.ib3e = h->ib3e;
+  ba.ib3i = h->ib3i;
   const float one = 1.0f, zero = 0.0f;
   for (int seg = h->nck - 1; seg >= 0; --seg) {
     const int s0 = seg * C, s1 = std::min(T, s0 + C);
@@ -692,13 +853,29 @@ int bs_bptt_backward(bs_bptt* h, float* g_in, float* g_rec) {
       if (kmax > 0) {
         const int M = kmax * B;
         const size_t src = (size_t)(w0 + d) * BN;
-        if (nE > 0)
-          RBCHK(rocblas_sgemm(h->blas, rocblas_operation_transpose, rocblas_operation_none, nE, M, N,
-                              &one, h->w_rec, N, h->ibex + src, N, &zero, h->sbar, N));
-        if (nI > 0)
-          RBCHK(rocblas_sgemm(h->blas, rocblas_operation_transpose, rocblas_operation_none, nI, M, N,
-                              &one, h->w_rec + (size_t)nE * N, N, h->ibin + src, N, &zero,
-                              h->sbar + nE, N));
+        if (h->w3) {  // bf16 three-term split on the matrix cores, fp32 accumulate
+          const int K3 = 3 * N;
+          if (nE > 0)
+            RBCHK(rocblas_gemm_ex(h->blas, rocblas_operation_transpose, rocblas_operation_none, nE, M, K3,
+                                  &one, h->w3, rocblas_datatype_bf16_r, K3, h->ib3e + src * 3,
+                                  rocblas_datatype_bf16_r, K3, &zero, h->sbar, rocblas_datatype_f32_r, N,
+                                  h->sbar, rocblas_datatype_f32_r, N, rocblas_datatype_f32_r,
+                                  rocblas_gemm_algo_standard, 0, 0));
+          if (nI > 0)
+            RBCHK(rocblas_gemm_ex(h->blas, rocblas_operation_transpose, rocblas_operation_none, nI, M, K3,
+                                  &one, h->w3 + (size_t)nE * K3, rocblas_datatype_bf16_r, K3,
+                                  h->ib3i + src * 3, rocblas_datatype_bf16_r, K3, &zero, h->sbar + nE,
+                                  rocblas_datatype_f32_r, N, h->sbar + nE, rocblas_datatype_f32_r, N,
+                                  rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0));
+        } else {
+          if (nE > 0)
+            RBCHK(rocblas_sgemm(h->blas, rocblas_operation_transpose, rocblas_operation_none, nE, M, N,
+                                &one, h->w_rec, N, h->ibex + src, N, &zero, h->sbar, N));
+          if (nI > 0)
+            RBCHK(rocblas_sgemm(h->blas, rocblas_operation_transpose, rocblas_operation_none, nI, M, N,
+                                &one, h->w_rec + (size_t)nE * N, N, h->ibin + src, N, &zero,
+                                h->sbar + nE, N));
+        }
       }
       ba.n0 = w0;
       ba.nsteps = nsteps;
@@ -706,21 +883,41 @@ int bs_bptt_backward(bs_bptt* h, float* g_in, float* g_rec) {
       hipLaunchKernelGGL(k_bptt_bwd, grid, dim3(kTile), 0, h->stream, ba);
     }
   }
-  if (T > d)
-    hipLaunchKernelGGL(k_bptt_wgrad_rec, dim3((unsigned)N), dim3(kTile), 0, h->stream, N, h->NW, B, T, d,
-                       nE, (const unsigned long long*)h->bm, (const float*)h->ibex,
-                       (const float*)h->ibin, g_rec);
-  else
+  WgArgs wg{};
+  wg.N = N;
+  wg.B = B;
+  wg.T = T;
+  wg.n_exc = nE;
+  wg.bmT = h->bmT;
+  wg.cntT = h->cntT;
+  wg.ibex = h->ibex;
+  wg.ibin = h->ibin;
+  if (T > d) {
+    wg.kind = 0;
+    wg.d = d;
+    wg.g = g_rec;
+    hipLaunchKernelGGL(k_bptt_wgrad, dim3((unsigned)N), dim3(kTile), 0, h->stream, wg);
+  } else {
     BCHK(hipMemsetAsync(g_rec, 0, sizeof(float) * (size_t)N * N, h->stream));
+  }
   if (h->d.n_in > 0) {
-    if (T > h->d.delay_in_steps)
-      hipLaunchKernelGGL(k_bptt_wgrad_in, dim3((unsigned)h->d.n_in), dim3(kTile), 0, h->stream, N, B, T,
-                         h->d.delay_in_steps, h->d.n_in, h->cnt, (const float*)h->ibex, g_in);
-    else
+    if (T > h->d.delay_in_steps) {
+      const size_t TB = (size_t)T * B;
+      hipLaunchKernelGGL(k_bptt_transpose_counts, dim3((unsigned)((TB * h->d.n_in + kTile - 1) / kTile)),
+                         dim3(kTile), 0, h->stream, h->d.n_in, TB, h->cnt, h->cntT);
+      wg.kind = 1;
+      wg.d = h->d.delay_in_steps;
+      wg.g = g_in;
+      hipLaunchKernelGGL(k_bptt_wgrad, dim3((unsigned)h->d.n_in), dim3(kTile), 0, h->stream, wg);
+    } else {
       BCHK(hipMemsetAsync(g_in, 0, sizeof(float) * (size_t)h->d.n_in * N, h->stream));
+    }
   }
   BCHK(hipGetLastError());
+  int err = 0;
+  BCHK(hipMemcpyAsync(&err, h->err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   BCHK(hipStreamSynchronize(h->stream));
+  if (err) return bs_internal_fail(BS_ECAPACITY, "bs_bptt_backward: event capacity exceeded");
   return BS_OK;
 }
 

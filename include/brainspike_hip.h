@@ -200,8 +200,10 @@ typedef struct bs_bptt_desc {
   int32_t delay_in_steps;       /* input delay, >= 1                         */
   int32_t checkpoint;           /* steps between stored states, >= 1         */
   int32_t loss;                 /* BS_LOSS_*                                 */
-  int32_t ref_steps;            /* refractory counter value set by a spike   */
-  int32_t _pad;
+  int32_t ref_steps;            /* refractory steps: the counter after a spike */
+  int32_t gemm;                 /* transposed delivery: 0 = bf16 three-term
+                                   split (hi*hi + hi*lo + lo*hi, fp32
+                                   accumulate), 1 = fp32                      */
   double dt_ms;
   double E_L, V_th, V_reset, I_e;                 /* mV, mV, mV, pA          */
   double P22, P21ex, P21in, P11ex, P11in, P20;    /* exact propagators       */
