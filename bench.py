@@ -16,6 +16,11 @@ The JSON line carries:
                HIP-event duration, against 8 TB/s
   cpu_baseline the CPU oracle (one thread) on a bounded sample of the same
                network, events/s
+  bptt         the differentiation workload (recurrent iaf_psc_exp, 4096
+               neurons, 10 % trainable recurrent and input weights, 200 Poisson
+               inputs, batch 64, 1000 steps, checkpoint 100): gradient steps/s
+               of the device adjoint, and the CPU reference (torch float64
+               autograd) on a bounded sample
 """
 from __future__ import annotations
 
@@ -43,6 +48,12 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--profile-only', action='store_true',
                     help='run warmup + timed steps only (for rocprofv3)')
+    ap.add_argument('--bptt-steps', type=int, default=5, help='timed gradient steps (0: skip)')
+    ap.add_argument('--bptt-warmup', type=int, default=1)
+    ap.add_argument('--no-pmc', action='store_true',
+                    help='skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes behind roofline.traffic')
+    ap.add_argument('--bptt-only', action='store_true',
+                    help='run only the BPTT leg (for rocprofv3)')
     ap.add_argument('--direct', action='store_true',
                     help='launch every window directly instead of replaying the captured graph')
     args = ap.parse_args()
@@ -53,6 +64,11 @@ def main():
 
     import numpy as np
     from brainpy.state_amd.networks import brunel_delta
+
+    if args.bptt_only:
+        args.no_cpu = True
+        print(json.dumps({'bptt': run_bptt(args)}), flush=True)
+        return
 
     t0 = time.time()
     sim, h = brunel_delta(order=2250, delay=('uniform', 1.0, 3.0), n_vm=10, dtype=args.dtype)
@@ -93,9 +109,11 @@ def main():
     log(f'k_window: {kms * 1e3:.2f} us/launch, {kbytes / 1e6:.3f} MB/launch '
         f'(gather {by2["deliver_bytes"] / n_launch / 1e6:.3f} MB, update '
         f'{by2["update_bytes"] / n_launch / 1e6:.3f} MB), {n_launch} launches')
+    traffic, tnote = (None, 'skipped (--no-pmc)') if args.no_pmc else pmc_traffic('k_window')
+    log(f'k_window HBM traffic per launch: {traffic} ({tnote})')
     roofline = {'bound': 'hbm', 'kernel': 'k_window', 'achieved': round(achieved, 2),
                 'peak': 8000.0, 'unit': 'GB/s', 'frac': round(achieved / 8000.0, 5),
-                'traffic': None, 'bytes_per_launch': round(kbytes),
+                'traffic': traffic, 'traffic_note': tnote, 'bytes_per_launch': round(kbytes),
                 'us_per_launch': round(kms * 1e3, 3),
                 'launch_fraction_of_wall': round(tm['update_ms'] / max(1e-9, tm['total_ms']), 4)}
 
@@ -119,6 +137,8 @@ def main():
                'rtf': round(cwall / (args.cpu_steps * sim.dt / 1000.0), 3)}
         log(f'cpu oracle: {args.cpu_steps} steps in {cwall:.1f}s, {ev / cwall:.3e} ev/s')
 
+    bptt = run_bptt(args) if args.bptt_steps > 0 else None
+
     out = {
         'metric': 'synaptic events/s (Brunel-11250, 1 GPU)',
         'value': round(value, 1),
@@ -141,8 +161,106 @@ def main():
         'firing_rate_hz': round(rate, 2),
         'roofline': roofline,
         'cpu_baseline': cpu,
+        'bptt': bptt,
     }
     print(json.dumps(out), flush=True)
+
+
+def pmc_traffic(kernel, skip=50):
+    """HBM bytes per launch of ``kernel`` from two rocprofv3 counter passes
+    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) over a short profile-only
+    run, each scaled by what the counter reads for a 256 MiB device copy.
+    The first ``skip`` dispatches (the network's start-up transient) are left
+    out. Returns (bytes or None, note)."""
+    import glob
+    import shutil
+    import sqlite3
+    import subprocess
+    import tempfile
+    exe = shutil.which('rocprofv3')
+    if exe is None:
+        return None, 'rocprofv3 not found'
+    tmp = tempfile.mkdtemp(prefix='bench_pmc_', dir='/tmp')
+    env = dict(os.environ, TMPDIR='/tmp')
+    prof = [os.path.join(ROOT, 'bench.py'), '--steps', '2000', '--warmup', '200', '--profile-only',
+            '--no-cpu', '--bptt-steps', '0', '--no-pmc']
+    copy = ['-c', "import torch; x = torch.ones(1 << 28, dtype=torch.uint8, device='cuda'); "
+                  "y = torch.empty_like(x); [y.copy_(x) for _ in range(4)]; torch.cuda.synchronize()"]
+
+    def values(counter, argv, tag, name):
+        out = os.path.join(tmp, tag)
+        cmd = [exe, '--pmc', counter, '-d', out, '-o', tag, '--', sys.executable, '-u'] + argv
+        try:
+            subprocess.run(cmd, env=env, cwd='/tmp', capture_output=True, timeout=90, check=False)
+        except subprocess.TimeoutExpired:
+            return None
+        dbs = glob.glob(os.path.join(out, '**', '*.db'), recursive=True)
+        if not dbs:
+            return None
+        con = sqlite3.connect(dbs[0])
+        rows = con.execute('select value from counters_collection where kernel_name like ? '
+                           'order by dispatch_id', (f'%{name}%',)).fetchall()
+        con.close()
+        return [r[0] for r in rows] or None
+
+    fetch = values('FETCH_SIZE', prof, 'fetch', kernel)
+    write = values('WRITE_SIZE', prof, 'write', kernel) if fetch else None
+    cf = values('FETCH_SIZE', copy, 'copy_fetch', 'copyBuffer') if write else None
+    cw = values('WRITE_SIZE', copy, 'copy_write', 'copyBuffer') if cf else None
+    shutil.rmtree(tmp, ignore_errors=True)
+    if not cw:
+        return None, 'rocprofv3 counter passes gave no rows'
+    copy_kib = (1 << 28) / 1024.0
+    ff, fw = copy_kib / (sum(cf) / len(cf)), copy_kib / (sum(cw) / len(cw))
+    f = fetch[skip:] or fetch
+    w = write[skip:] or write
+    kib = sum(f) / len(f) * ff + sum(w) / len(w) * fw
+    return round(kib * 1024.0), (f'FETCH_SIZE x {ff:.3f} + WRITE_SIZE x {fw:.3f} (copy-kernel '
+                                 f'scale), mean of {len(f)} dispatches after the first {skip}')
+
+
+def run_bptt(args):
+    """Gradient steps/s of the device BPTT on the differentiation workload."""
+    import numpy as np
+    import torch
+    from brainpy.state_amd.bptt import BPTT, bptt_problem
+    net, w_in, w_rec, cnt, target = bptt_problem()
+    B, T = cnt.shape[1], cnt.shape[0]
+    bp = BPTT(net, batch=B, n_steps=T, checkpoint=100)
+    dev = [torch.tensor(a).cuda() for a in (w_in, w_rec, cnt, target)]
+    for _ in range(args.bptt_warmup):
+        bp.loss_and_grad(*dev)
+    t0 = time.perf_counter()
+    for _ in range(args.bptt_steps):
+        loss, g_in, g_rec = bp.loss_and_grad(*dev)
+    wall = (time.perf_counter() - t0) / args.bptt_steps
+    _, counts = bp.spikes()
+    log(f'bptt: {wall * 1e3:.2f} ms per gradient step ({1 / wall:.2f}/s), loss {loss:.3f}, '
+        f'mean rate {counts.mean() / (T * net.dt * 1e-3):.1f} Hz')
+    res = {'metric': 'BPTT gradient steps/s', 'value': round(1.0 / wall, 3), 'unit': 'grad-steps/s',
+           'ms_per_grad_step': round(wall * 1e3, 3), 'steps': args.bptt_steps,
+           'warmup': args.bptt_warmup, 'dtype': 'f32 (transposed delivery bf16x3 split, f32 accumulate)',
+           'config': {'workload': 'recurrent iaf_psc_exp 3277 E + 819 I, 10% trainable recurrent and '
+                                  '200x4096 10% input weights, Poisson 5-50 Hz inputs, rate_mse to '
+                                  'U[2,20] Hz, fast-sigmoid surrogate 5/mV, delay 1 ms',
+                      'batch': B, 'n_steps': T, 'checkpoint': 100, 'dt_ms': net.dt},
+           'vs_baseline': None, 'cpu_baseline': None}
+    if not args.no_cpu:
+        from oracle import bptt_ref as R
+        Bs, Ts = 2, 60
+        c_s = np.ascontiguousarray(cnt[:Ts, :Bs])
+        t1 = time.perf_counter()
+        fwd = R.forward_f32(net, w_in, w_rec, c_s, target)
+        R.grad_autograd(net, w_in, w_rec, c_s, target, fwd)
+        cw = time.perf_counter() - t1
+        per_step = cw * (B / Bs) * (T / Ts)
+        res['cpu_baseline'] = {
+            'value': round(1.0 / per_step, 6), 'unit': 'grad-steps/s', 'cores': torch.get_num_threads(),
+            'kind': 'port',
+            'sample': f'{Bs} trials x {Ts} steps of the same network (NumPy float32 forward + torch '
+                      f'float64 autograd), {cw:.1f}s wall, scaled linearly to {B} x {T}'}
+        log(f'bptt cpu reference: {cw:.1f}s for {Bs}x{Ts}, ~{per_step:.0f}s per full gradient step')
+    return res
 
 
 if __name__ == '__main__':

@@ -127,6 +127,13 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise EngineUnavailable(
             f'{LIB_PATH} is not built; run __graft_entry__.build() (hipcc, gfx950)')
+    # torch ships its own HIP runtime. When the engine's (system ROCm) runtime is
+    # loaded and initialised first, torch's later device initialisation in the
+    # same process finds no GPU; loading torch's libraries first avoids that.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     handle = C.CDLL(LIB_PATH)
     for name, (res, args) in _SIGS.items():
         fn = getattr(handle, name)

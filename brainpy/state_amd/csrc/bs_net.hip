@@ -7,19 +7,26 @@
 //     at step s + 1 + d (the Simulator's one-step holder lag plus the delay,
 //     simulator.py:1727-1730 + delay.py:114-128), so no spike emitted inside a
 //     window can reach a step of the same window.
-//   * k_update advances every neuron through the W steps of a window with its
-//     state in registers: it drains the arrival ring, draws the Poisson drive
-//     from a counter-based stream, applies the model step, appends spikes to a
-//     spike list and writes voltmeter taps.
-//   * k_deliver walks the CSR row of every spike of the window and adds one
-//     event per edge into the arrival-ring word (integer atomics: exact and
-//     order independent, so every run is bit-reproducible). Homogeneous
-//     weights are applied at drain time through S[k], the sequential sum of k
-//     copies of the weight, which equals the reference's CSR event-matmul sum.
-//   * Plastic (stdp_synapse) projections run at arrival time with W = 1:
-//     k_stdp_post applies facilitation on the incoming edges of neurons that
-//     fired in the previous step, k_stdp_pre applies depression to arriving
-//     edges and deposits their weights as exact 2^-F fixed-point integers.
+//   * One launch of k_window advances one window. Each block owns a tile of
+//     TS post neurons of one population (XCD-aware tile order) and runs five
+//     phases: stage (parameters, channels, count tables, Poisson CDFs into
+//     LDS); gather (every spike whose delayed arrival falls in the window
+//     walks its (pre, tile) edge segment and adds one event per edge to an
+//     LDS counter per (step, channel, lane), integer atomics, so the result is
+//     exact and order independent); inputs (per step and lane, the channels in
+//     the reference's summation order: S[k] tables for homogeneous weights,
+//     exact 2^-F fixed-point sums for per-edge weights, count * weight for
+//     Poisson drive drawn from a counter-based Threefry stream); update (the
+//     model step for W steps with the state in registers); emit (per-step spike
+//     buckets for the following windows, the spike record, voltmeter taps).
+//   * Full chunks of windows are captured once into a HIP graph and replayed;
+//     the kernels read the chunk's first step from device memory.
+//   * Plastic (stdp_synapse) projections force W = 1 and add two kernels per
+//     step: k_stdp_traces (pre-trace history and post traces) and
+//     k_stdp_events (facilitation of the incoming edges of neurons that fired
+//     in the previous step, then depression of the edges whose pre spike
+//     arrives, depositing their new weights as exact 2^-F fixed-point sums
+//     that the window kernel reads as a channel).
 #include <hip/hip_runtime.h>
 #include <hipcub/device/device_radix_sort.hpp>
 #include <stdint.h>

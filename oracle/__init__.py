@@ -6,7 +6,30 @@ It loads ``oracle/liboracle.so`` (``oracle.cpp``, built by ``oracle/build.py``
 with g++ -ffp-contract=off) and runs the reference's algorithm on one CPU
 thread from the same ``bs_net_desc`` the engine consumes.
 
-Pinning status of each restated function is recorded in ``oracle/PINNING.md``.
+Pinning of each restated function (tests/test_oracle_kat.py,
+tests/test_stdp_oracle.py, tests/test_bptt_oracle.py):
+
+* threefry2x32 -- Random123 published known-answer vectors;
+* exp / log / pow -- agreement with libm to about one ulp;
+* Poisson sampler -- moments, Fano factor, pmf against scipy, window rule;
+* iaf_psc_delta -- closed-form recurrence and refractory cases
+  (iaf_psc_delta_test.py:88-224);
+* iaf_psc_exp -- DC first spike at 4.8 ms (iaf_psc_exp_test.py:84-103) and the
+  closed-form PSC kernel;
+* iaf_psc_alpha -- the NEST V_m table (iaf_psc_alpha_test.py:200-245);
+* aeif_cond_alpha -- an independent RKF45 restated from
+  aeif_cond_alpha_test.py:40-200, and the one-step current known answer;
+* delivery -- latency s + 1 + d, event conservation, count tables equal to
+  sequential sums;
+* stdp_synapse -- stdp_synapse_rule_test.py:65-131 known answers and a
+  closed-form two-neuron trajectory;
+* bptt_ref -- DC / refractory / PSC closed forms for the forward, central
+  finite differences and a closed-form surrogate gradient for autograd.
+
+Parity unpinned: the reference's JAX random streams (Poisson counts,
+connectivity), XLA's exp, brainevent's float accumulation order for per-edge
+weights, jax.grad of the kernel learner and braintools' surrogate definition;
+none of them can be run or imported here.
 """
 from __future__ import annotations
 
