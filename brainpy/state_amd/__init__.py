@@ -15,6 +15,7 @@ from .plasticity import stdp_synapse
 from .rules import (all_to_all, explicit_edges, fixed_indegree, fixed_total_number, one_to_one,
                     pairwise_bernoulli)
 from .simulator import SimulationResult, Simulator
+from .ensemble import Ensemble
 
 __version__ = '0.1.0'
 
@@ -22,4 +23,4 @@ __all__ = ['Simulator', 'SimulationResult', 'NodeView', 'all_to_all', 'one_to_on
            'fixed_indegree', 'pairwise_bernoulli', 'fixed_total_number', 'explicit_edges',
            'iaf_psc_delta', 'iaf_psc_exp', 'iaf_psc_alpha', 'aeif_cond_alpha',
            'poisson_generator', 'spike_recorder', 'voltmeter', 'multimeter', 'stdp_synapse',
-           'units', 'init', 'dist']
+           'units', 'init', 'dist', 'Ensemble']

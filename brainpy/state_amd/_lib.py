@@ -73,7 +73,7 @@ class NetDesc(C.Structure):
                 ('max_window', C.c_int32), ('pops', C.POINTER(Population)),
                 ('channels', C.POINTER(Channel)), ('projs', C.POINTER(Projection)),
                 ('tables', C.POINTER(C.c_double)), ('table_off', C.POINTER(C.c_int64)),
-                ('n_tables', C.c_int64)]
+                ('n_tables', C.c_int64), ('tile_size', C.c_int32), ('_pad2', C.c_int32)]
 
 
 class BpttDesc(C.Structure):

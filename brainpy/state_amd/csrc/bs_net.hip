@@ -939,6 +939,11 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
       return t;
     };
     while (TS > 64 && ntiles(TS) < 1024) TS /= 2;
+    if (d->tile_size != 0) {
+      if (d->tile_size != 64 && d->tile_size != 128 && d->tile_size != 256)
+        BAD("bs_net_create: tile_size must be 0, 64, 128 or 256");
+      TS = d->tile_size;
+    }
   }
   net->TS = TS;
 

@@ -124,3 +124,21 @@ def brunel_exp(N, *, frac_e=0.8, epsilon=0.1, g=5.0, eta=2.0, J=20.0, delay=1.5,
         sim.connect(ne, esr)
         sim.connect(ni, isr)
     return sim, dict(ne=ne, ni=ni, esr=esr, isr=isr, CE=CE, CI=CI, p_rate=p_rate)
+
+
+def brunel_exp_ensemble(n_networks=256, *, n_min=500, n_max=20000, g_range=(3.0, 7.0),
+                        eta_range=(0.9, 4.0), seed=0, dtype='float32', **kw):
+    """Independent Brunel iaf_psc_exp networks of uneven size: N log-uniform in
+    [n_min, n_max] (80 % E), 10 % fixed in-degree, g and nu_ext/nu_thr uniform
+    per network, tau_syn = 0.5 ms. Returns ``(sims, params)``; advance them
+    together with :class:`~brainpy.state_amd.Ensemble`."""
+    rng = np.random.default_rng(seed)
+    N = np.exp(rng.uniform(np.log(n_min), np.log(n_max), n_networks)).astype(np.int64)
+    g = rng.uniform(*g_range, n_networks)
+    eta = rng.uniform(*eta_range, n_networks)
+    sims = []
+    for k in range(n_networks):
+        sim, _ = brunel_exp(int(N[k]), g=float(g[k]), eta=float(eta[k]), seed=2 * (seed * n_networks + k) + 1,
+                            dtype=dtype, **kw)
+        sims.append(sim)
+    return sims, dict(N=N, g=g, eta=eta)

@@ -200,10 +200,13 @@ class Simulator:
         ``brainstate.environ`` precision).
     max_window : int
         Upper bound on the steps advanced per update launch (0 = automatic).
+    tile_size : int
+        Post neurons per block of the window kernel: 64, 128 or 256 (0 =
+        automatic). Results do not depend on it; ``Ensemble.autotune`` picks it.
     """
     __module__ = 'brainpy.state_amd'
 
-    def __init__(self, *, dt, dtype='float64', max_window=0):
+    def __init__(self, *, dt, dtype='float64', max_window=0, tile_size=0):
         self._dt_q = dt
         self.dt = u.scalar(dt, 'time')
         if self.dt <= 0:
@@ -215,6 +218,9 @@ class Simulator:
         else:
             raise ValueError(f"dtype must be 'float64' or 'float32', got {dtype!r}")
         self.max_window = int(max_window)
+        if int(tile_size) not in (0, 64, 128, 256):
+            raise ValueError('tile_size must be 0, 64, 128 or 256')
+        self.tile_size = int(tile_size)
         self._pops = []              # neuron populations, creation order
         self._taps = {}              # id(spike_recorder) -> (pop, idx)
         self._analog_taps = {}       # id(multimeter) -> (pop, idx, names)
@@ -691,6 +697,7 @@ class _NetBuilder:
         d.n_channels = c
         d.n_projs = len(projs)
         d.max_window = sim.max_window
+        d.tile_size = sim.tile_size
         d.pops = cpops
         d.channels = chans
         d.projs = cprojs
@@ -698,6 +705,8 @@ class _NetBuilder:
         d.table_off = D.arr(np.asarray(table_off or [-1], dtype=np.int64), C.c_int64)
         d.n_tables = len(tab)
         D.keep += [cpops, chans, cprojs]
+        D.tables_np = tab
+        D.table_off_np = np.asarray(table_off or [-1], dtype=np.int64)
         D.c = d
         D.ntot = gid
         return D

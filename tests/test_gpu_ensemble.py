@@ -1,0 +1,53 @@
+"""GPU checks of the batched ensemble: every network advanced inside one engine
+gives exactly the raster of its own plain run; the launch shape (tile size,
+window bound) changes nothing; the autotuner returns a measured candidate."""
+import numpy as np
+import pytest
+
+from brainpy.state_amd import Ensemble
+from brainpy.state_amd.networks import brunel_exp_ensemble
+
+pytestmark = pytest.mark.gpu
+
+
+def _plain(sim, n):
+    eng = sim._ensure_engine()
+    eng.reset()
+    vm, (g, s) = eng.run(n)
+    return g, s
+
+
+def test_ensemble_equals_sequential_plain_runs():
+    sims, p = brunel_exp_ensemble(6, n_min=200, n_max=1500, seed=5)
+    ens = Ensemble(sims)
+    parts = ens.run(2000)
+    total = 0
+    for sim, (vm, g, s) in zip(sims, parts):
+        g0, s0 = _plain(sim, 2000)
+        assert np.array_equal(g, g0) and np.array_equal(s, s0)
+        total += len(g)
+    assert total > 2000
+    ens.close()
+
+
+def test_ensemble_launch_shape_does_not_change_results_and_simulate_api():
+    from brainpy.state_amd import units as u
+    sims, _ = brunel_exp_ensemble(4, n_min=300, n_max=900, seed=7)
+    a = Ensemble(sims, tile_size=64)
+    ra = a.simulate(100. * u.ms)
+    a.close()
+    b = Ensemble(sims, tile_size=256, max_window=4)
+    rb = b.simulate(100. * u.ms)
+    rb2 = b.cont(50. * u.ms)
+    b.close()
+    for x, y, sim in zip(ra, rb, sims):
+        rid = next(iter(sim._taps))
+        assert np.array_equal(x._rec[rid], y._rec[rid])
+    assert rb2[0].times.to_decimal(u.ms)[0] == pytest.approx(100.0)
+
+
+def test_autotune_returns_a_timed_candidate():
+    sims, _ = brunel_exp_ensemble(3, n_min=300, n_max=600, seed=9)
+    best, table = Ensemble.autotune(sims, candidates=[(64, 0), (256, 0), (128, 4)], steps=100)
+    assert best in table and len(table) == 3
+    assert all(v > 0 for v in table.values())
