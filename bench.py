@@ -52,6 +52,9 @@ def main():
     ap.add_argument('--bptt-warmup', type=int, default=1)
     ap.add_argument('--no-pmc', action='store_true',
                     help='skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes behind roofline.traffic')
+    ap.add_argument('--ensemble', type=int, default=0, metavar='M',
+                    help='instead: the batched-ensemble workload with M networks (256 = full size)')
+    ap.add_argument('--ensemble-steps', type=int, default=5000)
     ap.add_argument('--bptt-only', action='store_true',
                     help='run only the BPTT leg (for rocprofv3)')
     ap.add_argument('--direct', action='store_true',
@@ -65,6 +68,9 @@ def main():
     import numpy as np
     from brainpy.state_amd.networks import brunel_delta
 
+    if args.ensemble:
+        print(json.dumps(run_ensemble(args)), flush=True)
+        return
     if args.bptt_only:
         args.no_cpu = True
         print(json.dumps({'bptt': run_bptt(args)}), flush=True)
@@ -217,6 +223,73 @@ def pmc_traffic(kernel, skip=50):
     kib = sum(f) / len(f) * ff + sum(w) / len(w) * fw
     return round(kib * 1024.0), (f'FETCH_SIZE x {ff:.3f} + WRITE_SIZE x {fw:.3f} (copy-kernel '
                                  f'scale), mean of {len(f)} dispatches after the first {skip}')
+
+
+def run_ensemble(args):
+    """Independent Brunel iaf_psc_exp networks (N log-uniform 500-20000, 10 %
+    in-degree, g ~ U[3,7], nu_ext/nu_thr ~ U[0.9,4], tau_syn 0.5 ms, float32)
+    advanced together; the launch shape comes from the autotuner run on a
+    sample of the networks, which are also run one by one for comparison."""
+    import numpy as np
+    from brainpy.state_amd import Ensemble
+    from brainpy.state_amd.networks import brunel_exp_ensemble
+    M, steps = args.ensemble, args.ensemble_steps
+    t0 = time.time()
+    sims, prm = brunel_exp_ensemble(M, seed=0)
+    syn = int(sum(0.1 * n * n for n in prm['N']))
+    log(f'ensemble: {M} networks, {int(prm["N"].sum())} neurons, ~{syn:.3e} synapses, '
+        f'built in {time.time() - t0:.1f}s')
+    rng = np.random.default_rng(1)
+    sample = sorted(rng.choice(M, size=min(16, M), replace=False).tolist())
+    best, table = Ensemble.autotune([sims[k] for k in sample], steps=300)
+    log('autotune (tile_size, max_window) -> ms/step: ' +
+        ', '.join(f'{k}: {v * 1e3:.3f}' for k, v in sorted(table.items())))
+    t1 = time.time()
+    ens = Ensemble(sims, tile_size=best[0], max_window=best[1])
+    log(f'engine created in {time.time() - t1:.1f}s (tile {best[0]}, window bound {best[1]})')
+    ens.reset()
+    ens.run_raw(200)
+    t2 = time.perf_counter()
+    vm, (g_all, st_all) = ens.run_raw(steps)
+    wall = time.perf_counter() - t2
+    by = ens.engine_bytes()
+    parts = ens.split(vm, g_all, st_all)
+    n_sp = len(g_all)
+    counts_ens = {k: len(parts[k][1]) for k in sample}
+    ens.close()
+    log(f'ensemble: {steps} steps in {wall:.3f}s, {n_sp} spikes, {by["n_events"]:.3e} events')
+    # the sample networks one by one (plain runs), same steps after the same warm-up
+    seq = 0.0
+    same = True
+    for k in sample:
+        eng = sims[k]._ensure_engine()
+        eng.reset()
+        eng.run(200)
+        t3 = time.perf_counter()
+        _, (g, _) = eng.run(steps)
+        seq += time.perf_counter() - t3
+        same &= len(g) == counts_ens[k]
+        eng.close()
+        sims[k]._engine = None
+    t4 = time.time()
+    ens_s = Ensemble([sims[k] for k in sample], tile_size=best[0], max_window=best[1])
+    ens_s.reset()
+    ens_s.run_raw(200)
+    t5 = time.perf_counter()
+    ens_s.run_raw(steps)
+    ens_sample = time.perf_counter() - t5
+    ens_s.close()
+    log(f'sample of {len(sample)}: sequential plain runs {seq:.3f}s, one ensemble {ens_sample:.3f}s, '
+        f'spike counts identical: {same}')
+    return {'metric': 'synaptic events/s (batched ensemble, 1 GPU)', 'value': round(by['n_events'] / wall, 1),
+            'unit': 'events/s', 'steps': steps, 'ms_per_step': round(wall / steps * 1e3, 4),
+            'rtf': round(wall / (steps * sims[0].dt * 1e-3), 4), 'dtype': 'f32',
+            'config': {'workload': 'brunel iaf_psc_exp ensemble', 'networks': M,
+                       'neurons': int(prm['N'].sum()), 'synapses_approx': syn, 'tile_size': best[0],
+                       'max_window': best[1]},
+            'autotune_ms_per_step': {f'{k[0]}/{k[1]}': round(v * 1e3, 4) for k, v in table.items()},
+            'sample': {'networks': len(sample), 'sequential_s': round(seq, 4),
+                       'ensemble_s': round(ens_sample, 4), 'spike_counts_identical': bool(same)}}
 
 
 def run_bptt(args):

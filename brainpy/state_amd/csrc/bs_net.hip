@@ -747,6 +747,7 @@ struct bs_net {
   std::vector<DevChan> chans;
   std::vector<Tile> tiles;
   std::vector<int> pop_record;
+  bool all_recorded = true;  // every population keeps its spikes for the caller
   // device buffers
   Tile* d_tiles = nullptr;
   DevPop* d_pops = nullptr;
@@ -1032,6 +1033,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
       net->tiles.push_back(Tile{p, s0, std::min(TS, sp.n - s0), 0});
     gid += sp.n;
     net->pop_record.push_back(sp.record_spikes);
+    if (!sp.record_spikes) net->all_recorded = false;
   }
   net->Ntot = gid;
   if (gid >= (1ll << 31)) BAD("more than 2^31 neurons");
@@ -1345,7 +1347,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   CHK(dmalloc(&net->d_vm_col, gid));
   {  // spike record list: worst case every neuron fires every step of K windows
     const long long per_win = std::max<long long>(1, gid) * W;
-    long long k = (64ll << 20) / per_win;
+    long long k = (256ll << 20) / per_win;
     if (k < 1) k = 1;
     if (k > 512) k = 512;
     net->rec_cap = per_win * k;
@@ -1610,21 +1612,42 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
       HIPCHK(hipStreamSynchronize(net->stream));
     }
     const unsigned long long gmask = (1ull << net->gid_bits) - 1;
-    net->out_gid.reserve(net->out_gid.size() + cnt);
-    net->out_step.reserve(net->out_step.size() + cnt);
-    for (unsigned long long i = 0; i < cnt; ++i) {
-      const int g = (int)(hrec[i] & gmask);
-      const int st = (int)(c0 + (long long)(hrec[i] >> net->gid_bits));
-      const long long row = net->outdeg[g];
-      net->n_events += row;
-      // gather: the spike's bucket entry, its row's segment bounds, 4 B per edge
-      net->b_deliver += 8.0 + 4.0 * row;
-      net->b_update += 8.0;  // record entry
-      if (net->pop_record[net->gid_pop[g]]) {
-        net->out_gid.push_back(g);
-        net->out_step.push_back(st);
+    const size_t base = net->out_gid.size();
+    net->out_gid.resize(base + cnt);
+    net->out_step.resize(base + cnt);
+    size_t w = base;
+    long long ev = 0;
+    const int gb = net->gid_bits;
+    const long long* od = net->outdeg.data();
+    if (net->all_recorded) {
+      int32_t* og = net->out_gid.data();
+      int32_t* os = net->out_step.data();
+      for (unsigned long long i = 0; i < cnt; ++i) {
+        const unsigned long long k = hrec[i];
+        const int g = (int)(k & gmask);
+        ev += od[g];
+        og[w] = g;
+        os[w] = (int)(c0 + (long long)(k >> gb));
+        ++w;
+      }
+    } else {
+      for (unsigned long long i = 0; i < cnt; ++i) {
+        const unsigned long long k = hrec[i];
+        const int g = (int)(k & gmask);
+        ev += od[g];
+        if (net->pop_record[net->gid_pop[g]]) {
+          net->out_gid[w] = g;
+          net->out_step[w] = (int)(c0 + (long long)(k >> gb));
+          ++w;
+        }
       }
     }
+    net->out_gid.resize(w);
+    net->out_step.resize(w);
+    net->n_events += ev;
+    // gather: each spike's bucket entry and segment bounds, 4 B per edge; 8 B record
+    net->b_deliver += 8.0 * (double)cnt + 4.0 * (double)ev;
+    net->b_update += 8.0 * (double)cnt;
     if (vm_out && ntaps > 0) {
       hvm.resize((size_t)chunk * std::max(1, net->n_vm_dev));
       HIPCHK(hipMemcpy(hvm.data(), net->d_vm, 8 * hvm.size(), hipMemcpyDeviceToHost));

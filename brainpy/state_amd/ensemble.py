@@ -130,13 +130,16 @@ class Ensemble:
             s._base_step = 0
         self._ready = True
 
-    def run(self, n_steps):
-        """Advance every network ``n_steps``; returns per network
-        ``(vm [n_steps, n_taps], gids, steps)`` with network-local gids,
-        spikes sorted by (step, gid)."""
+    def run_raw(self, n_steps):
+        """Advance every network ``n_steps``; returns ``(vm, (gids, steps))``
+        with ensemble-global gids (network k owns ``desc.instances[k]``'s
+        range), spikes sorted by (step, gid)."""
         if not self._ready:
             self.reset()
-        vm, (g, st) = self._engine.run(int(n_steps))
+        return self._engine.run(int(n_steps))
+
+    def split(self, vm, g, st):
+        """Per-network ``(vm, gids, steps)`` of a :meth:`run_raw` result."""
         inst = self.desc.instances
         bounds = np.array([i['gid0'] for i in inst] + [self.desc.ntot])
         which = np.searchsorted(bounds, g, side='right') - 1
@@ -148,6 +151,13 @@ class Ensemble:
             a, b = cuts[k], cuts[k + 1]
             out.append((vm[:, i['tap0']:i['tap0'] + i['n_taps']], g[a:b] - i['gid0'], st[a:b]))
         return out
+
+    def run(self, n_steps):
+        """Advance every network ``n_steps``; returns per network
+        ``(vm [n_steps, n_taps], gids, steps)`` with network-local gids,
+        spikes sorted by (step, gid)."""
+        vm, (g, st) = self.run_raw(n_steps)
+        return self.split(vm, g, st)
 
     def simulate(self, duration):
         """Reset every network and run ``duration``; one SimulationResult each."""
