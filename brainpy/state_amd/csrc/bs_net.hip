@@ -128,6 +128,14 @@ struct WinArgs {
   unsigned char* spk_hist;  // [Ls][Ntot] spike flags (plastic networks), or null
   int max_span;        // longest gathered step range, LDS prefix size
   int TS;              // neurons per tile
+  // arrival ring of events that land after their delivery window:
+  // ring_cnt [Lr][NCW][Ntot] packed counts, ring_fx [Lr][NFX][Ntot] quanta
+  uint32_t* ring_cnt;
+  long long* ring_fx;
+  int Lr, NCW, NFX;
+  int H;                    // LDS counters cover arrival offsets 0 .. H-1
+  int prevlen;              // steps of the previous window (its spikes are gathered now)
+  const int* prevlen_ptr;   // graph replays, first window of the chunk
   unsigned long long* rec;  // (step - vm_step0) << gid_bits | gid
   int gid_bits;
   unsigned long long* rec_count;
@@ -170,12 +178,13 @@ struct WinLayout {
 __host__ __device__ inline size_t al8(size_t x) { return (x + 7) & ~(size_t)7; }
 
 // LDS layout of k_window for one population (tab_bytes: staged tables).
-__host__ __device__ inline WinLayout win_layout(int Wm, int TS, int ncw, int nfx, int ngp, int SP,
+// H: arrival offsets held in LDS counters (>= Wm).
+__host__ __device__ inline WinLayout win_layout(int Wm, int H, int TS, int ncw, int nfx, int ngp, int SP,
                                                 int nch, int nrec, int tsz, size_t tab_bytes) {
   WinLayout L;
   L.cnt = 0;
-  L.fxa = al8(L.cnt + (size_t)Wm * ncw * TS * 4);
-  L.spl = al8(L.fxa + (size_t)Wm * nfx * TS * 8);
+  L.fxa = al8(L.cnt + (size_t)H * ncw * TS * 4);
+  L.spl = al8(L.fxa + (size_t)H * nfx * TS * 8);
   L.spn = al8(L.spl + (size_t)Wm * TS * 2);
   L.base = al8(L.spn + (size_t)Wm * 4);
   L.pref = al8(L.base + (size_t)Wm * 16);
@@ -337,7 +346,8 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   const int ncw = gpop.n_cw, nfx = gpop.n_fx, ngp = gpop.gp_count, nch = gpop.ch_count;
   const int SP = a.max_span + 1;
   const int nrec = gpop.model == BS_AEIF_COND_ALPHA ? 2 : 1;
-  const WinLayout Lo = win_layout(Wm, TS, ncw, nfx, ngp, SP, nch, nrec, (int)sizeof(T), gpop.tab_bytes);
+  const int H = a.H;
+  const WinLayout Lo = win_layout(Wm, H, TS, ncw, nfx, ngp, SP, nch, nrec, (int)sizeof(T), gpop.tab_bytes);
   uint32_t* cnt = (uint32_t*)(lds + Lo.cnt);
   long long* fxa = (long long*)(lds + Lo.fxa);
   unsigned short* spl = (unsigned short*)(lds + Lo.spl);
@@ -359,8 +369,28 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
       a.sb_count[(st % a.Ls) * a.n_pops + t % a.n_pops] = 0u;
     }
   }
-  for (int k = tid; k < W * ncw * TS; k += 256) cnt[k] = 0u;
-  for (int k = tid; k < W * nfx * TS; k += 256) fxa[k] = 0;
+  // events already deposited for this window's steps: take them out of the ring
+  const long long tile_gid0 = (long long)gpop.gid0 + tl.start;
+  for (int k = tid; k < H * ncw * TS; k += 256) {
+    const int w = k / (ncw * TS), r = k - w * ncw * TS, cw = r / TS, ln = r - cw * TS;
+    uint32_t v = 0u;
+    if (w < W && ln < tl.count) {
+      const size_t o = ((size_t)((a.n0 + w) % a.Lr) * a.NCW + cw) * a.Ntot + tile_gid0 + ln;
+      v = a.ring_cnt[o];
+      a.ring_cnt[o] = 0u;
+    }
+    cnt[k] = v;
+  }
+  for (int k = tid; k < H * nfx * TS; k += 256) {
+    const int w = k / (nfx * TS), r = k - w * nfx * TS, fx = r / TS, ln = r - fx * TS;
+    long long v = 0;
+    if (w < W && ln < tl.count) {
+      const size_t o = ((size_t)((a.n0 + w) % a.Lr) * a.NFX + fx) * a.Ntot + tile_gid0 + ln;
+      v = a.ring_fx[o];
+      a.ring_fx[o] = 0;
+    }
+    fxa[k] = v;
+  }
   for (int k = tid; k < Wm; k += 256) spn[k] = 0;
   for (int k = tid; k < BS_NPARAM; k += 256) P[k] = gpop.P[k];
   {
@@ -382,15 +412,15 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
       for (int k = tid; k < 64; k += 256) dg[k] = ch.guide[k];
     }
   }
-  // bucket sizes of every (projection, step) the gather reads, loaded in parallel
+  // bucket sizes of every (projection, step) the gather reads: the spikes of
+  // the previous window, each delivered once with all of its delays
+  const int PL = (a.base_ptr && a.widx == 0) ? *a.prevlen_ptr : a.prevlen;
   for (int k = tid; k < ngp * SP; k += 256) {
     const int g = k / SP, t = k % SP;
     const GProj& G = a.gproj[gpop.gp_begin + g];
-    int s_lo = a.n0 - 1 - G.d_max;
-    if (s_lo < 0) s_lo = 0;
-    const int s_hi = a.n0 + W - 2 - G.d_min;
+    const int s_lo = a.n0 - PL;
     const int s = s_lo + t;
-    pref[k] = (t > 0 && s - 1 <= s_hi) ? (int)a.sb_count[((s - 1) % a.Ls) * a.n_pops + G.pre_pop] : 0;
+    pref[k] = (t > 0 && t <= PL) ? (int)a.sb_count[((s - 1) % a.Ls) * a.n_pops + G.pre_pop] : 0;
     if (t == 0) {
       const DevChan& ch = a.chans[G.chan];
       GInfo I;
@@ -499,19 +529,45 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
           const uint32_t e = e0[q] + (uint32_t)(rd * 8 + u);
           if (e < e1[q]) {
             const int w = dn[q] + (int)(wd[q][u] >> 8);
-            if (w >= 0 && w < W) {
-              const int ln = (int)(wd[q][u] & 255u);
+            const int ln = (int)(wd[q][u] & 255u);
+            if (w < H) {  // an LDS counter (this window's steps, and later ones when they fit)
               if (kc[q])
                 atomicAdd(&cnt[(w * ncw + cwq[q]) * TS + ln], incq[q]);
               else
                 atomicAdd((unsigned long long*)&fxa[(w * nfx + fxq[q]) * TS + ln],
                           (unsigned long long)qp[q][e]);
+            } else {  // beyond the LDS horizon: this tile's slot of the arrival ring (no other block touches it)
+              const size_t slot = (size_t)((a.n0 + w) % a.Lr);
+              if (kc[q])
+                __hip_atomic_fetch_add(&a.ring_cnt[(slot * a.NCW + cwq[q]) * a.Ntot + tile_gid0 + ln],
+                                       incq[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              else
+                __hip_atomic_fetch_add(
+                    (unsigned long long*)&a.ring_fx[(slot * a.NFX + fxq[q]) * a.Ntot + tile_gid0 + ln],
+                    (unsigned long long)qp[q][e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
           }
         }
     }
   }
   __syncthreads();
+  // arrivals after this window held in LDS: add them to this tile's ring slots
+  for (int k = W * ncw * TS + tid; k < H * ncw * TS; k += 256) {
+    const int w = k / (ncw * TS), r = k - w * ncw * TS, cw = r / TS, ln = r - cw * TS;
+    const uint32_t v = cnt[k];
+    if (v != 0u && ln < tl.count) {
+      const size_t o = ((size_t)((a.n0 + w) % a.Lr) * a.NCW + cw) * a.Ntot + tile_gid0 + ln;
+      a.ring_cnt[o] += v;
+    }
+  }
+  for (int k = W * nfx * TS + tid; k < H * nfx * TS; k += 256) {
+    const int w = k / (nfx * TS), r = k - w * nfx * TS, fx = r / TS, ln = r - fx * TS;
+    const long long v = fxa[k];
+    if (v != 0 && ln < tl.count) {
+      const size_t o = ((size_t)((a.n0 + w) % a.Lr) * a.NFX + fx) * a.Ntot + tile_gid0 + ln;
+      a.ring_fx[o] += v;
+    }
+  }
 
   // ---- 2. inputs of every (step, lane), channels summed in order, all threads
   {
@@ -779,6 +835,14 @@ struct bs_net {
   std::vector<std::vector<long long>> w_ref;  // reference CSR position -> stored position
   long long* d_base = nullptr;     // first step of the chunk a graph replay advances
   long long h_base = 0;
+  // arrival ring (events landing after their delivery window), per tile slice
+  uint32_t* d_ring_cnt = nullptr;
+  long long* d_ring_fx = nullptr;
+  int Lr = 1, NCW = 1, NFX = 0;
+  int H = 1;                       // arrival offsets held in LDS by k_window
+  int* d_prevlen = nullptr;        // graph replays: length of the window before the chunk
+  int h_prevlen = 0;
+  int last_win = 0;                // length of the last window run (0 after reset)
   hipGraphExec_t graph = nullptr;  // one full chunk of window launches
   std::vector<void*> owned;  // per-projection gather arrays, generator maps
   // host copies
@@ -815,7 +879,7 @@ static void free_net(bs_net* net) {
   F(net->d_state); F(net->d_ref); F(net->d_sb); F(net->d_sb_count); F(net->d_rec); F(net->d_rec_sorted); F(net->d_sort_tmp);
   F(net->d_rec_count); F(net->d_flags); F(net->d_vm_col); F(net->d_vm);
   for (void* p : net->owned) F(p);
-  F(net->d_base);
+  F(net->d_base); F(net->d_ring_cnt); F(net->d_ring_fx); F(net->d_prevlen);
   F(net->d_stdp);
   F(net->d_spk_hist);
   if (net->graph) (void)hipGraphExecDestroy(net->graph);
@@ -1256,29 +1320,33 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   if (W > maxw) W = maxw;
   if (W < 1) W = 1;
   // LDS layout of k_window (win_layout), first without staged tables
-  auto span_of = [&](int w) {
-    int span = 1;
-    for (const GProj& G : gp_sorted) span = std::max(span, G.d_max - G.d_min + w);
-    return span;
-  };
+  // each window gathers the spikes of the previous one (at most W steps)
+  auto span_of = [&](int w) { return w; };
+  // LDS counters for every arrival offset 0..d_max when they fit (the window's
+  // later arrivals are then added to the ring with plain stores), else only
+  // for the window's own W steps (later arrivals go to the ring atomically)
+  bool full_h = true;
+  auto hz = [&](int w) { return full_h ? std::max(w, dmax + 1) : w; };
   auto lds_of = [&](int w) {
     size_t best = 0;
     const int SP = span_of(w) + 1;
     for (const DevPop& p : net->pops)
-      best = std::max(best, win_layout(w, TS, p.n_cw, p.n_fx, p.gp_count, SP, p.ch_count,
+      best = std::max(best, win_layout(w, hz(w), TS, p.n_cw, p.n_fx, p.gp_count, SP, p.ch_count,
                                        p.model == BS_AEIF_COND_ALPHA ? 2 : 1, (int)net->tsize, 0).end);
     return best;
   };
+  if (lds_of(W) > kLdsBudget) full_h = false;
   while (W > 1 && lds_of(W) > kLdsBudget) --W;
   if (lds_of(W) > kLdsBudget) BAD("too many input channels per population for the LDS budget");
   net->W = W;
+  net->H = hz(W);
   net->max_span = span_of(W);
   // stage count tables and Poisson CDFs into LDS while they fit
   {
     const int SP = net->max_span + 1;
     size_t need = 0;
     for (DevPop& p : net->pops) {
-      const size_t base_end = win_layout(W, TS, p.n_cw, p.n_fx, p.gp_count, SP, p.ch_count,
+      const size_t base_end = win_layout(W, net->H, TS, p.n_cw, p.n_fx, p.gp_count, SP, p.ch_count,
                                          p.model == BS_AEIF_COND_ALPHA ? 2 : 1, (int)net->tsize, 0).end;
       size_t off = 0;
       for (int c = p.ch_begin; c < p.ch_begin + p.ch_count; ++c) {
@@ -1338,6 +1406,19 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   CHK(dmalloc(&net->d_rec_count, 1));
   CHK(dmalloc(&net->d_flags, 1));
   CHK(dmalloc(&net->d_base, 1));
+  CHK(dmalloc(&net->d_prevlen, 1));
+  {  // arrival ring: slots for steps n0 .. n0 + d_max
+    net->Lr = net->d_max + 1;
+    int ncw = 1, nfx = 0;
+    for (const DevPop& p : net->pops) {
+      ncw = std::max(ncw, p.n_cw);
+      nfx = std::max(nfx, p.n_fx);
+    }
+    net->NCW = ncw;
+    net->NFX = nfx;
+    CHK(dmalloc(&net->d_ring_cnt, (size_t)net->Lr * ncw * std::max<long long>(1, gid)));
+    CHK(dmalloc(&net->d_ring_fx, nfx > 0 ? (size_t)net->Lr * nfx * std::max<long long>(1, gid) : 1));
+  }
   if (!net->stdp.empty()) {
     CHK(dmalloc(&net->d_spk_hist, (size_t)net->Ls * gid));
     CHK(dmalloc(&net->d_stdp, net->stdp.size()));
@@ -1417,8 +1498,12 @@ static int upload_state(bs_net* net) {
       HIPCHK(hipMemcpy(S.w, net->w0[q].data(), 8 * net->w0[q].size(), hipMemcpyHostToDevice));
   }
   if (net->d_spk_hist) HIPCHK(hipMemset(net->d_spk_hist, 0, (size_t)net->Ls * gid));
+  HIPCHK(hipMemset(net->d_ring_cnt, 0, 4 * (size_t)net->Lr * net->NCW * std::max<long long>(1, gid)));
+  if (net->NFX > 0)
+    HIPCHK(hipMemset(net->d_ring_fx, 0, 8 * (size_t)net->Lr * net->NFX * std::max<long long>(1, gid)));
   HIPCHK(hipDeviceSynchronize());
   net->step = 0;
+  net->last_win = 0;
   return BS_OK;
 }
 
@@ -1436,9 +1521,17 @@ int bs_net_step(bs_net* net, int64_t* step) {
 }
 
 template <typename T>
-static void launch_window(bs_net* net, int n0, int W, long long vm_step0,
+static void launch_window(bs_net* net, int n0, int W, long long vm_step0, int prevlen,
                           const long long* base_ptr = nullptr, int widx = 0) {
   WinArgs a;
+  a.ring_cnt = net->d_ring_cnt;
+  a.ring_fx = net->d_ring_fx;
+  a.Lr = net->Lr;
+  a.H = net->H;
+  a.NCW = net->NCW;
+  a.NFX = net->NFX;
+  a.prevlen = prevlen;
+  a.prevlen_ptr = net->d_prevlen;
   a.base_ptr = base_ptr;
   a.widx = widx;
   a.spk_hist = net->d_spk_hist;
@@ -1546,9 +1639,9 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
         for (int w = 0; w < nwin; ++w) {
           if (!net->stdp.empty()) launch_stdp(net, 0, net->d_base, w);
           if (net->tsize == 8)
-            launch_window<double>(net, 0, net->W, 0, net->d_base, w);
+            launch_window<double>(net, 0, net->W, 0, net->W, net->d_base, w);
           else
-            launch_window<float>(net, 0, net->W, 0, net->d_base, w);
+            launch_window<float>(net, 0, net->W, 0, net->W, net->d_base, w);
         }
         HIPCHK(hipStreamEndCapture(net->stream, &g));
         hipError_t e = hipGraphInstantiate(&net->graph, g, nullptr, nullptr, 0);
@@ -1556,9 +1649,13 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
         if (e != hipSuccess) return fail(BS_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
       }
       net->h_base = c0;  // stays valid until the chunk's synchronisation below
+      net->h_prevlen = net->last_win;
       HIPCHK(hipMemcpyAsync(net->d_base, &net->h_base, sizeof(long long), hipMemcpyHostToDevice,
                             net->stream));
+      HIPCHK(hipMemcpyAsync(net->d_prevlen, &net->h_prevlen, sizeof(int), hipMemcpyHostToDevice,
+                            net->stream));
       HIPCHK(hipGraphLaunch(net->graph, net->stream));
+      net->last_win = net->W;
       net->n_update += nwin;
       net->b_update += state_bytes * chunk;
     } else {
@@ -1568,9 +1665,10 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
         if (net->timing) HIPCHK(hipEventRecord(evs[2 * w], net->stream));
         if (!net->stdp.empty()) launch_stdp(net, (int)n0, nullptr, 0);
         if (net->tsize == 8)
-          launch_window<double>(net, (int)n0, Wc, c0);
+          launch_window<double>(net, (int)n0, Wc, c0, net->last_win);
         else
-          launch_window<float>(net, (int)n0, Wc, c0);
+          launch_window<float>(net, (int)n0, Wc, c0, net->last_win);
+        net->last_win = Wc;
         if (net->timing) HIPCHK(hipEventRecord(evs[2 * w + 1], net->stream));
         HIPCHK(hipGetLastError());
         net->n_update++;

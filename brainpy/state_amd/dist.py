@@ -11,7 +11,7 @@ import numpy as np
 
 from . import units as u
 
-__all__ = ['Distribution', 'Normal', 'LogNormal', 'Uniform']
+__all__ = ['Distribution', 'Normal', 'LogNormal', 'Uniform', 'UniformGrid']
 
 
 def _split(x):
@@ -65,3 +65,19 @@ class Uniform(Distribution):
             hi = float(u.Quantity(hi, hu).to_decimal(lu))
         x = lo + (hi - lo) * np.random.Generator(np.random.PCG64(seed)).random(shape)
         return x if lu is None else u.Quantity(x, lu)
+
+
+@dataclass
+class UniformGrid(Distribution):
+    """Uniform over the grid ``low, low + step, ..., high`` (e.g. delays in whole
+    steps of dt)."""
+    low: float
+    high: float
+    step: float
+
+    def sample(self, shape, seed):
+        lo, hi, st = (float(u.scalar(x, 'time')) if isinstance(x, u.Quantity) else float(x)
+                      for x in (self.low, self.high, self.step))
+        k0, k1 = int(round(lo / st)), int(round(hi / st))
+        k = np.random.Generator(np.random.PCG64(seed)).integers(k0, k1 + 1, size=shape)
+        return k.astype(np.float64) * st

@@ -17,9 +17,11 @@ import numpy as np
 
 from . import units as u
 from .devices import poisson_generator, spike_recorder, voltmeter
+from .dist import UniformGrid
 from .init import Constant
-from .models import iaf_psc_delta, iaf_psc_exp
-from .rules import fixed_indegree
+from .models import aeif_cond_alpha, iaf_psc_delta, iaf_psc_exp
+from .plasticity import stdp_synapse
+from .rules import fixed_indegree, pairwise_bernoulli
 from .simulator import Simulator
 
 __all__ = ['brunel_delta', 'brunel_exp']
@@ -142,3 +144,44 @@ def brunel_exp_ensemble(n_networks=256, *, n_min=500, n_max=20000, g_range=(3.0,
                             dtype=dtype, **kw)
         sims.append(sim)
     return sims, dict(N=N, g=g, eta=eta)
+
+
+def aeif_stdp(n=50000, *, frac_e=0.8, p=0.01, g_E=1.0, g_I=4.0, rate=8000.0, w_noise=1.0,
+              delay=(1.0, 5.0), dt=0.1, seed=1, n_vm=10, plastic=True, tau_plus=20.0,
+              lambda_=0.01, alpha=1.05, Wmax=3.0, mu=0.0, dtype='float64', record=True):
+    """Conductance-based network of aeif_cond_alpha neurons (NEST defaults:
+    a = 4 nS, b = 80.5 pA, tau_w = 144 ms, Delta_T = 2 mV; adaptive RKF45),
+    random connectivity with probability ``p``, g_E / g_I in nS, per-edge delays
+    uniform over the dt grid in ``delay`` ms, stdp_synapse (additive for
+    ``mu = 0``) on E->E, Poisson drive ``rate`` Hz at ``w_noise`` nS on every
+    neuron. Returns ``(sim, handles)``."""
+    NE = int(round(frac_e * n))
+    NI = int(n) - NE
+    sim = Simulator(dt=dt * u.ms, dtype=dtype)
+    ne = sim.create(aeif_cond_alpha, NE)
+    ni = sim.create(aeif_cond_alpha, NI)
+    noise = sim.create(poisson_generator, rate=rate * u.Hz)
+    d = UniformGrid(delay[0], delay[1], dt)
+    sim.connect(noise, ne + ni, weight=w_noise, delay=dt * u.ms, receptor_type=1)
+    if plastic:
+        syn = stdp_synapse(weight=g_E, tau_plus=tau_plus, lambda_=lambda_, alpha=alpha,
+                           mu_plus=mu, mu_minus=mu, Wmax=Wmax)
+        pee = sim.connect(ne, ne, rule=pairwise_bernoulli(p), synapse=syn, delay=d, seed=seed,
+                          receptor_type=1, allow_autapses=False)
+    else:
+        pee = sim.connect(ne, ne, weight=g_E, rule=pairwise_bernoulli(p), delay=d, seed=seed,
+                          receptor_type=1, allow_autapses=False)
+    sim.connect(ne, ni, weight=g_E, rule=pairwise_bernoulli(p), delay=d, seed=seed + 1,
+                receptor_type=1)
+    sim.connect(ni, ne + ni, weight=g_I, rule=pairwise_bernoulli(p), delay=d, seed=seed + 2,
+                receptor_type=2)
+    esr = isr = vm = None
+    if record:
+        esr = sim.create(spike_recorder)
+        isr = sim.create(spike_recorder)
+        sim.connect(ne, esr)
+        sim.connect(ni, isr)
+    if n_vm:
+        vm = sim.create(voltmeter)
+        sim.connect(vm, ne[:n_vm])
+    return sim, dict(ne=ne, ni=ni, noise=noise, pee=pee, esr=esr, isr=isr, vm=vm)
