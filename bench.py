@@ -55,6 +55,11 @@ def main():
     ap.add_argument('--ensemble', type=int, default=0, metavar='M',
                     help='instead: the batched-ensemble workload with M networks (256 = full size)')
     ap.add_argument('--ensemble-steps', type=int, default=5000)
+    ap.add_argument('--aeif-stdp', type=int, default=0, metavar='STEPS',
+                    help='instead: the conductance-based aeif_cond_alpha + stdp_synapse network '
+                         '(50,000 neurons, p = 0.01) for STEPS steps')
+    ap.add_argument('--aeif-wnoise', type=float, default=1.0,
+                    help='Poisson drive weight in nS (1.0 as specified leaves the network silent)')
     ap.add_argument('--bptt-only', action='store_true',
                     help='run only the BPTT leg (for rocprofv3)')
     ap.add_argument('--direct', action='store_true',
@@ -70,6 +75,9 @@ def main():
 
     if args.ensemble:
         print(json.dumps(run_ensemble(args)), flush=True)
+        return
+    if args.aeif_stdp:
+        print(json.dumps(run_aeif_stdp(args)), flush=True)
         return
     if args.bptt_only:
         args.no_cpu = True
@@ -290,6 +298,37 @@ def run_ensemble(args):
             'autotune_ms_per_step': {f'{k[0]}/{k[1]}': round(v * 1e3, 4) for k, v in table.items()},
             'sample': {'networks': len(sample), 'sequential_s': round(seq, 4),
                        'ensemble_s': round(ens_sample, 4), 'spike_counts_identical': bool(same)}}
+
+
+def run_aeif_stdp(args):
+    """aeif_cond_alpha (adaptive RKF45) 40,000 E + 10,000 I, p = 0.01 (~25M
+    synapses), stdp_synapse (additive) on E->E (~16M plastic edges), delays
+    1-5 ms per edge, Poisson 8 kHz at --aeif-wnoise nS, float64."""
+    import numpy as np
+    from brainpy.state_amd.networks import aeif_stdp
+    t0 = time.time()
+    sim, h = aeif_stdp(n=50000, w_noise=args.aeif_wnoise, n_vm=0)
+    desc = sim.build_desc()
+    E = sum(desc.c.projs[k].n_edges for k in range(desc.c.n_projs))
+    eng = sim._ensure_engine()
+    log(f'aeif+stdp: {E} synapses, built in {time.time() - t0:.1f}s')
+    eng.reset()
+    t1 = time.perf_counter()
+    _, (g, st) = eng.run(args.aeif_stdp)
+    wall = time.perf_counter() - t1
+    by = eng.bytes()
+    w = sim.get_weights(h['pee'])
+    sim_s = args.aeif_stdp * sim.dt * 1e-3
+    log(f'aeif+stdp: {args.aeif_stdp} steps in {wall:.2f}s, {len(g)} spikes '
+        f'({len(g) / 50000 / sim_s:.2f} Hz), weights mean {w.mean():.4f} [{w.min():.4f}, {w.max():.4f}]')
+    return {'metric': 'synaptic events/s (aeif_cond_alpha + stdp_synapse, 1 GPU)',
+            'value': round(by['n_events'] / wall, 1), 'unit': 'events/s', 'steps': args.aeif_stdp,
+            'ms_per_step': round(wall / args.aeif_stdp * 1e3, 4), 'rtf': round(wall / sim_s, 4),
+            'dtype': 'f64', 'rate_hz': round(len(g) / 50000 / sim_s, 3),
+            'config': {'workload': 'aeif_cond_alpha 40000 E + 10000 I, p=0.01, stdp_synapse E->E, '
+                                   'delays 1-5 ms, Poisson 8 kHz', 'w_noise_nS': args.aeif_wnoise,
+                       'synapses': int(E)},
+            'weights': {'mean': float(w.mean()), 'min': float(w.min()), 'max': float(w.max())}}
 
 
 def run_bptt(args):
