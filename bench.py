@@ -58,6 +58,8 @@ def main():
     ap.add_argument('--aeif-stdp', type=int, default=0, metavar='STEPS',
                     help='instead: the conductance-based aeif_cond_alpha + stdp_synapse network '
                          '(50,000 neurons, p = 0.01) for STEPS steps')
+    ap.add_argument('--fp32-report', type=int, default=0, metavar='STEPS',
+                    help='instead: Brunel-11250 in float32 against float64 over STEPS steps')
     ap.add_argument('--aeif-wnoise', type=float, default=1.0,
                     help='Poisson drive weight in nS (1.0 as specified leaves the network silent)')
     ap.add_argument('--bptt-only', action='store_true',
@@ -78,6 +80,9 @@ def main():
         return
     if args.aeif_stdp:
         print(json.dumps(run_aeif_stdp(args)), flush=True)
+        return
+    if args.fp32_report:
+        print(json.dumps(run_fp32_report(args)), flush=True)
         return
     if args.bptt_only:
         args.no_cpu = True
@@ -298,6 +303,30 @@ def run_ensemble(args):
             'autotune_ms_per_step': {f'{k[0]}/{k[1]}': round(v * 1e3, 4) for k, v in table.items()},
             'sample': {'networks': len(sample), 'sequential_s': round(seq, 4),
                        'ensemble_s': round(ens_sample, 4), 'spike_counts_identical': bool(same)}}
+
+
+def run_fp32_report(args):
+    """Brunel-11250 (per-edge delays) in float32 against float64 (the float64
+    engine is bit-exact with the float64 oracle): V_m error before the first
+    raster divergence, and the deviation of the mean rate and ISI CV."""
+    from brainpy.state_amd import stats
+    from brainpy.state_amd.networks import brunel_delta
+    n = args.fp32_report
+    runs = {}
+    for dt in ('float64', 'float32'):
+        sim, h = brunel_delta(order=2250, delay=('uniform', 1.0, 3.0), n_vm=10, dtype=dt)
+        eng = sim._ensure_engine()
+        eng.reset()
+        t0 = time.perf_counter()
+        vm, (g, st) = eng.run(n)
+        runs[dt] = (g, st, vm)
+        log(f'{dt}: {n} steps in {time.perf_counter() - t0:.3f}s, {len(g)} spikes')
+        eng.close()
+        sim._engine = None
+    rep = stats.compare_runs(runs['float64'], runs['float32'], 11250, n, 0.1)
+    log(json.dumps(rep))
+    return {'fp32_report': rep, 'steps': n, 'config': {'workload': 'brunel-11250 iaf_psc_delta, '
+                                                                   'per-edge delays 1.0-3.0 ms'}}
 
 
 def run_aeif_stdp(args):

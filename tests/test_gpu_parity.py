@@ -191,3 +191,21 @@ def test_generator_subsets_and_zero_delay_window():
     gpu, orc = _both(sim, 1000)
     assert len(orc[1]) > 100
     _assert_same(gpu, orc)
+
+
+def test_brunel_11250_fp32_statistics_within_distributional_band():
+    """float32 against float64 over 4 s: mean rate and ISI CV within 5 %
+    (the reference's distributional category); V_m agrees to float32 precision
+    until the rasters first diverge."""
+    from brainpy.state_amd import stats
+    runs = {}
+    for dt in ('float64', 'float32'):
+        sim, h = brunel_delta(order=2250, delay=('uniform', 1.0, 3.0), n_vm=10, dtype=dt)
+        eng = sim._ensure_engine()
+        eng.reset()
+        vm, (g, s) = eng.run(40000)
+        runs[dt] = (g, s, vm)
+    rep = stats.compare_runs(runs['float64'], runs['float32'], 11250, 40000, 0.1)
+    assert rep['rate_rel_dev'] < 0.05 and rep['cv_isi_rel_dev'] < 0.05, rep
+    assert rep['first_divergence_step'] is not None and rep['first_divergence_step'] > 10
+    assert rep['vm_max_abs_err_mV_before_divergence'] < 1e-3, rep

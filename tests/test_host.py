@@ -173,3 +173,18 @@ def test_recorder_taps_registered():
     d = sim.build_desc()
     assert d.c.pops[0].record_spikes == 1
     assert d.c.pops[0].n_vm == 3 and d.n_taps == 3
+
+
+def test_stats_rate_cv_and_divergence():
+    from brainpy.state_amd import stats
+    # two neurons: regular (CV 0) and alternating ISIs 2, 4 (CV 1/3)
+    g = np.array([0, 0, 0, 0, 1, 1, 1, 1])
+    s = np.array([0, 5, 10, 15, 0, 2, 6, 8])
+    assert stats.rate(s, 2, 100, 0.1) == pytest.approx(8 / 2 / 0.01)
+    assert stats.cv_isi(g, s, 2) == pytest.approx((0.0 + np.std([2, 4, 2]) / np.mean([2, 4, 2])) / 2)
+    o = np.lexsort((g, s))
+    a_g, a_s = g[o], s[o]
+    assert stats.first_divergence(a_g, a_s, a_g, a_s) is None
+    b_s = a_s.copy()
+    b_s[-1] += 1
+    assert stats.first_divergence(a_g, a_s, a_g, b_s) == 15
