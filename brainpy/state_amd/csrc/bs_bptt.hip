@@ -57,7 +57,7 @@ namespace {
 
 constexpr int kTile = 256;       // neurons per block
 constexpr int kMaxWords = 128;   // N <= 8192
-constexpr int kMaxIn = 2048;     // input sources
+constexpr int kMaxIn = 512;      // input sources
 
 struct Consts {
   float E_L, V_th, V_reset, I_e, P22, P21ex, P21in, P11ex, P11in, P20, span, alpha;
@@ -124,6 +124,7 @@ __global__ __launch_bounds__(kTile) void k_bptt_fwd(FwdArgs a) {
   __shared__ unsigned char ev_sc[kEvCap];
   __shared__ unsigned char ev_k[kEvCap];
   __shared__ float xs[2][kChunk][kTile];  // summed inputs (ex, in) per step of the chunk
+  __shared__ unsigned char cl[kChunk][kMaxIn];  // input counts of the chunk's steps
   __shared__ int ev_off[kChunk + 1];
   __shared__ int st_cnt[kChunk];
   const int tid = (int)threadIdx.x, b = (int)blockIdx.y;
@@ -157,19 +158,33 @@ __global__ __launch_bounds__(kTile) void k_bptt_fwd(FwdArgs a) {
   const int nex_rows = a.n_in + a.n_exc;  // rows below this feed I_syn_ex
   for (int c0 = 0; c0 < a.nsteps; c0 += kChunk) {
     const int nc = min(kChunk, a.nsteps - c0);
-    // ---- count the events of every step of the chunk (one wave per step)
+    // ---- count the events of every step of the chunk (one wave per step):
+    // the step's spike words stay in registers, its input counts in LDS
+    static_assert(kChunk <= 2 * (kTile / 64), "two steps per wave");
+    unsigned long long wr[2][2];
     __syncthreads();
-    for (int k = wave; k < nc; k += kTile / 64) {
-      const int n = a.n0 + c0 + k;
-      const int s_in = n - a.d_in, s_rec = n - a.d;
-      int m = 0;
-      if (s_in >= 0)
-        for (int i = lane; i < a.n_in; i += 64) m += a.cnt[((size_t)s_in * a.B + b) * a.n_in + i] != 0;
-      if (s_rec >= 0)
-        for (int w = lane; w < a.NW; w += 64) m += __popcll(a.bm[((size_t)s_rec * a.B + b) * a.NW + w]);
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) m += __shfl_xor(m, o, 64);
-      if (lane == 0) st_cnt[k] = m;
+    for (int j = 0; j < 2; ++j) {
+      const int k = wave + j * (kTile / 64);
+      wr[j][0] = wr[j][1] = 0ull;
+      if (k < nc) {
+        const int n = a.n0 + c0 + k;
+        const int s_in = n - a.d_in, s_rec = n - a.d;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          const int w = lane + 64 * t;
+          if (s_rec >= 0 && w < a.NW) wr[j][t] = a.bm[((size_t)s_rec * a.B + b) * a.NW + w];
+        }
+        int m = __popcll(wr[j][0]) + __popcll(wr[j][1]);
+        for (int i = lane; i < a.n_in; i += 64) {
+          const unsigned char q = s_in >= 0 ? a.cnt[((size_t)s_in * a.B + b) * a.n_in + i] : (unsigned char)0;
+          cl[k][i] = q;
+          m += q != 0;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m += __shfl_xor(m, o, 64);
+        if (lane == 0) st_cnt[k] = m;
+      }
     }
     __syncthreads();
     if (tid == 0) {
@@ -186,39 +201,39 @@ __global__ __launch_bounds__(kTile) void k_bptt_fwd(FwdArgs a) {
       return;
     }
     // ---- write the lists: inputs ascending, then presynaptic neurons ascending
-    for (int k = wave; k < nc; k += kTile / 64) {
-      const int n = a.n0 + c0 + k;
-      const int s_in = n - a.d_in, s_rec = n - a.d;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int k = wave + j * (kTile / 64);
+      if (k >= nc) continue;
       int pos = ev_off[k];
-      if (s_in >= 0)
-        for (int i0 = 0; i0 < a.n_in; i0 += 64) {
-          const int i = i0 + lane;
-          const unsigned q = i < a.n_in ? a.cnt[((size_t)s_in * a.B + b) * a.n_in + i] : 0u;
-          const unsigned long long bal = __ballot(q != 0);
-          if (q) {
-            const int slot = pos + __popcll(bal & ((1ull << lane) - 1ull));
-            ev_row[slot] = (unsigned short)i;
-            ev_sc[slot] = (unsigned char)q;
-            ev_k[slot] = (unsigned char)k;
-          }
-          pos += __popcll(bal);
+      for (int i0 = 0; i0 < a.n_in; i0 += 64) {
+        const int i = i0 + lane;
+        const unsigned q = i < a.n_in ? cl[k][i] : 0u;
+        const unsigned long long bal = __ballot(q != 0);
+        if (q) {
+          const int slot = pos + __popcll(bal & ((1ull << lane) - 1ull));
+          ev_row[slot] = (unsigned short)i;
+          ev_sc[slot] = (unsigned char)q;
+          ev_k[slot] = (unsigned char)k;
         }
-      if (s_rec >= 0)
-        for (int w0 = 0; w0 < a.NW; w0 += 64) {
-          const int w = w0 + lane;
-          unsigned long long word = w < a.NW ? a.bm[((size_t)s_rec * a.B + b) * a.NW + w] : 0ull;
-          const int pc = __popcll(word);
-          int slot = pos + wave_excl_scan(pc, lane);
-          while (word) {
-            const int bit = __builtin_ctzll(word);
-            word &= word - 1;
-            ev_row[slot] = (unsigned short)(a.n_in + w * 64 + bit);
-            ev_sc[slot] = 1;
-            ev_k[slot] = (unsigned char)k;
-            ++slot;
-          }
-          pos = __shfl(slot, 63, 64);  // end of lane 63's bits
+        pos += __popcll(bal);
+      }
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        if (64 * t >= a.NW) break;
+        unsigned long long word = wr[j][t];
+        const int pc = __popcll(word);
+        int slot = pos + wave_excl_scan(pc, lane);
+        while (word) {
+          const int bit = __builtin_ctzll(word);
+          word &= word - 1;
+          ev_row[slot] = (unsigned short)(a.n_in + (lane + 64 * t) * 64 + bit);
+          ev_sc[slot] = 1;
+          ev_k[slot] = (unsigned char)k;
+          ++slot;
         }
+        pos = __shfl(slot, 63, 64);  // end of lane 63's bits
+      }
     }
     __syncthreads();
     // ---- gather: every event of the chunk in list order (per step: inputs,
@@ -577,6 +592,7 @@ struct bs_bptt {
   float* vbuf = nullptr;
   float *ibex = nullptr, *ibin = nullptr;
   unsigned short *w3 = nullptr, *ib3e = nullptr, *ib3i = nullptr;
+
   double* lpart_blocks = nullptr;
   float* sbar = nullptr;
   float *ay = nullptr, *ae = nullptr, *ai = nullptr;
@@ -628,6 +644,7 @@ FwdArgs fwd_args(bs_bptt* h) {
   a.loss = h->d.loss;
   a.w_in = h->w_in;
   a.w_rec = h->w_rec;
+
   a.cnt = h->cnt;
   a.bm = h->bm;
   a.bmT = h->bmT;
@@ -658,7 +675,7 @@ int bs_bptt_create(const bs_bptt_desc* d, bs_bptt** out) {
   if (d->n_exc < 0 || d->n_inh < 0 || N < 1 || N > 64 * kMaxWords)
     return bs_internal_fail(BS_EINVAL, "bs_bptt_create: need 1 <= n_exc + n_inh <= 8192");
   if (d->n_in < 0 || d->n_in > kMaxIn)
-    return bs_internal_fail(BS_EINVAL, "bs_bptt_create: need 0 <= n_in <= 2048");
+    return bs_internal_fail(BS_EINVAL, "bs_bptt_create: need 0 <= n_in <= 512");
   if (d->batch < 1 || d->batch > 65535 || d->n_steps < 1)
     return bs_internal_fail(BS_EINVAL, "bs_bptt_create: batch and n_steps must be >= 1");
   if (d->delay_steps < 1 || d->delay_in_steps < 1)
@@ -730,6 +747,7 @@ int bs_bptt_create(const bs_bptt_desc* d, bs_bptt** out) {
     BALLOC(h->ib3i, (size_t)T * BN * 3);
   }
   BALLOC(h->lpart_blocks, kLossBlocks);
+
   BALLOC(h->sbar, (size_t)h->W * BN);
   BALLOC(h->ay, BN);
   BALLOC(h->ae, BN);
