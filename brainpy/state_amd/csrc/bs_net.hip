@@ -94,8 +94,14 @@ struct DevChan {
   long long* pacc;               // FIXED fed by a plastic projection: deposits per post
 };
 
+// One block's tile: everything its prologue needs before the population's
+// blob (DevPop, channels, gather sources, staged tables) has reached LDS.
 struct Tile {
-  int pop, start, count, pad;
+  int pop, start, count, tile_in_pop;
+  int ncw, nfx, ngp, nch;
+  int nrec, blob_words, gp_begin, pad;
+  int pre_pop[4];   // pre populations of the first four gather sources
+  long long gid0, blob_off, tab_bytes;
 };
 
 // An incoming static projection of a post population. Its edges are grouped
@@ -112,6 +118,7 @@ struct GProj {
 
 struct WinArgs {
   const Tile* tiles;
+  const unsigned char* blob;  // per-population blobs (BlobLayout), 16-byte aligned
   const DevPop* pops;
   const DevChan* chans;
   const void* tables;
@@ -172,10 +179,26 @@ struct GInfo {
 };
 
 struct WinLayout {
-  size_t cnt, fxa, spl, spn, base, pref, gtot, gslo, ginfo, P, ch, inp, tab, end;
+  size_t cnt, fxa, spl, spn, base, pref, gtot, gslo, inp, blob, end;
 };
 
 __host__ __device__ inline size_t al8(size_t x) { return (x + 7) & ~(size_t)7; }
+__host__ __device__ inline size_t al16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+// A population's blob, copied to LDS by every block of the population in one
+// pass: its DevPop, channel records, gather-source records and staged tables.
+struct BlobLayout {
+  size_t pop, ch, gi, tab, end;
+};
+__host__ __device__ inline BlobLayout blob_layout(int nch, int ngp, size_t tab_bytes) {
+  BlobLayout B;
+  B.pop = 0;
+  B.ch = al16(sizeof(DevPop));
+  B.gi = al16(B.ch + (size_t)nch * sizeof(DevChan));
+  B.tab = al16(B.gi + (size_t)ngp * sizeof(GInfo));
+  B.end = al16(B.tab + tab_bytes);
+  return B;
+}
 
 // LDS layout of k_window for one population (tab_bytes: staged tables).
 // H: arrival offsets held in LDS counters (>= Wm).
@@ -190,12 +213,9 @@ __host__ __device__ inline WinLayout win_layout(int Wm, int H, int TS, int ncw, 
   L.pref = al8(L.base + (size_t)Wm * 16);
   L.gtot = al8(L.pref + (size_t)ngp * SP * 4);
   L.gslo = L.gtot + (size_t)(ngp + 1) * 4;
-  L.ginfo = al8(L.gslo + (size_t)ngp * 4);
-  L.P = al8(L.ginfo + (size_t)ngp * sizeof(GInfo));
-  L.ch = L.P + (size_t)BS_NPARAM * 8;
-  L.inp = al8(L.ch + (size_t)nch * sizeof(DevChan));
-  L.tab = al8(L.inp + (size_t)Wm * TS * nrec * tsz);
-  L.end = L.tab + tab_bytes;
+  L.inp = al16(L.gslo + (size_t)ngp * 4);
+  L.blob = al16(L.inp + (size_t)Wm * TS * nrec * tsz);
+  L.end = L.blob + blob_layout(nch, ngp, tab_bytes).end;
   return L;
 }
 
@@ -338,16 +358,16 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
     return x * q + min(x, rem) + k;
   }();
   const Tile tl = a.tiles[tix];
-  const DevPop& gpop = a.pops[tl.pop];
   const int tid = (int)threadIdx.x;
   const int TS = a.TS;
-  const int tile_in_pop = tix - gpop.tile0;
+  const int tile_in_pop = tl.tile_in_pop;
   const int W = a.W, Wm = a.Wmax;
-  const int ncw = gpop.n_cw, nfx = gpop.n_fx, ngp = gpop.gp_count, nch = gpop.ch_count;
+  const int ncw = tl.ncw, nfx = tl.nfx, ngp = tl.ngp, nch = tl.nch;
   const int SP = a.max_span + 1;
-  const int nrec = gpop.model == BS_AEIF_COND_ALPHA ? 2 : 1;
+  const int nrec = tl.nrec;
   const int H = a.H;
-  const WinLayout Lo = win_layout(Wm, H, TS, ncw, nfx, ngp, SP, nch, nrec, (int)sizeof(T), gpop.tab_bytes);
+  const WinLayout Lo = win_layout(Wm, H, TS, ncw, nfx, ngp, SP, nch, nrec, (int)sizeof(T), tl.tab_bytes);
+  const BlobLayout Bl = blob_layout(nch, ngp, tl.tab_bytes);
   uint32_t* cnt = (uint32_t*)(lds + Lo.cnt);
   long long* fxa = (long long*)(lds + Lo.fxa);
   unsigned short* spl = (unsigned short*)(lds + Lo.spl);
@@ -356,13 +376,21 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   int* pref = (int*)(lds + Lo.pref);
   int* gtot = (int*)(lds + Lo.gtot);
   int* gslo = (int*)(lds + Lo.gslo);
-  GInfo* gi = (GInfo*)(lds + Lo.ginfo);
-  double* P = (double*)(lds + Lo.P);
-  DevChan* chs = (DevChan*)(lds + Lo.ch);
-  unsigned char* tab = lds + Lo.tab;
+  unsigned char* blob = lds + Lo.blob;
+  const DevPop& gpop = *(const DevPop*)(blob + Bl.pop);
+  const double* P = gpop.P;
+  DevChan* chs = (DevChan*)(blob + Bl.ch);
+  const GInfo* gi = (const GInfo*)(blob + Bl.gi);
+  unsigned char* tab = blob + Bl.tab;
   T* inp = (T*)(lds + Lo.inp);  // [nrec][Wm][TS] summed inputs per step and lane
 
-  // ---- 0. stage
+  // ---- 0. stage: one level of loads after the tile record -- the blob, the
+  // ring slots of this window, the bucket counts the gather needs
+  {
+    const uint4* src = (const uint4*)(a.blob + tl.blob_off);
+    uint4* dst = (uint4*)blob;
+    for (int k = tid; k < tl.blob_words; k += 256) dst[k] = src[k];
+  }
   if (tix == 0) {  // clear the bucket counts of the next window's steps
     for (int t = tid; t < Wm * a.n_pops; t += 256) {
       const int st = a.n0 + W + t / a.n_pops;
@@ -370,7 +398,7 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
     }
   }
   // events already deposited for this window's steps: take them out of the ring
-  const long long tile_gid0 = (long long)gpop.gid0 + tl.start;
+  const long long tile_gid0 = tl.gid0 + tl.start;
   for (int k = tid; k < H * ncw * TS; k += 256) {
     const int w = k / (ncw * TS), r = k - w * ncw * TS, cw = r / TS, ln = r - cw * TS;
     uint32_t v = 0u;
@@ -392,52 +420,16 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
     fxa[k] = v;
   }
   for (int k = tid; k < Wm; k += 256) spn[k] = 0;
-  for (int k = tid; k < BS_NPARAM; k += 256) P[k] = gpop.P[k];
-  {
-    const int words = nch * (int)(sizeof(DevChan) / 8);
-    const unsigned long long* src = (const unsigned long long*)(a.chans + gpop.ch_begin);
-    for (int k = tid; k < words; k += 256) ((unsigned long long*)chs)[k] = src[k];
-  }
-  for (int c = 0; c < nch; ++c) {  // staged count tables and Poisson CDFs
-    const DevChan& ch = a.chans[gpop.ch_begin + c];
-    if (ch.lds_off < 0) continue;
-    if (ch.kind == BS_CH_COUNT) {
-      const T* src = (const T*)a.tables + ch.table_off;
-      T* dst = (T*)(tab + ch.lds_off);
-      for (int k = tid; k <= ch.max_count; k += 256) dst[k] = src[k];
-    } else if (ch.kind == BS_CH_POISSON) {
-      double* dc = (double*)(tab + ch.lds_off);
-      for (int k = tid; k < ch.cdf_len; k += 256) dc[k] = ch.cdf[k];
-      unsigned short* dg = (unsigned short*)(dc + ch.cdf_len);
-      for (int k = tid; k < 64; k += 256) dg[k] = ch.guide[k];
-    }
-  }
   // bucket sizes of every (projection, step) the gather reads: the spikes of
   // the previous window, each delivered once with all of its delays
   const int PL = (a.base_ptr && a.widx == 0) ? *a.prevlen_ptr : a.prevlen;
   for (int k = tid; k < ngp * SP; k += 256) {
     const int g = k / SP, t = k % SP;
-    const GProj& G = a.gproj[gpop.gp_begin + g];
+    const int pre_pop = g < 4 ? tl.pre_pop[g] : a.gproj[tl.gp_begin + g].pre_pop;
     const int s_lo = a.n0 - PL;
     const int s = s_lo + t;
-    pref[k] = (t > 0 && t <= PL) ? (int)a.sb_count[((s - 1) % a.Ls) * a.n_pops + G.pre_pop] : 0;
-    if (t == 0) {
-      const DevChan& ch = a.chans[G.chan];
-      GInfo I;
-      I.seg = G.seg;
-      I.edges = G.edges;
-      I.q = G.q;
-      I.pre_gid0 = a.pops[G.pre_pop].gid0;
-      I.nt1 = G.nt + 1;
-      I.kind = ch.kind;
-      I.cw = ch.cw;
-      I.half = ch.half;
-      I.fx = ch.fx;
-      I.s_lo = s_lo;
-      I.pad = 0;
-      gi[g] = I;
-      gslo[g] = s_lo;
-    }
+    pref[k] = (t > 0 && t <= PL) ? (int)a.sb_count[((s - 1) % a.Ls) * a.n_pops + pre_pop] : 0;
+    if (t == 0) gslo[g] = s_lo;
   }
   __syncthreads();
   if (tid < ngp) {  // per-projection inclusive prefix over steps
@@ -838,6 +830,10 @@ struct bs_net {
   // arrival ring (events landing after their delivery window), per tile slice
   uint32_t* d_ring_cnt = nullptr;
   long long* d_ring_fx = nullptr;
+  unsigned char* d_blob = nullptr;          // per-population blobs for k_window
+  std::vector<unsigned char> h_blob;
+  std::vector<std::vector<double>> h_cdf;   // Poisson tables per channel (host copies)
+  std::vector<std::vector<unsigned short>> h_guide;
   int Lr = 1, NCW = 1, NFX = 0;
   int H = 1;                       // arrival offsets held in LDS by k_window
   int* d_prevlen = nullptr;        // graph replays: length of the window before the chunk
@@ -879,7 +875,7 @@ static void free_net(bs_net* net) {
   F(net->d_state); F(net->d_ref); F(net->d_sb); F(net->d_sb_count); F(net->d_rec); F(net->d_rec_sorted); F(net->d_sort_tmp);
   F(net->d_rec_count); F(net->d_flags); F(net->d_vm_col); F(net->d_vm);
   for (void* p : net->owned) F(p);
-  F(net->d_base); F(net->d_ring_cnt); F(net->d_ring_fx); F(net->d_prevlen);
+  F(net->d_base); F(net->d_blob); F(net->d_ring_cnt); F(net->d_ring_fx); F(net->d_prevlen);
   F(net->d_stdp);
   F(net->d_spk_hist);
   if (net->graph) (void)hipGraphExecDestroy(net->graph);
@@ -1016,6 +1012,8 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   long long gid = 0;
   net->pops.resize(d->n_pops);
   net->chans.resize(d->n_channels);
+  net->h_cdf.assign(d->n_channels, {});
+  net->h_guide.assign(d->n_channels, {});
   for (int c = 0; c < d->n_channels; ++c) memset(&net->chans[c], 0, sizeof(DevChan));
   std::vector<int> chan_pop(d->n_channels, -1);
   for (int p = 0; p < d->n_pops; ++p) {
@@ -1069,6 +1067,8 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
         std::vector<unsigned short> guide;
         poisson_table(ch.lam, cdf, guide, dc.parts);
         dc.cdf_len = (int)cdf.size();
+        net->h_cdf[c] = cdf;
+        net->h_guide[c] = guide;
         double* dcdf = nullptr;
         unsigned short* dgd = nullptr;
         CHK(upload(net, &dcdf, cdf.data(), cdf.size()));
@@ -1355,17 +1355,86 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
         size_t sz = 0;
         if (dc.kind == BS_CH_COUNT) sz = al8((size_t)(dc.max_count + 1) * net->tsize);
         else if (dc.kind == BS_CH_POISSON) sz = al8((size_t)dc.cdf_len * 8 + 64 * 2);
-        if (sz && base_end + off + sz <= kLdsBudget) {
+        if (sz && base_end + off + sz + 16 <= kLdsBudget) {
           dc.lds_off = (long long)off;
           off += sz;
         }
       }
       p.tab_bytes = (long long)off;
-      need = std::max(need, base_end + off);
+      need = std::max(need, win_layout(W, net->H, TS, p.n_cw, p.n_fx, p.gp_count, SP, p.ch_count,
+                                       p.model == BS_AEIF_COND_ALPHA ? 2 : 1, (int)net->tsize,
+                                       (size_t)off).end);
     }
     net->lds_bytes = need;
   }
   net->Ls = dmax + 2 * W + 2;
+
+  // ---- per-population blobs (DevPop, channels, gather sources, staged
+  // tables in the LDS layout) and the complete tile records
+  {
+    std::vector<unsigned char>& B = net->h_blob;
+    std::vector<long long> boff(net->pops.size());
+    for (size_t p = 0; p < net->pops.size(); ++p) {
+      const DevPop& dp = net->pops[p];
+      const BlobLayout L = blob_layout(dp.ch_count, dp.gp_count, (size_t)dp.tab_bytes);
+      const size_t o = al16(B.size());
+      B.resize(o + L.end, 0);
+      boff[p] = (long long)o;
+      memcpy(&B[o + L.pop], &dp, sizeof(DevPop));
+      if (dp.ch_count)
+        memcpy(&B[o + L.ch], &net->chans[dp.ch_begin], sizeof(DevChan) * dp.ch_count);
+      for (int g = 0; g < dp.gp_count; ++g) {
+        const GProj& G = gp_sorted[dp.gp_begin + g];
+        const DevChan& ch = net->chans[G.chan];
+        GInfo I;
+        memset(&I, 0, sizeof(I));
+        I.seg = G.seg;
+        I.edges = G.edges;
+        I.q = G.q;
+        I.pre_gid0 = net->pops[G.pre_pop].gid0;
+        I.nt1 = G.nt + 1;
+        I.kind = ch.kind;
+        I.cw = ch.cw;
+        I.half = ch.half;
+        I.fx = ch.fx;
+        memcpy(&B[o + L.gi + g * sizeof(GInfo)], &I, sizeof(GInfo));
+      }
+      for (int c = dp.ch_begin; c < dp.ch_begin + dp.ch_count; ++c) {
+        const DevChan& ch = net->chans[c];
+        if (ch.lds_off < 0) continue;
+        unsigned char* dst = &B[o + L.tab + (size_t)ch.lds_off];
+        if (ch.kind == BS_CH_COUNT) {
+          for (int k = 0; k <= ch.max_count; ++k) {
+            const double v = d->tables[ch.table_off + k];
+            if (net->tsize == 8) memcpy(dst + 8 * k, &v, 8);
+            else {
+              const float f = (float)v;
+              memcpy(dst + 4 * k, &f, 4);
+            }
+          }
+        } else if (ch.kind == BS_CH_POISSON) {
+          memcpy(dst, net->h_cdf[c].data(), 8 * net->h_cdf[c].size());
+          memcpy(dst + 8 * net->h_cdf[c].size(), net->h_guide[c].data(), 2 * net->h_guide[c].size());
+        }
+      }
+    }
+    for (size_t t = 0; t < net->tiles.size(); ++t) {
+      Tile& T = net->tiles[t];
+      const DevPop& dp = net->pops[T.pop];
+      T.tile_in_pop = (int)t - dp.tile0;
+      T.ncw = dp.n_cw;
+      T.nfx = dp.n_fx;
+      T.ngp = dp.gp_count;
+      T.nch = dp.ch_count;
+      T.nrec = dp.model == BS_AEIF_COND_ALPHA ? 2 : 1;
+      T.blob_words = (int)(blob_layout(dp.ch_count, dp.gp_count, (size_t)dp.tab_bytes).end / 16);
+      T.gp_begin = dp.gp_begin;
+      for (int g = 0; g < 4; ++g) T.pre_pop[g] = g < dp.gp_count ? gp_sorted[dp.gp_begin + g].pre_pop : 0;
+      T.gid0 = dp.gid0;
+      T.blob_off = boff[T.pop];
+      T.tab_bytes = dp.tab_bytes;
+    }
+  }
 
   // ---- voltmeter columns
   std::vector<int> vm_col(gid, -1);
@@ -1407,6 +1476,8 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   CHK(dmalloc(&net->d_flags, 1));
   CHK(dmalloc(&net->d_base, 1));
   CHK(dmalloc(&net->d_prevlen, 1));
+  CHK(dmalloc(&net->d_blob, std::max<size_t>(16, net->h_blob.size())));
+  HCHK(hipMemcpy(net->d_blob, net->h_blob.data(), net->h_blob.size(), hipMemcpyHostToDevice));
   {  // arrival ring: slots for steps n0 .. n0 + d_max
     net->Lr = net->d_max + 1;
     int ncw = 1, nfx = 0;
@@ -1536,6 +1607,7 @@ static void launch_window(bs_net* net, int n0, int W, long long vm_step0, int pr
   a.widx = widx;
   a.spk_hist = net->d_spk_hist;
   a.tiles = net->d_tiles;
+  a.blob = net->d_blob;
   a.pops = net->d_pops;
   a.chans = net->d_chans;
   a.tables = net->d_tables;
