@@ -450,13 +450,15 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   }
   __syncthreads();
 
-  // ---- 1. gather: up to four spikes per thread per pass, loads in phases
+  // ---- 1. gather: two spikes per thread per pass, loads in phases; a
+  // segment's edge words come in as aligned 16-byte chunks, five per round
+  constexpr int kQ = 2, kCH = 5;
   const int total = gtot[ngp];
-  for (int b0 = 0; b0 < total; b0 += 1024) {
-    uint32_t si[4], e0[4], e1[4];
-    int ss[4], sg[4];
+  for (int b0 = 0; b0 < total; b0 += kQ * 256) {
+    uint32_t si[kQ], e0[kQ], e1[kQ];
+    int ss[kQ], sg[kQ];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < kQ; ++q) {
       const int idx = b0 + q * 256 + tid;
       sg[q] = -1;
       ss[q] = 0;
@@ -478,7 +480,7 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
       }
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < kQ; ++q) {
       e0[q] = e1[q] = 0;
       if (sg[q] >= 0) {
         const uint32_t* sp = gi[sg[q]].seg + (long long)si[q] * gi[sg[q]].nt1 + tile_in_pop;
@@ -486,57 +488,61 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
         e1[q] = sp[1];
       }
     }
-    // edges: rounds of eight per spike, 32 independent loads in flight
-    const uint32_t* ep[4];
-    const long long* qp[4];
-    int dn[4], kc[4], cwq[4], fxq[4];
-    uint32_t incq[4];
+    const uint4* ep[kQ];
+    const long long* qp[kQ];
+    int dn[kQ], kc[kQ], cwq[kQ], fxq[kQ];
+    uint32_t incq[kQ], a0[kQ], nchk[kQ];
     int rounds = 0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < kQ; ++q) {
       const int g = sg[q] < 0 ? 0 : sg[q];
-      ep[q] = gi[g].edges;
+      ep[q] = (const uint4*)gi[g].edges;
       qp[q] = gi[g].q;
       kc[q] = gi[g].kind == BS_CH_COUNT;
       cwq[q] = gi[g].cw;
       fxq[q] = gi[g].fx;
       incq[q] = 1u << (16 * gi[g].half);
       dn[q] = ss[q] + 1 - a.n0;
-      const int len = (int)(e1[q] - e0[q]);
-      rounds = max(rounds, (len + 7) >> 3);
+      a0[q] = e0[q] & ~3u;
+      nchk[q] = e1[q] > e0[q] ? (e1[q] - a0[q] + 3u) >> 2 : 0u;
+      rounds = max(rounds, (int)((nchk[q] + kCH - 1) / kCH));
     }
     for (int rd = 0; rd < rounds; ++rd) {
-      uint32_t wd[4][8];
+      uint4 v[kQ][kCH];
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < kQ; ++q)
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const uint32_t e = e0[q] + (uint32_t)(rd * 8 + u);
-          wd[q][u] = e < e1[q] ? ep[q][e] : 0u;
+        for (int c = 0; c < kCH; ++c) {
+          const uint32_t ck = (uint32_t)(rd * kCH + c);
+          v[q][c] = ck < nchk[q] ? ep[q][(a0[q] >> 2) + ck] : make_uint4(0u, 0u, 0u, 0u);
         }
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < kQ; ++q)
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const uint32_t e = e0[q] + (uint32_t)(rd * 8 + u);
-          if (e < e1[q]) {
-            const int w = dn[q] + (int)(wd[q][u] >> 8);
-            const int ln = (int)(wd[q][u] & 255u);
-            if (w < H) {  // an LDS counter (this window's steps, and later ones when they fit)
-              if (kc[q])
-                atomicAdd(&cnt[(w * ncw + cwq[q]) * TS + ln], incq[q]);
-              else
-                atomicAdd((unsigned long long*)&fxa[(w * nfx + fxq[q]) * TS + ln],
-                          (unsigned long long)qp[q][e]);
-            } else {  // beyond the LDS horizon: this tile's slot of the arrival ring (no other block touches it)
-              const size_t slot = (size_t)((a.n0 + w) % a.Lr);
-              if (kc[q])
-                __hip_atomic_fetch_add(&a.ring_cnt[(slot * a.NCW + cwq[q]) * a.Ntot + tile_gid0 + ln],
-                                       incq[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-              else
-                __hip_atomic_fetch_add(
-                    (unsigned long long*)&a.ring_fx[(slot * a.NFX + fxq[q]) * a.Ntot + tile_gid0 + ln],
-                    (unsigned long long)qp[q][e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        for (int c = 0; c < kCH; ++c) {
+          const uint32_t wv[4] = {v[q][c].x, v[q][c].y, v[q][c].z, v[q][c].w};
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {
+            const uint32_t e = a0[q] + (uint32_t)((rd * kCH + c) * 4 + m);
+            if (e >= e0[q] && e < e1[q]) {
+              const int w = dn[q] + (int)(wv[m] >> 8);
+              const int ln = (int)(wv[m] & 255u);
+              if (w < H) {  // an LDS counter (this window's steps, and later ones when they fit)
+                if (kc[q])
+                  atomicAdd(&cnt[(w * ncw + cwq[q]) * TS + ln], incq[q]);
+                else
+                  atomicAdd((unsigned long long*)&fxa[(w * nfx + fxq[q]) * TS + ln],
+                            (unsigned long long)qp[q][e]);
+              } else {  // beyond the LDS horizon: this tile's slot of the arrival ring
+                const size_t slot = (size_t)((a.n0 + w) % a.Lr);
+                if (kc[q])
+                  __hip_atomic_fetch_add(&a.ring_cnt[(slot * a.NCW + cwq[q]) * a.Ntot + tile_gid0 + ln],
+                                         incq[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                else
+                  __hip_atomic_fetch_add(
+                      (unsigned long long*)&a.ring_fx[(slot * a.NFX + fxq[q]) * a.Ntot + tile_gid0 + ln],
+                      (unsigned long long)qp[q][e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              }
             }
           }
         }
@@ -1284,6 +1290,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     uint32_t *dseg = nullptr, *dew = nullptr;
     long long* deq = nullptr;
     CHK(upload(net, &dseg, seg.data(), seg.size()));
+    ew.resize(ew.size() + 4, 0u);  // the gather reads whole 16-byte chunks past a segment's end
     CHK(upload(net, &dew, ew.data(), ew.size()));
     if (kind == BS_CH_FIXED) CHK(upload(net, &deq, eq.data(), eq.size()));
     G.seg = dseg;
