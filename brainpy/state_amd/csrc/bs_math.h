@@ -79,7 +79,12 @@ __device__ __forceinline__ uint32_t poisson_inv(uint32_t seed, uint32_t step, ui
 #define BS_INV_LN2 0x1.71547652b82fep+0
 
 // exp(x): k = rint(x/ln2), r = (x - k*LN2_HI) - k*LN2_LO, |r| <= 0.347;
-// exp(r) by the Taylor series to r^13 (Horner), then scaled by 2^k.
+// exp(r) by the Taylor series to r^13, then scaled by 2^k. The series is
+// evaluated as 1 + (r + r^2 (1/2 + r T(r))) with T(r) = sum_{n=3..13} r^(n-3)/n!
+// in Estrin form (pairs, then r^2, r^4, r^8): a dependency chain of about a
+// dozen operations instead of Horner's 26, since aeif_cond_alpha's RKF45
+// substeps wait on it. Every operation is one IEEE-rounded op (no contraction),
+// in exactly this order, so the CPU evaluation gives the same bits.
 __device__ __forceinline__ double exp_p(double x) {
   if (x != x) return x;
   if (x > 709.782712893384) return __builtin_inf();
@@ -87,25 +92,23 @@ __device__ __forceinline__ double exp_p(double x) {
   const double kd = __builtin_rint(x * BS_INV_LN2);
   const int k = (int)kd;
   const double r = (x - kd * BS_LN2_HI) - kd * BS_LN2_LO;
-  double p = 1.0 / 6227020800.0;          // 1/13!
-  p = p * r + 1.0 / 479001600.0;          // 1/12!
-  p = p * r + 1.0 / 39916800.0;           // 1/11!
-  p = p * r + 1.0 / 3628800.0;            // 1/10!
-  p = p * r + 1.0 / 362880.0;             // 1/9!
-  p = p * r + 1.0 / 40320.0;              // 1/8!
-  p = p * r + 1.0 / 5040.0;               // 1/7!
-  p = p * r + 1.0 / 720.0;                // 1/6!
-  p = p * r + 1.0 / 120.0;                // 1/5!
-  p = p * r + 1.0 / 24.0;                 // 1/4!
-  p = p * r + 1.0 / 6.0;                  // 1/3!
-  p = p * r + 0.5;                        // 1/2!
-  p = p * r + 1.0;
-  p = p * r + 1.0;
+  const double r2 = r * r, r4 = r2 * r2, r8 = r4 * r4;
+  const double a0 = 1.0 / 6.0 + 1.0 / 24.0 * r;                // 1/3!, 1/4!
+  const double a1 = 1.0 / 120.0 + 1.0 / 720.0 * r;             // 1/5!, 1/6!
+  const double a2 = 1.0 / 5040.0 + 1.0 / 40320.0 * r;          // 1/7!, 1/8!
+  const double a3 = 1.0 / 362880.0 + 1.0 / 3628800.0 * r;      // 1/9!, 1/10!
+  const double a4 = 1.0 / 39916800.0 + 1.0 / 479001600.0 * r;  // 1/11!, 1/12!
+  const double b0 = a0 + a1 * r2;
+  const double b1 = a2 + a3 * r2;
+  const double b2 = a4 + 1.0 / 6227020800.0 * r2;             // 1/13!
+  const double t = (b0 + b1 * r4) + b2 * r8;
+  const double p = 1.0 + (r + r2 * (0.5 + r * t));
   return __builtin_ldexp(p, k);
 }
 
 // log(x) for x > 0: x = m 2^e, m in [sqrt(1/2), sqrt(2)); s = (m-1)/(m+1);
-// log(m) = 2 atanh(s) = 2 s (1 + z/3 + z^2/5 + ... + z^11/23), z = s^2.
+// log(m) = 2 atanh(s) = 2 s (1 + z/3 + z^2/5 + ... + z^11/23), z = s^2,
+// the series in Estrin form like exp_p's.
 __device__ __forceinline__ double log_p(double x) {
   if (x != x || x < 0.0) return __builtin_nan("");
   if (x == 0.0) return -__builtin_inf();
@@ -118,18 +121,15 @@ __device__ __forceinline__ double log_p(double x) {
   }
   const double s = (m - 1.0) / (m + 1.0);
   const double z = s * s;
-  double q = 1.0 / 23.0;
-  q = q * z + 1.0 / 21.0;
-  q = q * z + 1.0 / 19.0;
-  q = q * z + 1.0 / 17.0;
-  q = q * z + 1.0 / 15.0;
-  q = q * z + 1.0 / 13.0;
-  q = q * z + 1.0 / 11.0;
-  q = q * z + 1.0 / 9.0;
-  q = q * z + 1.0 / 7.0;
-  q = q * z + 1.0 / 5.0;
-  q = q * z + 1.0 / 3.0;
-  q = q * z + 1.0;
+  // q = 1 + z/3 + z^2 Q(z), Q(z) = sum_{j=0..9} z^j/(2j+5), Q in Estrin form
+  const double z2 = z * z, z4 = z2 * z2, z8 = z4 * z4;
+  const double a0 = 1.0 / 5.0 + z * (1.0 / 7.0);
+  const double a1 = 1.0 / 9.0 + z * (1.0 / 11.0);
+  const double a2 = 1.0 / 13.0 + z * (1.0 / 15.0);
+  const double a3 = 1.0 / 17.0 + z * (1.0 / 19.0);
+  const double a4 = 1.0 / 21.0 + z * (1.0 / 23.0);
+  const double qq = ((a0 + a1 * z2) + (a2 + a3 * z2) * z4) + a4 * z8;
+  const double q = 1.0 + z * (1.0 / 3.0 + z * qq);
   const double lm = 2.0 * s * q;
   const double ed = (double)e;
   return ed * BS_LN2_HI + (ed * BS_LN2_LO + lm);

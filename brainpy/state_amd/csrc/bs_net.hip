@@ -19,6 +19,10 @@
 //     Poisson drive drawn from a counter-based Threefry stream); update (the
 //     model step for W steps with the state in registers); emit (per-step spike
 //     buckets for the following windows, the spike record, voltmeter taps).
+//   * When the tiles leave CUs idle, the launch carries extra producer blocks
+//     that draw the next window's Poisson counts (the first Poisson channel
+//     of each population) into a two-window ring; the tiles then read counts
+//     in the stage phase instead of drawing them in the inputs phase.
 //   * Full chunks of windows are captured once into a HIP graph and replayed;
 //     the kernels read the chunk's first step from device memory.
 //   * Plastic (stdp_synapse) projections force W = 1 and add two kernels per
@@ -45,6 +49,15 @@
 #include "bs_models.h"
 
 using namespace bs;
+
+// Gather batch: spikes per thread per pass, and 16-byte edge chunks per spike
+// and round (the loads one thread has in flight at once).
+#ifndef BS_GATHER_SPIKES
+#define BS_GATHER_SPIKES 2
+#endif
+#ifndef BS_GATHER_CHUNKS
+#define BS_GATHER_CHUNKS 5
+#endif
 
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_err;
@@ -85,7 +98,8 @@ struct DevChan {
   long long lds_off;   // offset of the staged table in the LDS table area, -1 if not staged
   double weight;
   int parts, delay;    // POISSON: lam split into parts; delay in steps
-  int cdf_len, pad;
+  int cdf_len;
+  int pre;             // POISSON: counts drawn ahead by producer blocks (pcnt)
   long long tmin, tmax;
   unsigned seed, pad2;
   const int* gen_index;
@@ -99,7 +113,7 @@ struct DevChan {
 struct Tile {
   int pop, start, count, tile_in_pop;
   int ncw, nfx, ngp, nch;
-  int nrec, blob_words, gp_begin, pad;
+  int nrec, blob_words, gp_begin, pois;  // pois: the population has a drawn-ahead channel
   int pre_pop[4];   // pre populations of the first four gather sources
   long long gid0, blob_off, tab_bytes;
 };
@@ -116,8 +130,20 @@ struct GProj {
   const long long* q;  // FIXED channels: 2^F fixed-point weights per edge
 };
 
+// Poisson channel whose counts producer blocks draw one window ahead: the
+// population's neurons gid0 .. gid0 + n - 1, channel chans[ch].
+struct PJob {
+  long long gid0;
+  int n, ch;
+};
+
 struct WinArgs {
   const Tile* tiles;
+  int n_tiles;               // blocks past n_tiles are Poisson producers
+  unsigned short* pcnt;      // [2 * Wmax][Ntot] drawn counts, slot = step % (2 * Wmax)
+  const PJob* pjobs;
+  int n_pjobs;
+  long long pjob_n;          // neurons over all jobs (elements per step)
   const unsigned char* blob;  // per-population blobs (BlobLayout), 16-byte aligned
   const DevPop* pops;
   const DevChan* chans;
@@ -179,7 +205,7 @@ struct GInfo {
 };
 
 struct WinLayout {
-  size_t cnt, fxa, spl, spn, base, pref, gtot, gslo, inp, blob, end;
+  size_t cnt, fxa, spl, spn, base, pref, gtot, gslo, inp, pc, blob, end;
 };
 
 __host__ __device__ inline size_t al8(size_t x) { return (x + 7) & ~(size_t)7; }
@@ -203,7 +229,7 @@ __host__ __device__ inline BlobLayout blob_layout(int nch, int ngp, size_t tab_b
 // LDS layout of k_window for one population (tab_bytes: staged tables).
 // H: arrival offsets held in LDS counters (>= Wm).
 __host__ __device__ inline WinLayout win_layout(int Wm, int H, int TS, int ncw, int nfx, int ngp, int SP,
-                                                int nch, int nrec, int tsz, size_t tab_bytes) {
+                                                int nch, int nrec, int tsz, size_t tab_bytes, int npc) {
   WinLayout L;
   L.cnt = 0;
   L.fxa = al8(L.cnt + (size_t)H * ncw * TS * 4);
@@ -214,7 +240,8 @@ __host__ __device__ inline WinLayout win_layout(int Wm, int H, int TS, int ncw, 
   L.gtot = al8(L.pref + (size_t)ngp * SP * 4);
   L.gslo = L.gtot + (size_t)(ngp + 1) * 4;
   L.inp = al16(L.gslo + (size_t)ngp * 4);
-  L.blob = al16(L.inp + (size_t)Wm * TS * nrec * tsz);
+  L.pc = al16(L.inp + (size_t)Wm * TS * nrec * tsz);
+  L.blob = al16(L.pc + (size_t)Wm * TS * 2 * npc);
   L.end = L.blob + blob_layout(nch, ngp, tab_bytes).end;
   return L;
 }
@@ -222,7 +249,7 @@ __host__ __device__ inline WinLayout win_layout(int Wm, int H, int TS, int ncw, 
 // Channel descriptor held in registers by the update loop.
 template <typename T>
 struct ChReg {
-  int kind, receptor, cw, half, max_count, fx, shift, delay, parts, cdf_len;
+  int kind, receptor, cw, half, max_count, fx, shift, delay, parts, cdf_len, pre;
   unsigned seed;
   long long tmin, tmax;
   T weight;
@@ -247,6 +274,7 @@ __device__ __forceinline__ ChReg<T> load_ch(const DevChan& ch, const unsigned ch
   c.delay = ch.delay;
   c.parts = ch.parts;
   c.cdf_len = ch.cdf_len;
+  c.pre = ch.pre;
   c.seed = ch.seed;
   c.tmin = ch.tmin;
   c.tmax = ch.tmax;
@@ -276,6 +304,7 @@ struct UpdCtx {
   long long gid;
   const T* inp;
   int nrec, Wm;
+  const unsigned short* pc;  // [Wm][TS] drawn-ahead Poisson counts of this tile
 };
 
 // Input of one channel at window step w for lane tid (neuron j).
@@ -291,6 +320,7 @@ __device__ __forceinline__ T chan_value(const UpdCtx<T>& u, const ChReg<T>& c, i
     return c.tab[k];
   }
   if (c.kind == BS_CH_POISSON) {
+    if (c.pre) return (T)u.pc[w * u.TS + lane] * c.weight;
     const int j = u.start + lane;
     const int e = c.gen_index ? c.gen_index[j] : j;
     const long long sgs = (long long)step - 1 - c.delay;
@@ -340,6 +370,41 @@ __device__ __forceinline__ void update_loop(const UpdCtx<T>& u, const ChReg<T>* 
   a.ref[u.gid] = r;
 }
 
+// Producer blocks: draw the Poisson counts of steps n1 .. n1 + Wmax - 1 for
+// every drawn-ahead channel (the next window's inputs) while the tile blocks
+// run the current window on the other CUs. The counts are functions of
+// (seed, step, element) only, so they equal what the inputs phase would draw.
+__device__ void produce_poisson(const WinArgs& a, int n1, int blk, int nblk) {
+  const long long per = a.pjob_n;
+  const long long total = per * a.Wmax;
+  const int L2 = 2 * a.Wmax;
+  for (long long idx = (long long)blk * 256 + threadIdx.x; idx < total; idx += (long long)nblk * 256) {
+    const int w = (int)(idx / per);
+    long long r = idx - (long long)w * per;
+    int j = 0;
+    while (j + 1 < a.n_pjobs && r >= a.pjobs[j].n) {
+      r -= a.pjobs[j].n;
+      ++j;
+    }
+    const PJob J = a.pjobs[j];
+    const DevChan& c = a.chans[J.ch];
+    const int step = n1 + w;
+    const int i = (int)r;
+    const int e = c.gen_index ? c.gen_index[i] : i;
+    const long long sgs = (long long)step - 1 - c.delay;
+    uint32_t cn = 0;
+    if (e >= 0 && sgs >= 0 && sgs > c.tmin && sgs <= c.tmax)
+      cn = poisson_inv(c.seed, (uint32_t)sgs, (uint32_t)e, c.cdf, c.guide, c.cdf_len, c.parts);
+    a.pcnt[(long long)(step % L2) * a.Ntot + J.gid0 + i] = (unsigned short)cn;
+  }
+}
+
+// The counts of a run's first window (later windows get theirs from the
+// producer blocks of the launch before).
+__global__ __launch_bounds__(256) void k_pois_fill(WinArgs a) {
+  produce_poisson(a, a.n0, (int)blockIdx.x, (int)gridDim.x);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
@@ -348,12 +413,16 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
     a.n0 = (int)b + a.widx * a.Wmax;
     a.vm_step0 = b;
   }
+  if ((int)blockIdx.x >= a.n_tiles) {  // spare CUs: the next window's Poisson counts
+    produce_poisson(a, a.n0 + a.W, (int)blockIdx.x - a.n_tiles, (int)gridDim.x - a.n_tiles);
+    return;
+  }
   // XCD-aware tile order: the dispatcher deals blocks round-robin over the
   // eight XCDs (block b runs on XCD b % 8), so each XCD is given one
   // contiguous run of tiles. A spike's edge row is laid out tile-major, and
   // every XCD then fetches only its own slice of the row into its L2.
-  const int tix = [] {
-    const int G = (int)gridDim.x, x = (int)(blockIdx.x & 7u), k = (int)(blockIdx.x >> 3);
+  const int tix = [&a] {
+    const int G = a.n_tiles, x = (int)(blockIdx.x & 7u), k = (int)(blockIdx.x >> 3);
     const int q = G >> 3, rem = G & 7;
     return x * q + min(x, rem) + k;
   }();
@@ -366,7 +435,8 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   const int SP = a.max_span + 1;
   const int nrec = tl.nrec;
   const int H = a.H;
-  const WinLayout Lo = win_layout(Wm, H, TS, ncw, nfx, ngp, SP, nch, nrec, (int)sizeof(T), tl.tab_bytes);
+  const WinLayout Lo = win_layout(Wm, H, TS, ncw, nfx, ngp, SP, nch, nrec, (int)sizeof(T), tl.tab_bytes,
+                                  tl.pois);
   const BlobLayout Bl = blob_layout(nch, ngp, tl.tab_bytes);
   uint32_t* cnt = (uint32_t*)(lds + Lo.cnt);
   long long* fxa = (long long*)(lds + Lo.fxa);
@@ -383,6 +453,7 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   const GInfo* gi = (const GInfo*)(blob + Bl.gi);
   unsigned char* tab = blob + Bl.tab;
   T* inp = (T*)(lds + Lo.inp);  // [nrec][Wm][TS] summed inputs per step and lane
+  unsigned short* pc = (unsigned short*)(lds + Lo.pc);
 
   // ---- 0. stage: one level of loads after the tile record -- the blob, the
   // ring slots of this window, the bucket counts the gather needs
@@ -399,6 +470,13 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   }
   // events already deposited for this window's steps: take them out of the ring
   const long long tile_gid0 = tl.gid0 + tl.start;
+  if (tl.pois) {  // this window's drawn-ahead Poisson counts (from the launch before)
+    const int L2 = 2 * Wm;
+    for (int k = tid; k < W * TS; k += 256) {
+      const int w = k / TS, l = k - w * TS;
+      pc[k] = l < tl.count ? a.pcnt[(long long)((a.n0 + w) % L2) * a.Ntot + tile_gid0 + l] : (unsigned short)0;
+    }
+  }
   for (int k = tid; k < H * ncw * TS; k += 256) {
     const int w = k / (ncw * TS), r = k - w * ncw * TS, cw = r / TS, ln = r - cw * TS;
     uint32_t v = 0u;
@@ -425,7 +503,12 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   const int PL = (a.base_ptr && a.widx == 0) ? *a.prevlen_ptr : a.prevlen;
   for (int k = tid; k < ngp * SP; k += 256) {
     const int g = k / SP, t = k % SP;
-    const int pre_pop = g < 4 ? tl.pre_pop[g] : a.gproj[tl.gp_begin + g].pre_pop;
+    // constant indices only: a runtime index into tl would move it to scratch
+    const int pre_pop = g == 0   ? tl.pre_pop[0]
+                        : g == 1 ? tl.pre_pop[1]
+                        : g == 2 ? tl.pre_pop[2]
+                        : g == 3 ? tl.pre_pop[3]
+                                 : a.gproj[tl.gp_begin + g].pre_pop;
     const int s_lo = a.n0 - PL;
     const int s = s_lo + t;
     pref[k] = (t > 0 && t <= PL) ? (int)a.sb_count[((s - 1) % a.Ls) * a.n_pops + pre_pop] : 0;
@@ -452,7 +535,7 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
 
   // ---- 1. gather: two spikes per thread per pass, loads in phases; a
   // segment's edge words come in as aligned 16-byte chunks, five per round
-  constexpr int kQ = 2, kCH = 5;
+  constexpr int kQ = BS_GATHER_SPIKES, kCH = BS_GATHER_CHUNKS;
   const int total = gtot[ngp];
   for (int b0 = 0; b0 < total; b0 += kQ * 256) {
     uint32_t si[kQ], e0[kQ], e1[kQ];
@@ -555,7 +638,7 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
     const uint32_t v = cnt[k];
     if (v != 0u && ln < tl.count) {
       const size_t o = ((size_t)((a.n0 + w) % a.Lr) * a.NCW + cw) * a.Ntot + tile_gid0 + ln;
-      a.ring_cnt[o] += v;
+      atomicAdd(&a.ring_cnt[o], v);  // result unused: nothing waits for it
     }
   }
   for (int k = W * nfx * TS + tid; k < H * nfx * TS; k += 256) {
@@ -563,7 +646,7 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
     const long long v = fxa[k];
     if (v != 0 && ln < tl.count) {
       const size_t o = ((size_t)((a.n0 + w) % a.Lr) * a.NFX + fx) * a.Ntot + tile_gid0 + ln;
-      a.ring_fx[o] += v;
+      atomicAdd((unsigned long long*)&a.ring_fx[o], (unsigned long long)v);
     }
   }
 
@@ -575,7 +658,7 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
     for (int c = 0; c < 4; ++c)
       if (c < nch) cr[c] = load_ch<T>(chs[c], tab, tables);
     UpdCtx<T> u{a, cnt, fxa, spl, spn, chs, tab, tables, TS, ncw, nfx, nch, tl.start, tid,
-                tid < tl.count, (long long)gpop.gid0 + tl.start + tid, inp, nrec, Wm};
+                tid < tl.count, (long long)gpop.gid0 + tl.start + tid, inp, nrec, Wm, pc};
     for (int idx = tid; idx < W * TS; idx += 256) {
       const int w = idx / TS, l = idx - w * TS;
       if (l >= tl.count) continue;
@@ -604,7 +687,7 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
     const T* tables = (const T*)a.tables;
     ChReg<T> cr[4];
     UpdCtx<T> u{a, cnt, fxa, spl, spn, chs, tab, tables, TS, ncw, nfx, nch, tl.start, tid,
-                tid < tl.count, (long long)gpop.gid0 + tl.start + tid, inp, nrec, Wm};
+                tid < tl.count, (long long)gpop.gid0 + tl.start + tid, inp, nrec, Wm, pc};
     switch (gpop.model) {
       case BS_IAF_PSC_DELTA: {
         const DeltaP<T> mp = load_delta<T>(P);
@@ -846,6 +929,13 @@ struct bs_net {
   int h_prevlen = 0;
   int last_win = 0;                // length of the last window run (0 after reset)
   hipGraphExec_t graph = nullptr;  // one full chunk of window launches
+  // Poisson producer blocks (when the tiles leave CUs idle)
+  std::vector<int> pop_pc;         // population has a drawn-ahead Poisson channel
+  std::vector<PJob> pjobs;
+  PJob* d_pjobs = nullptr;
+  unsigned short* d_pcnt = nullptr;
+  long long pjob_n = 0;
+  int n_prod = 0;
   std::vector<void*> owned;  // per-projection gather arrays, generator maps
   // host copies
   std::vector<double> state0;  // [BS_NSTATE][Ntot]
@@ -884,6 +974,8 @@ static void free_net(bs_net* net) {
   F(net->d_base); F(net->d_blob); F(net->d_ring_cnt); F(net->d_ring_fx); F(net->d_prevlen);
   F(net->d_stdp);
   F(net->d_spk_hist);
+  F(net->d_pjobs);
+  F(net->d_pcnt);
   if (net->graph) (void)hipGraphExecDestroy(net->graph);
   if (net->stream) (void)hipStreamDestroy(net->stream);
   delete net;
@@ -1320,6 +1412,32 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   net->d_min = dmin;
   net->d_max = dmax;
 
+  // ---- Poisson producers: when the tiles leave at least 8 CUs idle, blocks
+  // on those CUs draw each population's first Poisson channel one window
+  // ahead (counts up to 65535) and the tiles read the counts instead
+  net->pop_pc.assign(net->pops.size(), 0);
+  {
+    int n_cu = 0;
+    HCHK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, net->device));
+    const long long spare = (long long)n_cu - (long long)net->tiles.size();
+    if (spare >= 8) {
+      for (size_t p = 0; p < net->pops.size(); ++p) {
+        const DevPop& dp = net->pops[p];
+        for (int c = dp.ch_begin; c < dp.ch_begin + dp.ch_count; ++c) {
+          DevChan& dc = net->chans[c];
+          if (dc.kind != BS_CH_POISSON) continue;
+          if ((long long)(dc.cdf_len - 1) * dc.parts > 65535) continue;
+          dc.pre = 1;
+          net->pop_pc[p] = 1;
+          net->pjobs.push_back(PJob{dp.gid0, dp.n, c});
+          net->pjob_n += dp.n;
+          break;
+        }
+      }
+      if (!net->pjobs.empty()) net->n_prod = (int)std::min<long long>(spare, 256);
+    }
+  }
+
   // ---- window length: W = d_min + 1 (holder lag) capped by the LDS budget
   int W = dmin + 1;
   if (!net->stdp.empty()) W = 1;  // plastic updates interleave with every step
@@ -1337,9 +1455,12 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   auto lds_of = [&](int w) {
     size_t best = 0;
     const int SP = span_of(w) + 1;
-    for (const DevPop& p : net->pops)
+    for (size_t q = 0; q < net->pops.size(); ++q) {
+      const DevPop& p = net->pops[q];
       best = std::max(best, win_layout(w, hz(w), TS, p.n_cw, p.n_fx, p.gp_count, SP, p.ch_count,
-                                       p.model == BS_AEIF_COND_ALPHA ? 2 : 1, (int)net->tsize, 0).end);
+                                       p.model == BS_AEIF_COND_ALPHA ? 2 : 1, (int)net->tsize, 0,
+                                       net->pop_pc[q]).end);
+    }
     return best;
   };
   if (lds_of(W) > kLdsBudget) full_h = false;
@@ -1352,9 +1473,11 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   {
     const int SP = net->max_span + 1;
     size_t need = 0;
-    for (DevPop& p : net->pops) {
+    for (size_t q = 0; q < net->pops.size(); ++q) {
+      DevPop& p = net->pops[q];
       const size_t base_end = win_layout(W, net->H, TS, p.n_cw, p.n_fx, p.gp_count, SP, p.ch_count,
-                                         p.model == BS_AEIF_COND_ALPHA ? 2 : 1, (int)net->tsize, 0).end;
+                                         p.model == BS_AEIF_COND_ALPHA ? 2 : 1, (int)net->tsize, 0,
+                                         net->pop_pc[q]).end;
       size_t off = 0;
       for (int c = p.ch_begin; c < p.ch_begin + p.ch_count; ++c) {
         DevChan& dc = net->chans[c];
@@ -1370,7 +1493,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
       p.tab_bytes = (long long)off;
       need = std::max(need, win_layout(W, net->H, TS, p.n_cw, p.n_fx, p.gp_count, SP, p.ch_count,
                                        p.model == BS_AEIF_COND_ALPHA ? 2 : 1, (int)net->tsize,
-                                       (size_t)off).end);
+                                       (size_t)off, net->pop_pc[q]).end);
     }
     net->lds_bytes = need;
   }
@@ -1436,6 +1559,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
       T.nrec = dp.model == BS_AEIF_COND_ALPHA ? 2 : 1;
       T.blob_words = (int)(blob_layout(dp.ch_count, dp.gp_count, (size_t)dp.tab_bytes).end / 16);
       T.gp_begin = dp.gp_begin;
+      T.pois = net->pop_pc[T.pop];
       for (int g = 0; g < 4; ++g) T.pre_pop[g] = g < dp.gp_count ? gp_sorted[dp.gp_begin + g].pre_pop : 0;
       T.gid0 = dp.gid0;
       T.blob_off = boff[T.pop];
@@ -1504,6 +1628,14 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
                    hipMemcpyHostToDevice));
   }
   CHK(dmalloc(&net->d_vm_col, gid));
+  if (net->n_prod > 0) {
+    CHK(dmalloc(&net->d_pcnt, (size_t)2 * W * std::max<long long>(1, gid)));
+    CHK(dmalloc(&net->d_pjobs, net->pjobs.size()));
+    HCHK(hipMemcpy(net->d_pjobs, net->pjobs.data(), sizeof(PJob) * net->pjobs.size(), hipMemcpyHostToDevice));
+    // about four draws per producer thread and window
+    const long long want = (W * net->pjob_n + 1023) / 1024;
+    net->n_prod = (int)std::max<long long>(1, std::min<long long>(net->n_prod, want));
+  }
   {  // spike record list: worst case every neuron fires every step of K windows
     const long long per_win = std::max<long long>(1, gid) * W;
     long long k = (256ll << 20) / per_win;
@@ -1640,8 +1772,28 @@ static void launch_window(bs_net* net, int n0, int W, long long vm_step0, int pr
   a.n_vm = net->n_vm_dev;
   a.vm_buf = net->d_vm;
   a.vm_step0 = vm_step0;
-  hipLaunchKernelGGL(k_window<T>, dim3((unsigned)net->tiles.size()), dim3(256), net->lds_bytes,
-                     net->stream, a);
+  a.n_tiles = (int)net->tiles.size();
+  a.pcnt = net->d_pcnt;
+  a.pjobs = net->d_pjobs;
+  a.n_pjobs = (int)net->pjobs.size();
+  a.pjob_n = net->pjob_n;
+  hipLaunchKernelGGL(k_window<T>, dim3((unsigned)(net->tiles.size() + net->n_prod)), dim3(256),
+                     net->lds_bytes, net->stream, a);
+}
+
+// Poisson counts of the window starting at n0 (before a run's first window).
+static void launch_pois_fill(bs_net* net, int n0) {
+  WinArgs a;
+  memset(&a, 0, sizeof(a));
+  a.chans = net->d_chans;
+  a.Ntot = net->Ntot;
+  a.n0 = n0;
+  a.Wmax = net->W;
+  a.pcnt = net->d_pcnt;
+  a.pjobs = net->d_pjobs;
+  a.n_pjobs = (int)net->pjobs.size();
+  a.pjob_n = net->pjob_n;
+  hipLaunchKernelGGL(k_pois_fill, dim3((unsigned)net->n_prod), dim3(256), 0, net->stream, a);
 }
 
 static void launch_stdp(bs_net* net, int n, const long long* base_ptr, int widx) {
@@ -1690,6 +1842,10 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
   HIPCHK(hipEventCreate(&ev_all1));
   std::vector<hipEvent_t> evs;
   HIPCHK(hipEventRecord(ev_all0, net->stream));
+  if (net->n_prod > 0 && n_steps > 0) {
+    launch_pois_fill(net, (int)net->step);
+    HIPCHK(hipGetLastError());
+  }
   long long done = 0;
   int rc = BS_OK;
   // Algorithmic update bytes per network step: every neuron reads and writes

@@ -102,7 +102,8 @@ extern "C" uint32_t orc_poisson(uint32_t seed, uint32_t step, uint32_t elem, dou
 
 // exp and log from first principles: Cody-Waite reduction by ln 2 (split
 // into a 42-bit head and a tail), Taylor series of exp to degree 13, and
-// log(m) = 2 artanh((m-1)/(m+1)) to degree 23. Both are also what the device
+// log(m) = 2 artanh((m-1)/(m+1)) to degree 23, both series summed in the
+// same Estrin grouping, op for op. Both are also what the device
 // evaluates (aeif_cond_alpha's exponential current and the RKF45 step-size
 // controller's fractional powers).
 static const double kLn2Hi = 0x1.62e42fefa3800p-1;
@@ -115,23 +116,19 @@ extern "C" double orc_exp(double x) {
   if (x < -745.1332191019412) return 0.0;
   const double k = rint(x * kInvLn2);
   const double r = (x - k * kLn2Hi) - k * kLn2Lo;
-  // Taylor coefficients 1/n!, n = 13 .. 0, Horner.
-  static const double inv_fact[14] = {1.0,
-                                      1.0,
-                                      0.5,
-                                      1.0 / 6.0,
-                                      1.0 / 24.0,
-                                      1.0 / 120.0,
-                                      1.0 / 720.0,
-                                      1.0 / 5040.0,
-                                      1.0 / 40320.0,
-                                      1.0 / 362880.0,
-                                      1.0 / 3628800.0,
-                                      1.0 / 39916800.0,
-                                      1.0 / 479001600.0,
-                                      1.0 / 6227020800.0};
-  double p = inv_fact[13];
-  for (int n = 12; n >= 0; --n) p = p * r + inv_fact[n];
+  // Taylor series to r^13 as 1 + (r + r^2 (1/2 + r T)), T = sum_{n=3..13}
+  // r^(n-3)/n! grouped in pairs, then by r^2, r^4, r^8 (Estrin's scheme).
+  static const double f[14] = {1.0,           1.0,           0.5,
+                               1.0 / 6.0,     1.0 / 24.0,    1.0 / 120.0,
+                               1.0 / 720.0,   1.0 / 5040.0,  1.0 / 40320.0,
+                               1.0 / 362880.0, 1.0 / 3628800.0, 1.0 / 39916800.0,
+                               1.0 / 479001600.0, 1.0 / 6227020800.0};
+  const double r2 = r * r, r4 = r2 * r2, r8 = r4 * r4;
+  double a[5];
+  for (int j = 0; j < 5; ++j) a[j] = f[3 + 2 * j] + f[4 + 2 * j] * r;
+  const double b0 = a[0] + a[1] * r2, b1 = a[2] + a[3] * r2, b2 = a[4] + f[13] * r2;
+  const double t = (b0 + b1 * r4) + b2 * r8;
+  const double p = 1.0 + (r + r2 * (0.5 + r * t));
   return ldexp(p, (int)k);
 }
 
@@ -147,8 +144,12 @@ extern "C" double orc_log(double x) {
   }
   const double s = (m - 1.0) / (m + 1.0);
   const double z = s * s;
-  double q = 1.0 / 23.0;
-  for (int k = 21; k >= 1; k -= 2) q = q * z + 1.0 / (double)k;
+  // 1 + z/3 + z^2 Q(z), Q = sum_{j=0..9} z^j/(2j+5) grouped like exp's
+  const double z2 = z * z, z4 = z2 * z2, z8 = z4 * z4;
+  double a[5];
+  for (int j = 0; j < 5; ++j) a[j] = 1.0 / (double)(5 + 4 * j) + z * (1.0 / (double)(7 + 4 * j));
+  const double qq = ((a[0] + a[1] * z2) + (a[2] + a[3] * z2) * z4) + a[4] * z8;
+  const double q = 1.0 + z * (1.0 / 3.0 + z * qq);
   const double lm = 2.0 * s * q;
   const double ed = (double)e;
   return ed * kLn2Hi + (ed * kLn2Lo + lm);
