@@ -30,5 +30,38 @@ def build(verbose: bool = False) -> str:
     return OUT
 
 
+def resource_usage(src: str) -> dict:
+    """Per-kernel register and private-memory use of one source file for
+    gfx950, from the compiler's resource-usage remarks (device-only compile,
+    nothing written): ``{mangled name: {'VGPRs': n, 'ScratchSize': bytes per
+    lane, 'Occupancy': waves per SIMD, ...}}``. Kernels of library headers
+    (rocprim) are left out."""
+    import re
+    import tempfile
+    hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
+    with tempfile.TemporaryDirectory() as td:
+        cmd = [hipcc] + [f for f in FLAGS if f != '-shared'] + [
+            '--cuda-device-only', '-c', src, '-o', os.path.join(td, 'k.o'),
+            '-Rpass-analysis=kernel-resource-usage']
+        res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError('hipcc failed:\n' + res.stderr[-4000:])
+    out, cur = {}, None
+    base = os.path.basename(src)
+    for line in res.stderr.splitlines():
+        if 'remark:' not in line:
+            continue
+        m = re.search(r'remark: Function Name: (\S+)', line)
+        if m:
+            cur = m.group(1) if os.path.basename(line.split(':', 1)[0]) == base else None
+            if cur:
+                out[cur] = {}
+            continue
+        m = re.search(r'remark:\s+([A-Za-z ]+?)(?: \[[^]]*\])?: (\d+)', line)
+        if cur and m:
+            out[cur][m.group(1).replace(' ', '')] = int(m.group(2))
+    return out
+
+
 if __name__ == '__main__':
     build(verbose=True)
