@@ -576,6 +576,11 @@ struct bs_bptt {
   int N = 0, NW = 0, W = 0, nck = 0;
   hipStream_t stream = nullptr;
   rocblas_handle blas = nullptr;
+  // the inhibitory rows' GEMM runs on a second stream beside the excitatory
+  // one (the smaller GEMM alone leaves most CUs idle); fork/join events
+  hipStream_t stream2 = nullptr;
+  rocblas_handle blas2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::vector<void*> owned;
   unsigned long long* bm = nullptr;
   unsigned long long* bmT = nullptr;
@@ -625,6 +630,10 @@ void free_bptt(bs_bptt* h) {
   if (!h) return;
   for (void* p : h->owned) (void)hipFree(p);
   if (h->blas) (void)rocblas_destroy_handle(h->blas);
+  if (h->blas2) (void)rocblas_destroy_handle(h->blas2);
+  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+  if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+  if (h->stream2) (void)hipStreamDestroy(h->stream2);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
@@ -758,9 +767,18 @@ int bs_bptt_create(const bs_bptt_desc* d, bs_bptt** out) {
     free_bptt(h);
     return bs_internal_fail(BS_EHIP, "bs_bptt_create: hipStreamCreate failed");
   }
+  if (hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess) {
+    free_bptt(h);
+    return bs_internal_fail(BS_EHIP, "bs_bptt_create: second stream or events");
+  }
   if (rocblas_create_handle(&h->blas) != rocblas_status_success ||
       rocblas_set_stream(h->blas, h->stream) != rocblas_status_success ||
-      rocblas_set_atomics_mode(h->blas, rocblas_atomics_not_allowed) != rocblas_status_success) {
+      rocblas_set_atomics_mode(h->blas, rocblas_atomics_not_allowed) != rocblas_status_success ||
+      rocblas_create_handle(&h->blas2) != rocblas_status_success ||
+      rocblas_set_stream(h->blas2, h->stream2) != rocblas_status_success ||
+      rocblas_set_atomics_mode(h->blas2, rocblas_atomics_not_allowed) != rocblas_status_success) {
     free_bptt(h);
     return bs_internal_fail(BS_EHIP, "bs_bptt_create: rocblas handle");
   }
@@ -871,6 +889,11 @@ int bs_bptt_backward(bs_bptt* h, float* g_in, float* g_rec) {
       if (kmax > 0) {
         const int M = kmax * B;
         const size_t src = (size_t)(w0 + d) * BN;
+        // I rows on the second stream, E rows here; disjoint rows of sbar
+        if (nI > 0) {
+          BCHK(hipEventRecord(h->ev_fork, h->stream));
+          BCHK(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+        }
         if (h->w3) {  // bf16 three-term split on the matrix cores, fp32 accumulate
           const int K3 = 3 * N;
           if (nE > 0)
@@ -880,7 +903,7 @@ int bs_bptt_backward(bs_bptt* h, float* g_in, float* g_rec) {
                                   h->sbar, rocblas_datatype_f32_r, N, rocblas_datatype_f32_r,
                                   rocblas_gemm_algo_standard, 0, 0));
           if (nI > 0)
-            RBCHK(rocblas_gemm_ex(h->blas, rocblas_operation_transpose, rocblas_operation_none, nI, M, K3,
+            RBCHK(rocblas_gemm_ex(h->blas2, rocblas_operation_transpose, rocblas_operation_none, nI, M, K3,
                                   &one, h->w3 + (size_t)nE * K3, rocblas_datatype_bf16_r, K3,
                                   h->ib3i + src * 3, rocblas_datatype_bf16_r, K3, &zero, h->sbar + nE,
                                   rocblas_datatype_f32_r, N, h->sbar + nE, rocblas_datatype_f32_r, N,
@@ -890,9 +913,13 @@ int bs_bptt_backward(bs_bptt* h, float* g_in, float* g_rec) {
             RBCHK(rocblas_sgemm(h->blas, rocblas_operation_transpose, rocblas_operation_none, nE, M, N,
                                 &one, h->w_rec, N, h->ibex + src, N, &zero, h->sbar, N));
           if (nI > 0)
-            RBCHK(rocblas_sgemm(h->blas, rocblas_operation_transpose, rocblas_operation_none, nI, M, N,
+            RBCHK(rocblas_sgemm(h->blas2, rocblas_operation_transpose, rocblas_operation_none, nI, M, N,
                                 &one, h->w_rec + (size_t)nE * N, N, h->ibin + src, N, &zero,
                                 h->sbar + nE, N));
+        }
+        if (nI > 0) {
+          BCHK(hipEventRecord(h->ev_join, h->stream2));
+          BCHK(hipStreamWaitEvent(h->stream, h->ev_join, 0));
         }
       }
       ba.n0 = w0;
