@@ -1089,7 +1089,10 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   HCHK(hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking));
 
   // ---- tile size: the largest of 256/128/64 neurons that still gives >= 1024
-  // tiles (four per CU), so small networks spread over the whole GPU
+  // tiles (four per CU), so small networks spread over the whole GPU. With
+  // aeif_cond_alpha the update (adaptive RKF45) dominates the window, and all
+  // four waves of a block take part in it only at 256 neurons per tile, so
+  // such networks keep the largest tiles that still give >= 128 blocks.
   int TS = 256;
   {
     auto ntiles = [&](int ts) {
@@ -1097,7 +1100,10 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
       for (int p = 0; p < d->n_pops; ++p) t += (d->pops[p].n + ts - 1) / ts;
       return t;
     };
-    while (TS > 64 && ntiles(TS) < 1024) TS /= 2;
+    bool rkf = false;
+    for (int p = 0; p < d->n_pops; ++p) rkf = rkf || d->pops[p].model == BS_AEIF_COND_ALPHA;
+    const long long min_tiles = rkf ? 128 : 1024;
+    while (TS > 64 && ntiles(TS) < min_tiles) TS /= 2;
     if (d->tile_size != 0) {
       if (d->tile_size != 64 && d->tile_size != 128 && d->tile_size != 256)
         BAD("bs_net_create: tile_size must be 0, 64, 128 or 256");
