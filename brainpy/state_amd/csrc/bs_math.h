@@ -135,6 +135,23 @@ __device__ __forceinline__ double log_p(double x) {
   return ed * BS_LN2_HI + (ed * BS_LN2_LO + lm);
 }
 
+// x / c given rc = 1/c (the IEEE quotient, computed once per divisor):
+// q = x*rc lies within one ulp of x/c, r = x - q*c is exact with an FMA, and
+// q + r*rc rounded once is the correctly rounded quotient (Markstein's
+// theorem on FMA division). Three dependent operations, no branch,
+// bit-identical to x / c whenever |q| is in [2^-1000, 2^1000]; zero,
+// infinite and NaN x give q itself (also the IEEE result). Any other case
+// (near-subnormal or huge quotients) sets *bad, and the caller redoes its
+// divisions with x / c.
+__device__ __forceinline__ double div_rc(double x, double c, double rc, bool& bad) {
+  const double q = x * rc;
+  const double aq = __builtin_fabs(q);
+  const bool ok = aq >= 0x1p-1000 && aq <= 0x1p+1000;
+  const bool pass = x == 0.0 || x != x || __builtin_fabs(x) == __builtin_inf();
+  bad = bad || (!ok && !pass);
+  return ok ? __builtin_fma(__builtin_fma(-q, c, x), rc, q) : q;
+}
+
 // x^y for x >= 0 (the only domain used: step-size control, STDP exponents).
 __device__ __forceinline__ double pow_p(double x, double y) {
   if (y == 0.0) return 1.0;

@@ -199,24 +199,47 @@ struct AeifY {
   double V, dgex, gex, dgin, gin, w;
 };
 
-__device__ __forceinline__ void aeif_rhs(const double* __restrict__ P, const AeifY& y, bool refr,
-                                         double istim, AeifY& k) {
-  // jnp.minimum: NaN-propagating
-  const double veff = refr ? P[C_VRESET] : ((y.V != y.V || y.V < P[C_VPEAK]) ? y.V : P[C_VPEAK]);
-  const double isex = y.gex * (veff - P[C_EEX]);
-  const double isin = y.gin * (veff - P[C_EIN]);
-  double ea = (veff - P[C_VTH]) / P[C_DTSAFE];
-  ea = ea < -500.0 ? -500.0 : (ea > 500.0 ? 500.0 : ea);
-  const double ispk = P[C_GLDT] * exp_p(ea);
-  const double dvraw = (P[C_NEGGL] * (veff - P[C_EL]) + ispk - isex - isin - y.w + P[C_IE] + istim) /
-                       P[C_CM];
-  k.V = refr ? 0.0 : dvraw;
-  k.dgex = -y.dgex / P[C_TAUEX];
-  k.gex = y.dgex - y.gex / P[C_TAUEX];
-  k.dgin = -y.dgin / P[C_TAUIN];
-  k.gin = y.dgin - y.gin / P[C_TAUIN];
-  k.w = (P[C_A] * (veff - P[C_EL]) - y.w) / P[C_TAUW];
+// Reciprocals of the divisors of the right-hand side (div_rc), once per step.
+struct AeifR {
+  double dts, cm, tex, tin, tw;
+};
+
+// Right-hand side; DIV(x, c, rc) is the division x / c (exact: plain or
+// reciprocal-corrected, both give the IEEE quotient).
+#define BS_AEIF_RHS(DIV)                                                                     \
+  const double veff = refr ? P[C_VRESET] : ((y.V != y.V || y.V < P[C_VPEAK]) ? y.V : P[C_VPEAK]); \
+  const double isex = y.gex * (veff - P[C_EEX]);                                            \
+  const double isin = y.gin * (veff - P[C_EIN]);                                            \
+  double ea = DIV(veff - P[C_VTH], P[C_DTSAFE], R.dts);                                     \
+  ea = ea < -500.0 ? -500.0 : (ea > 500.0 ? 500.0 : ea);                                   \
+  const double ispk = P[C_GLDT] * exp_p(ea);                                                \
+  const double dvraw =                                                                      \
+      DIV(P[C_NEGGL] * (veff - P[C_EL]) + ispk - isex - isin - y.w + P[C_IE] + istim, P[C_CM], R.cm); \
+  k.V = refr ? 0.0 : dvraw;                                                                 \
+  k.dgex = DIV(-y.dgex, P[C_TAUEX], R.tex);                                                 \
+  k.gex = y.dgex - DIV(y.gex, P[C_TAUEX], R.tex);                                           \
+  k.dgin = DIV(-y.dgin, P[C_TAUIN], R.tin);                                                 \
+  k.gin = y.dgin - DIV(y.gin, P[C_TAUIN], R.tin);                                           \
+  k.w = DIV(P[C_A] * (veff - P[C_EL]) - y.w, P[C_TAUW], R.tw);
+
+__device__ __forceinline__ void aeif_rhs_ieee(const double* __restrict__ P, const AeifR& R, const AeifY& y,
+                                              bool refr, double istim, AeifY& k) {
+#define BS_DIV_IEEE(x, c, rc) ((x) / (c))
+  BS_AEIF_RHS(BS_DIV_IEEE)
+#undef BS_DIV_IEEE
 }
+
+// Every division of one evaluation through div_rc, straight-line; in the rare
+// evaluation where one of them cannot use it, all are redone with x / c.
+__device__ __forceinline__ void aeif_rhs(const double* __restrict__ P, const AeifR& R, const AeifY& y,
+                                         bool refr, double istim, AeifY& k) {
+  bool bad = false;
+#define BS_DIV_RC(x, c, rc) div_rc((x), (c), (rc), bad)
+  BS_AEIF_RHS(BS_DIV_RC)
+#undef BS_DIV_RC
+  if (bad) aeif_rhs_ieee(P, R, y, refr, istim, k);
+}
+#undef BS_AEIF_RHS
 
 #define BS_AX(f, expr) t.f = y.f + h * (expr);
 __device__ __forceinline__ double nanmax(double a, double b) {
@@ -244,6 +267,7 @@ __device__ __forceinline__ bool step_aeif(const double* __restrict__ P, double* 
   double h = s[6];
   const double istim = s[7];
   const double dt = P[C_DT], minh = P[C_MINH], atol = P[C_ATOL];
+  const AeifR R{1.0 / P[C_DTSAFE], 1.0 / P[C_CM], 1.0 / P[C_TAUEX], 1.0 / P[C_TAUIN], 1.0 / P[C_TAUW]};
   const int refn = (int)P[C_REFN];
   bool spiked = false;
   h = h > minh ? h : minh;
@@ -257,22 +281,22 @@ __device__ __forceinline__ bool step_aeif(const double* __restrict__ P, double* 
     }
     const bool refr = r > 0;
     AeifY k0, k1, k2, k3, k4, k5, t;
-    aeif_rhs(P, y, refr, istim, k0);
+    aeif_rhs(P, R, y, refr, istim, k0);
 #define BS_STAGE1(f) t.f = y.f + h * (A10 * k0.f);
     BS_STAGE1(V) BS_STAGE1(dgex) BS_STAGE1(gex) BS_STAGE1(dgin) BS_STAGE1(gin) BS_STAGE1(w)
-    aeif_rhs(P, t, refr, istim, k1);
+    aeif_rhs(P, R, t, refr, istim, k1);
 #define BS_STAGE2(f) t.f = y.f + h * (A20 * k0.f + A21 * k1.f);
     BS_STAGE2(V) BS_STAGE2(dgex) BS_STAGE2(gex) BS_STAGE2(dgin) BS_STAGE2(gin) BS_STAGE2(w)
-    aeif_rhs(P, t, refr, istim, k2);
+    aeif_rhs(P, R, t, refr, istim, k2);
 #define BS_STAGE3(f) t.f = y.f + h * (A30 * k0.f + A31 * k1.f + A32 * k2.f);
     BS_STAGE3(V) BS_STAGE3(dgex) BS_STAGE3(gex) BS_STAGE3(dgin) BS_STAGE3(gin) BS_STAGE3(w)
-    aeif_rhs(P, t, refr, istim, k3);
+    aeif_rhs(P, R, t, refr, istim, k3);
 #define BS_STAGE4(f) t.f = y.f + h * (A40 * k0.f + A41 * k1.f + A42 * k2.f + A43 * k3.f);
     BS_STAGE4(V) BS_STAGE4(dgex) BS_STAGE4(gex) BS_STAGE4(dgin) BS_STAGE4(gin) BS_STAGE4(w)
-    aeif_rhs(P, t, refr, istim, k4);
+    aeif_rhs(P, R, t, refr, istim, k4);
 #define BS_STAGE5(f) t.f = y.f + h * (A50 * k0.f + A51 * k1.f + A52 * k2.f + A53 * k3.f + A54 * k4.f);
     BS_STAGE5(V) BS_STAGE5(dgex) BS_STAGE5(gex) BS_STAGE5(dgin) BS_STAGE5(gin) BS_STAGE5(w)
-    aeif_rhs(P, t, refr, istim, k5);
+    aeif_rhs(P, R, t, refr, istim, k5);
     AeifY yh, ye;
 #define BS_HIGH(f) yh.f = y.f + h * (B0 * k0.f + B2 * k2.f + B3 * k3.f + B4 * k4.f + B5 * k5.f);
     BS_HIGH(V) BS_HIGH(dgex) BS_HIGH(gex) BS_HIGH(dgin) BS_HIGH(gin) BS_HIGH(w)

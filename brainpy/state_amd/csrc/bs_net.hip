@@ -2086,6 +2086,11 @@ __global__ void k_dbg_math(int op, long long n, const double* a, const double* b
   if (op == 0) o[i] = exp_p(a[i]);
   else if (op == 1) o[i] = log_p(a[i]);
   else if (op == 2) o[i] = pow_p(a[i], b[i]);
+  else if (op == 3) {
+    bool bad = false;
+    const double q = div_rc(a[i], b[i], 1.0 / b[i], bad);
+    o[i] = bad ? a[i] / b[i] : q;
+  }
 }
 __global__ void k_dbg_rng(long long n, const uint32_t* k0, const uint32_t* k1, const uint32_t* c0,
                           const uint32_t* c1, uint32_t* o0, uint32_t* o1) {
@@ -2114,7 +2119,7 @@ static int to_dev(X** d, const X* h, long long n) {
 }
 
 int bs_debug_math(int32_t op, int64_t n, const double* a, const double* b, double* out) {
-  if (op < 0 || op > 2 || n < 0 || !a || !out) return fail(BS_EINVAL, "bs_debug_math: bad args");
+  if (op < 0 || op > 3 || n < 0 || !a || !out) return fail(BS_EINVAL, "bs_debug_math: bad args");
   if (n == 0) return BS_OK;
   double *da, *db = nullptr, *dout;
   int rc = to_dev(&da, a, n);

@@ -230,7 +230,8 @@ int bs_bptt_spikes(bs_bptt* h, uint64_t* bitmaps, float* counts);
 /* ---- test hooks: device arithmetic evaluated elementwise ------------------
  * No counterpart in the reference; they let tests compare the device's
  * exp/log/pow, Threefry-2x32 and Poisson sampler bit-for-bit with the C
- * oracle. op: 0 exp, 1 log, 2 pow(a, b). */
+ * oracle. op: 0 exp, 1 log, 2 pow(a, b), 3 a / b through the reciprocal-
+ * corrected division the aeif_cond_alpha update uses (IEEE quotient bits). */
 int bs_debug_math(int32_t op, int64_t n, const double* a, const double* b, double* out);
 int bs_debug_threefry(int64_t n, const uint32_t* k0, const uint32_t* k1, const uint32_t* c0,
                       const uint32_t* c1, uint32_t* o0, uint32_t* o1);

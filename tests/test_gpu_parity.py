@@ -53,6 +53,31 @@ def test_device_math_matches_oracle_bitwise():
     assert np.array_equal(out.view(np.uint64), np.array([oracle.pow(v, 0.2) for v in y]).view(np.uint64))
 
 
+def test_reciprocal_division_is_the_ieee_quotient():
+    """The aeif update divides through a precomputed reciprocal plus one FMA
+    correction; it must give the IEEE quotient's bits for every input,
+    including zeros, infinities, NaN and subnormal quotients."""
+    from brainpy.state_amd import _lib
+    import ctypes as C
+    lib = _lib.lib()
+    rng = np.random.default_rng(7)
+    n = 1 << 20
+    a = rng.standard_normal(n) * np.exp2(rng.integers(-1000, 1000, n).astype(np.float64))
+    b = np.abs(rng.standard_normal(n)) * np.exp2(rng.integers(-40, 40, n).astype(np.float64)) + 1e-300
+    b[: n // 4] = rng.choice([281.0, 0.2, 2.0, 144.0, 0.5, 1e-3], n // 4)  # the model's divisors
+    a[:8] = [0.0, -0.0, np.inf, -np.inf, np.nan, 5e-324, -3e-310, 1.7e308]
+    a[8:16] = [1e-307, 2.2e-308, -1e-306, 3e-300, 1e300, -4e299, 1e-320, 7e-309]
+    b[8:16] = [0.2, 281.0, 144.0, 1e7, 1e-8, 1e-9, 0.5, 3.0]
+    out = np.zeros_like(a)
+    P = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+    _lib.check(lib.bs_debug_math(3, n, P(a), P(b), P(out)))
+    with np.errstate(all='ignore'):
+        ref = a / b
+    nan = np.isnan(ref)
+    assert np.array_equal(np.isnan(out), nan)
+    assert np.array_equal(out[~nan].view(np.uint64), ref[~nan].view(np.uint64))
+
+
 def test_device_threefry_and_poisson_match_oracle():
     from brainpy.state_amd import _lib
     import ctypes as C
