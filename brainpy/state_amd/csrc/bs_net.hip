@@ -456,11 +456,62 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   unsigned short* pc = (unsigned short*)(lds + Lo.pc);
 
   // ---- 0. stage: one level of loads after the tile record -- the blob, the
-  // ring slots of this window, the bucket counts the gather needs
-  {
-    const uint4* src = (const uint4*)(a.blob + tl.blob_off);
-    uint4* dst = (uint4*)blob;
-    for (int k = tid; k < tl.blob_words; k += 256) dst[k] = src[k];
+  // drawn-ahead Poisson counts, the ring slots of this window, the bucket
+  // counts the gather needs. Every load is issued into registers before the
+  // first LDS store waits on one, so the stage costs one memory round trip
+  // (the register slots cover the common sizes; loops take the rest).
+  const long long tile_gid0 = tl.gid0 + tl.start;
+  const int PL = (a.base_ptr && a.widx == 0) ? *a.prevlen_ptr : a.prevlen;
+  const int lgTS = TS == 256 ? 8 : (TS == 128 ? 7 : 6);
+  const int lane_t = tid & (TS - 1), row0 = tid >> lgTS, rows_per = 256 >> lgTS;
+  const bool lane_ok = lane_t < tl.count;
+  const int L2 = 2 * Wm;
+  constexpr int kRB = 2, kRP = 4, kRC = 8, kRF = 4;
+  const uint4* bsrc = (const uint4*)(a.blob + tl.blob_off);
+  uint4 rb[kRB];
+  unsigned short rp[kRP];
+  uint32_t rc[kRC];
+  long long rf[kRF];
+  int rq = 0;
+#pragma unroll
+  for (int i = 0; i < kRB; ++i) {
+    const int k = tid + 256 * i;
+    rb[i] = k < tl.blob_words ? bsrc[k] : make_uint4(0u, 0u, 0u, 0u);
+  }
+#pragma unroll
+  for (int i = 0; i < kRP; ++i) {  // pc[w][lane]: row = step w
+    const int w = row0 + rows_per * i;
+    rp[i] = (tl.pois && w < W && lane_ok)
+                ? a.pcnt[(long long)((a.n0 + w) % L2) * a.Ntot + tile_gid0 + lane_t]
+                : (unsigned short)0;
+  }
+#pragma unroll
+  for (int i = 0; i < kRC; ++i) {  // cnt[w][cw][lane], w < W (no rows when ncw = 0)
+    const int r = row0 + rows_per * i, w = ncw > 0 ? r / ncw : W, cw = r - w * ncw;
+    rc[i] = (w < W && lane_ok)
+                ? a.ring_cnt[((size_t)((a.n0 + w) % a.Lr) * a.NCW + cw) * a.Ntot + tile_gid0 + lane_t]
+                : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < kRF; ++i) {  // fxa[w][fx][lane], w < W
+    const int r = row0 + rows_per * i;
+    const int w = nfx > 0 ? r / nfx : W, fx = r - w * nfx;
+    rf[i] = (nfx > 0 && w < W && lane_ok)
+                ? a.ring_fx[((size_t)((a.n0 + w) % a.Lr) * a.NFX + fx) * a.Ntot + tile_gid0 + lane_t]
+                : 0ll;
+  }
+  // bucket sizes of every (projection, step) the gather reads: the spikes of
+  // the previous window, each delivered once with all of its delays
+  if (tid < ngp * SP) {
+    const int g = tid / SP, t = tid - g * SP;
+    // constant indices only: a runtime index into tl would move it to scratch
+    const int pre_pop = g == 0   ? tl.pre_pop[0]
+                        : g == 1 ? tl.pre_pop[1]
+                        : g == 2 ? tl.pre_pop[2]
+                        : g == 3 ? tl.pre_pop[3]
+                                 : a.gproj[tl.gp_begin + g].pre_pop;
+    const int s = a.n0 - PL + t;
+    rq = (t > 0 && t <= PL) ? (int)a.sb_count[((s - 1) % a.Ls) * a.n_pops + pre_pop] : 0;
   }
   if (tix == 0) {  // clear the bucket counts of the next window's steps
     for (int t = tid; t < Wm * a.n_pops; t += 256) {
@@ -468,51 +519,77 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
       a.sb_count[(st % a.Ls) * a.n_pops + t % a.n_pops] = 0u;
     }
   }
-  // events already deposited for this window's steps: take them out of the ring
-  const long long tile_gid0 = tl.gid0 + tl.start;
-  if (tl.pois) {  // this window's drawn-ahead Poisson counts (from the launch before)
-    const int L2 = 2 * Wm;
-    for (int k = tid; k < W * TS; k += 256) {
-      const int w = k / TS, l = k - w * TS;
-      pc[k] = l < tl.count ? a.pcnt[(long long)((a.n0 + w) % L2) * a.Ntot + tile_gid0 + l] : (unsigned short)0;
+  // now the stores: LDS copies, and the ring slots taken out are zeroed
+  {
+    uint4* dst = (uint4*)blob;
+#pragma unroll
+    for (int i = 0; i < kRB; ++i) {
+      const int k = tid + 256 * i;
+      if (k < tl.blob_words) dst[k] = rb[i];
+    }
+    for (int k = tid + 256 * kRB; k < tl.blob_words; k += 256) dst[k] = bsrc[k];
+  }
+  if (tl.pois) {
+#pragma unroll
+    for (int i = 0; i < kRP; ++i) {
+      const int w = row0 + rows_per * i;
+      if (w < W) pc[w * TS + lane_t] = rp[i];
+    }
+    for (int w = row0 + rows_per * kRP; w < W; w += rows_per)
+      pc[w * TS + lane_t] = lane_ok ? a.pcnt[(long long)((a.n0 + w) % L2) * a.Ntot + tile_gid0 + lane_t]
+                                    : (unsigned short)0;
+  }
+  // events already deposited for this window (rows w < W), the rest zero
+#pragma unroll
+  for (int i = 0; i < kRC; ++i) {
+    const int r = row0 + rows_per * i, w = ncw > 0 ? r / ncw : W, cw = r - w * ncw;
+    if (r < H * ncw) {
+      if (w < W && lane_ok)
+        a.ring_cnt[((size_t)((a.n0 + w) % a.Lr) * a.NCW + cw) * a.Ntot + tile_gid0 + lane_t] = 0u;
+      cnt[r * TS + lane_t] = rc[i];
     }
   }
-  for (int k = tid; k < H * ncw * TS; k += 256) {
-    const int w = k / (ncw * TS), r = k - w * ncw * TS, cw = r / TS, ln = r - cw * TS;
+  for (int r = row0 + rows_per * kRC; r < H * ncw; r += rows_per) {
+    const int w = r / ncw, cw = r - w * ncw;
     uint32_t v = 0u;
-    if (w < W && ln < tl.count) {
-      const size_t o = ((size_t)((a.n0 + w) % a.Lr) * a.NCW + cw) * a.Ntot + tile_gid0 + ln;
+    if (w < W && lane_ok) {
+      const size_t o = ((size_t)((a.n0 + w) % a.Lr) * a.NCW + cw) * a.Ntot + tile_gid0 + lane_t;
       v = a.ring_cnt[o];
       a.ring_cnt[o] = 0u;
     }
-    cnt[k] = v;
+    cnt[r * TS + lane_t] = v;
   }
-  for (int k = tid; k < H * nfx * TS; k += 256) {
-    const int w = k / (nfx * TS), r = k - w * nfx * TS, fx = r / TS, ln = r - fx * TS;
+#pragma unroll
+  for (int i = 0; i < kRF; ++i) {
+    const int r = row0 + rows_per * i;
+    if (r < H * nfx) {
+      const int w = r / nfx, fx = r - w * nfx;
+      if (w < W && lane_ok)
+        a.ring_fx[((size_t)((a.n0 + w) % a.Lr) * a.NFX + fx) * a.Ntot + tile_gid0 + lane_t] = 0;
+      fxa[r * TS + lane_t] = rf[i];
+    }
+  }
+  for (int r = row0 + rows_per * kRF; r < H * nfx; r += rows_per) {
+    const int w = r / nfx, fx = r - w * nfx;
     long long v = 0;
-    if (w < W && ln < tl.count) {
-      const size_t o = ((size_t)((a.n0 + w) % a.Lr) * a.NFX + fx) * a.Ntot + tile_gid0 + ln;
+    if (w < W && lane_ok) {
+      const size_t o = ((size_t)((a.n0 + w) % a.Lr) * a.NFX + fx) * a.Ntot + tile_gid0 + lane_t;
       v = a.ring_fx[o];
       a.ring_fx[o] = 0;
     }
-    fxa[k] = v;
+    fxa[r * TS + lane_t] = v;
   }
   for (int k = tid; k < Wm; k += 256) spn[k] = 0;
-  // bucket sizes of every (projection, step) the gather reads: the spikes of
-  // the previous window, each delivered once with all of its delays
-  const int PL = (a.base_ptr && a.widx == 0) ? *a.prevlen_ptr : a.prevlen;
-  for (int k = tid; k < ngp * SP; k += 256) {
-    const int g = k / SP, t = k % SP;
-    // constant indices only: a runtime index into tl would move it to scratch
-    const int pre_pop = g == 0   ? tl.pre_pop[0]
-                        : g == 1 ? tl.pre_pop[1]
-                        : g == 2 ? tl.pre_pop[2]
-                        : g == 3 ? tl.pre_pop[3]
-                                 : a.gproj[tl.gp_begin + g].pre_pop;
-    const int s_lo = a.n0 - PL;
-    const int s = s_lo + t;
+  if (tid < ngp * SP) {
+    pref[tid] = rq;
+    if (tid % SP == 0) gslo[tid / SP] = a.n0 - PL;
+  }
+  for (int k = tid + 256; k < ngp * SP; k += 256) {  // more buckets than threads
+    const int g = k / SP, t = k - g * SP;
+    const int pre_pop = a.gproj[tl.gp_begin + g].pre_pop;
+    const int s = a.n0 - PL + t;
     pref[k] = (t > 0 && t <= PL) ? (int)a.sb_count[((s - 1) % a.Ls) * a.n_pops + pre_pop] : 0;
-    if (t == 0) gslo[g] = s_lo;
+    if (t == 0) gslo[g] = a.n0 - PL;
   }
   __syncthreads();
   if (tid < ngp) {  // per-projection inclusive prefix over steps
