@@ -727,34 +727,63 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
     }
   }
 
-  // ---- 2. inputs of every (step, lane), channels summed in order, all threads
+  // ---- 2. inputs of every (step, lane), channels summed in order, all threads.
+  // Channel-outer over the (up to four) items of each thread, so the items'
+  // LDS reads of one channel overlap; every item still adds its channels in
+  // the summation order.
   {
     const T* tables = (const T*)a.tables;
-    ChReg<T> cr[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-      if (c < nch) cr[c] = load_ch<T>(chs[c], tab, tables);
     UpdCtx<T> u{a, cnt, fxa, spl, spn, chs, tab, tables, TS, ncw, nfx, nch, tl.start, tid,
                 tid < tl.count, (long long)gpop.gid0 + tl.start + tid, inp, nrec, Wm, pc};
-    for (int idx = tid; idx < W * TS; idx += 256) {
-      const int w = idx / TS, l = idx - w * TS;
-      if (l >= tl.count) continue;
-      const int step = a.n0 + w;
-      T acc0 = (T)0, acc1 = (T)0;
+    constexpr int kIT = 4;
+    const int nit = W * TS;
+    if (nit <= 256 * kIT) {
+      T acc0[kIT], acc1[kIT];
+      bool ok[kIT];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        if (c < nch) {
-          const T v = chan_value<T>(u, cr[c], w, step, l);
-          if (cr[c].receptor == 0) acc0 = acc0 + v; else acc1 = acc1 + v;
+      for (int i = 0; i < kIT; ++i) {
+        const int idx = tid + 256 * i;
+        ok[i] = idx < nit && (idx & (TS - 1)) < tl.count;
+        acc0[i] = (T)0;
+        acc1[i] = (T)0;
+      }
+      for (int c = 0; c < nch; ++c) {
+        const ChReg<T> cc = load_ch<T>(chs[c], tab, tables);
+        T v[kIT];
+#pragma unroll
+        for (int i = 0; i < kIT; ++i) {
+          const int idx = tid + 256 * i, w = idx >> lgTS, l = idx & (TS - 1);
+          v[i] = ok[i] ? chan_value<T>(u, cc, w, a.n0 + w, l) : (T)0;
+        }
+        if (cc.receptor == 0) {
+#pragma unroll
+          for (int i = 0; i < kIT; ++i) acc0[i] = acc0[i] + v[i];
+        } else {
+#pragma unroll
+          for (int i = 0; i < kIT; ++i) acc1[i] = acc1[i] + v[i];
         }
       }
-      for (int c = 4; c < nch; ++c) {
-        const ChReg<T> cc = load_ch<T>(chs[c], tab, tables);
-        const T v = chan_value<T>(u, cc, w, step, l);
-        if (cc.receptor == 0) acc0 = acc0 + v; else acc1 = acc1 + v;
+#pragma unroll
+      for (int i = 0; i < kIT; ++i) {
+        const int idx = tid + 256 * i, w = idx >> lgTS, l = idx & (TS - 1);
+        if (ok[i]) {
+          inp[w * TS + l] = acc0[i];
+          if (nrec > 1) inp[(Wm + w) * TS + l] = acc1[i];
+        }
       }
-      inp[w * TS + l] = acc0;
-      if (nrec > 1) inp[(Wm + w) * TS + l] = acc1;
+    } else {
+      for (int idx = tid; idx < nit; idx += 256) {
+        const int w = idx >> lgTS, l = idx & (TS - 1);
+        if (l >= tl.count) continue;
+        T acc0 = (T)0, acc1 = (T)0;
+        for (int c = 0; c < nch; ++c) {
+          const ChReg<T> cc = load_ch<T>(chs[c], tab, tables);
+          const T v = chan_value<T>(u, cc, w, a.n0 + w, l);
+          if (cc.receptor == 0) acc0 = acc0 + v; else acc1 = acc1 + v;
+        }
+        inp[w * TS + l] = acc0;
+        if (nrec > 1) inp[(Wm + w) * TS + l] = acc1;
+      }
     }
   }
   __syncthreads();
