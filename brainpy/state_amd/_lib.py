@@ -15,7 +15,7 @@ __all__ = ['lib', 'EngineUnavailable', 'EngineError', 'check', 'BS_NPARAM', 'BS_
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libbrainspike_hip.so')
 
-BS_ABI_VERSION = 1
+BS_ABI_VERSION = 2
 BS_NPARAM = 40
 BS_NSTATE = 9
 BS_F64, BS_F32 = 0, 1
@@ -81,7 +81,8 @@ class BpttDesc(C.Structure):
                                           'delay_steps', 'delay_in_steps', 'checkpoint', 'loss',
                                           'ref_steps', 'gemm')] +
                 [(k, C.c_double) for k in ('dt_ms', 'E_L', 'V_th', 'V_reset', 'I_e', 'P22', 'P21ex',
-                                           'P21in', 'P11ex', 'P11in', 'P20', 'surrogate_alpha')])
+                                           'P21in', 'P11ex', 'P11in', 'P20', 'surrogate_alpha')] +
+                [('replay', C.c_int32), ('_pad', C.c_int32)])
 
 
 BS_LOSS_RATE_MSE, BS_LOSS_MEMBRANE_MSE = 0, 1

@@ -104,10 +104,17 @@ class BPTT:
         products of the hi/lo split of both operands accumulated in fp32
         (matrix-core rate, about 2^-16 relative error per product), or plain
         fp32.
+    replay : {'auto', True, False}
+        How the backward pass rebuilds a segment's surrogate factors from its
+        checkpoint: True stores every step's summed inputs during the forward
+        pass (8 bytes per neuron, trial and step) and replays the neuron steps
+        from them; False recomputes the segment with its event gathers;
+        'auto' stores when that takes at most 32 GiB. The gradients are the
+        same bits either way.
     """
 
     def __init__(self, net, *, batch, n_steps, checkpoint=100, loss='rate_mse', detach_reset=True,
-                 gemm='bf16x3'):
+                 gemm='bf16x3', replay='auto'):
         if not detach_reset:
             raise NotImplementedError('only detach_reset=True is implemented')
         if loss not in ('rate_mse', 'membrane_mse'):
@@ -126,6 +133,9 @@ class BPTT:
         if gemm not in ('bf16x3', 'fp32'):
             raise ValueError(f"gemm must be 'bf16x3' or 'fp32', got {gemm!r}")
         d.gemm = 0 if gemm == 'bf16x3' else 1
+        if replay not in ('auto', True, False):
+            raise ValueError(f"replay must be 'auto', True or False, got {replay!r}")
+        d.replay = -1 if replay == 'auto' else int(bool(replay))
         d.dt_ms = net.dt
         for f in ('E_L', 'V_th', 'V_reset', 'I_e', 'P22', 'P21ex', 'P21in', 'P11ex', 'P11in', 'P20',
                   'surrogate_alpha'):

@@ -85,6 +85,7 @@ struct FwdArgs {
   const float* target;
   float* act;
   float* vbuf;
+  float *xin_e, *xin_i;  // summed inputs per (step, trial, neuron), or null
   int* err;
 };
 
@@ -93,6 +94,29 @@ struct FwdArgs {
 __device__ __forceinline__ float surrogate_dv(float m, const Consts& c) {
   const float q = c.alpha * fabsf(m) + 1.0f;
   return 1.0f / (q * q) / c.span;
+}
+
+// One step of iaf_psc_exp (iaf_psc_exp.py:136-151) on the summed inputs, then
+// the spike rule; returns the margin m and whether the neuron was refractory.
+// The forward pass and the segment replay both use it, so a replayed step is
+// bit-identical to the forward one.
+__device__ __forceinline__ bool iaf_exp_step(const Consts& c, float& V, float& ie, float& ii, int& r, float xe,
+                                             float xi, bool on, float& mg) {
+  const bool ref = r > 0;
+  float y = V - c.E_L;
+  const float y_new = ((y * c.P22 + ie * c.P21ex) + ii * c.P21in) + c.I_e * c.P20;
+  if (!ref) y = y_new;
+  ie = ie * c.P11ex + xe;
+  ii = ii * c.P11in + xi;
+  V = y + c.E_L;
+  mg = (V - c.V_th) / (c.V_th - c.V_reset);
+  const bool hard = on && !ref && mg >= 0.f;
+  if (hard) {
+    V = c.V_reset;
+    r = c.reset_to;
+  }
+  r = r > 0 ? r - 1 : 0;
+  return ref;
 }
 
 constexpr int kChunk = 8;      // steps whose events are compacted together
@@ -269,25 +293,17 @@ __global__ __launch_bounds__(kTile) void k_bptt_fwd(FwdArgs a) {
         a.ck_r[o] = r;
       }
       const float xe = xs[0][k][tid], xi = xs[1][k][tid];
-      // the neuron update (iaf_psc_exp.py:136-151), then the spike rule
-      const bool ref = r > 0;
-      float y = V - c.E_L;
-      const float y_new = ((y * c.P22 + ie * c.P21ex) + ii * c.P21in) + c.I_e * c.P20;
-      if (!ref) y = y_new;
-      ie = ie * c.P11ex + xe;
-      ii = ii * c.P11in + xi;
-      V = y + c.E_L;
-      const float mg = (V - c.V_th) / (c.V_th - c.V_reset);
+      if (a.mode == 0 && on && a.xin_e) {
+        a.xin_e[(size_t)n * BN + bj] = xe;
+        a.xin_i[(size_t)n * BN + bj] = xi;
+      }
+      float mg;
+      const bool ref = iaf_exp_step(c, V, ie, ii, r, xe, xi, on, mg);
       const bool hard = on && !ref && mg >= 0.f;
       if (a.mode == 1 && on) {
         const size_t o = (size_t)(n - a.seg0) * BN + bj;
         a.act[o] = ref ? -1.0f : surrogate_dv(mg, c);
       }
-      if (hard) {
-        V = c.V_reset;
-        r = c.reset_to;
-      }
-      r = r > 0 ? r - 1 : 0;
       if (a.mode == 1 && on && a.vbuf) a.vbuf[(size_t)(n - a.seg0) * BN + bj] = V;
       if (a.mode == 0) {
         const unsigned long long mask = __ballot(hard);
@@ -310,6 +326,30 @@ __global__ __launch_bounds__(kTile) void k_bptt_fwd(FwdArgs a) {
     a.sr[bj] = r;
     a.counts[bj] = cnt;
     a.lpart[bj] = lp;
+  }
+}
+
+// Replay of a checkpoint segment from the stored summed inputs: the same
+// neuron steps as the forward pass (bit-identical V), writing the surrogate
+// factors (and V for the membrane loss) the reverse pass reads.
+__global__ __launch_bounds__(kTile) void k_bptt_replay(FwdArgs a) {
+  const int b = (int)blockIdx.y;
+  const int j = (int)blockIdx.x * kTile + (int)threadIdx.x;
+  if (j >= a.N) return;
+  const size_t BN = (size_t)a.B * a.N;
+  const size_t bj = (size_t)b * a.N + j;
+  const Consts& c = a.c;
+  const size_t o0 = (size_t)(a.seg0 / a.C) * BN + bj;
+  float V = a.ck_v[o0], ie = a.ck_e[o0], ii = a.ck_i[o0];
+  int r = a.ck_r[o0];
+  for (int k = 0; k < a.nsteps; ++k) {
+    const int n = a.n0 + k;
+    const float xe = a.xin_e[(size_t)n * BN + bj], xi = a.xin_i[(size_t)n * BN + bj];
+    float mg;
+    const bool ref = iaf_exp_step(c, V, ie, ii, r, xe, xi, true, mg);
+    const size_t o = (size_t)(n - a.seg0) * BN + bj;
+    a.act[o] = ref ? -1.0f : surrogate_dv(mg, c);
+    if (a.vbuf) a.vbuf[o] = V;
   }
 }
 
@@ -596,6 +636,7 @@ struct bs_bptt {
   float* act = nullptr;
   float* vbuf = nullptr;
   float *ibex = nullptr, *ibin = nullptr;
+  float *xin_e = nullptr, *xin_i = nullptr;  // stored summed inputs (segment replay)
   unsigned short *w3 = nullptr, *ib3e = nullptr, *ib3i = nullptr;
 
   double* lpart_blocks = nullptr;
@@ -670,6 +711,8 @@ FwdArgs fwd_args(bs_bptt* h) {
   a.target = h->target;
   a.act = h->act;
   a.vbuf = h->vbuf;
+  a.xin_e = h->xin_e;
+  a.xin_i = h->xin_i;
   a.err = h->err;
   return a;
 }
@@ -750,6 +793,18 @@ int bs_bptt_create(const bs_bptt_desc* d, bs_bptt** out) {
   if (d->loss == BS_LOSS_MEMBRANE_MSE) BALLOC(h->vbuf, (size_t)std::min(C, T) * BN);
   BALLOC(h->ibex, (size_t)T * BN);
   BALLOC(h->ibin, (size_t)T * BN);
+  // With replay, the forward pass also stores every step's summed inputs
+  // (by default when they take at most 32 GiB): the reverse pass then replays
+  // each checkpoint segment with the neuron steps alone instead of gathering
+  // its events again.
+  if (d->replay < -1 || d->replay > 1) {
+    free_bptt(h);
+    return bs_internal_fail(BS_EINVAL, "bs_bptt_create: replay must be -1, 0 or 1");
+  }
+  if (d->replay == 1 || (d->replay == -1 && (double)T * BN * 8.0 <= 32.0 * (1ull << 30))) {
+    BALLOC(h->xin_e, (size_t)T * BN);
+    BALLOC(h->xin_i, (size_t)T * BN);
+  }
   if (d->gemm == 0) {
     BALLOC(h->w3, (size_t)N * 3 * N);
     BALLOC(h->ib3e, (size_t)T * BN * 3);
@@ -876,7 +931,10 @@ int bs_bptt_backward(bs_bptt* h, float* g_in, float* g_rec) {
     fa.seg0 = s0;
     fa.n0 = s0;
     fa.nsteps = s1 - s0;
-    hipLaunchKernelGGL(k_bptt_fwd, grid, dim3(kTile), 0, h->stream, fa);
+    if (h->xin_e)
+      hipLaunchKernelGGL(k_bptt_replay, grid, dim3(kTile), 0, h->stream, fa);
+    else
+      hipLaunchKernelGGL(k_bptt_fwd, grid, dim3(kTile), 0, h->stream, fa);
     // windows of W steps from the end of the segment
     for (int w1 = s1; w1 > s0; w1 -= h->W) {
       const int w0 = std::max(s0, w1 - h->W);

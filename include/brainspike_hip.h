@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define BS_ABI_VERSION 1
+#define BS_ABI_VERSION 2
 
 /* ---- return codes ------------------------------------------------------ */
 #define BS_OK 0
@@ -211,6 +211,13 @@ typedef struct bs_bptt_desc {
   double E_L, V_th, V_reset, I_e;                 /* mV, mV, mV, pA          */
   double P22, P21ex, P21in, P11ex, P11in, P20;    /* exact propagators       */
   double surrogate_alpha;       /* slope of the fast sigmoid on the margin   */
+  int32_t replay;               /* reverse pass, per checkpoint segment: -1
+                                   auto (1 when the summed inputs take at most
+                                   32 GiB), 0 recompute the segment with its
+                                   event gathers, 1 the forward pass stores
+                                   every step's summed inputs and the segment
+                                   is replayed from them (same bits)          */
+  int32_t _pad;
 } bs_bptt_desc;
 
 int bs_bptt_create(const bs_bptt_desc* desc, bs_bptt** out);

@@ -43,11 +43,12 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize('checkpoint,gemm', [(100, 'bf16x3'), (37, 'bf16x3'), (100, 'fp32')])
-def test_bptt_forward_bitexact_and_gradient_vs_autograd(checkpoint, gemm):
+@pytest.mark.parametrize('checkpoint,gemm,replay', [(100, 'bf16x3', True), (37, 'bf16x3', True),
+                                                    (37, 'bf16x3', False), (100, 'fp32', 'auto')])
+def test_bptt_forward_bitexact_and_gradient_vs_autograd(checkpoint, gemm, replay):
     net, cnt, w_in, w_rec, target = _problem()
     T, B = cnt.shape[:2]
-    bp = BPTT(net, batch=B, n_steps=T, checkpoint=checkpoint, gemm=gemm)
+    bp = BPTT(net, batch=B, n_steps=T, checkpoint=checkpoint, gemm=gemm, replay=replay)
     win_d, wrec_d, cnt_d, tgt_d = _dev(w_in, w_rec, cnt, target)
     loss, g_in, g_rec = bp.loss_and_grad(win_d, wrec_d, cnt_d, tgt_d)
     fwd = R.forward_f32(net, w_in, w_rec, cnt, target)
@@ -100,3 +101,22 @@ def test_bptt_deterministic_and_torch_function():
     assert float(loss.detach()) == l1
     assert torch.allclose(wi.grad, 3.0 * a_in, rtol=1e-6, atol=0)
     assert torch.allclose(wr.grad, 3.0 * a_rec, rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize('loss', ['rate_mse', 'membrane_mse'])
+def test_bptt_segment_replay_gives_the_recompute_bits(loss):
+    """Replaying checkpoint segments from the stored summed inputs and
+    recomputing them with their event gathers give identical gradients."""
+    net, cnt, w_in, w_rec, target = _problem(T=300)
+    T, B = cnt.shape[:2]
+    if loss == 'membrane_mse':
+        target = np.full_like(target, -65.0)
+    win_d, wrec_d, cnt_d, tgt_d = _dev(w_in, w_rec, cnt, target)
+    out = {}
+    for replay in (True, False):
+        bp = BPTT(net, batch=B, n_steps=T, checkpoint=64, loss=loss, replay=replay)
+        out[replay] = bp.loss_and_grad(win_d, wrec_d, cnt_d, tgt_d)
+        bp.close()
+    assert out[True][0] == out[False][0]
+    assert torch.equal(out[True][1], out[False][1]) and torch.equal(out[True][2], out[False][2])
+    assert float(out[True][2].abs().max()) > 0
