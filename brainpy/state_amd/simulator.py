@@ -230,6 +230,7 @@ class Simulator:
         self._engine = None
         self._base_step = 0
         self._rollout_ready = False
+        self._edited_mid_rollout = False  # connections written while a rollout was running
 
     # -- creation --------------------------------------------------------
     def create(self, model_cls, size=1, *, params=None, positions=None, **kw) -> NodeView:
@@ -436,12 +437,17 @@ class Simulator:
         eng.reset()
         self._base_step = 0
         self._rollout_ready = True
+        self._edited_mid_rollout = False
 
     def simulate(self, duration, *, dt=None) -> SimulationResult:
         self.reset_rollout()
         return self._run_window(duration)
 
     def cont(self, duration, *, dt=None) -> SimulationResult:
+        if self._edited_mid_rollout:
+            raise RuntimeError('connections were edited (SynapseCollection.set) after this rollout '
+                               'started; the engine builds its delivery tables from them, so start a '
+                               'new rollout with simulate() (or reset_rollout())')
         if not self._rollout_ready:
             self.reset_rollout()
         return self._run_window(duration)
@@ -449,6 +455,20 @@ class Simulator:
     def n_steps_of(self, duration) -> int:
         T = u.scalar(duration, 'time')
         return int(math.ceil(T / self.dt))   # len(arange(0, T, dt))
+
+    def get_connections(self, source=None, target=None, synapse=None):
+        """Realised synapses matching the filters, as a ``SynapseCollection``
+        (NEST ``GetConnections``; simulator.py:832-866): ``source`` / ``target``
+        are NodeViews matched by population and local index, ``synapse`` a model
+        name ('static_synapse', 'stdp_synapse')."""
+        from .connections import collect_connections
+        return collect_connections(self, source, target, synapse)
+
+    def _connections_changed(self):
+        if self._rollout_ready:
+            self._edited_mid_rollout = True
+        self._invalidate()
+        self._rollout_ready = False
 
     def get_weights(self, proj):
         """Current weights of a plastic projection, in the reference's CSR
