@@ -34,10 +34,12 @@ enum { C_VPEAK = 0, C_VRESET, C_GL, C_CM, C_EEX, C_EIN, C_EL, C_DT_, C_TAUW, C_A
 
 // ---- iaf_psc_delta ------------------------------------------------------
 // v = E_L + (V-E_L)*decay + (I/C)*tau*(1-decay)  [the last term is the
-// constant D_CI]; v += sum(delta); refractory holds V; spike iff v >= V_th.
+// constant D_CI without current inputs; with them I = (I_e + 0) + currents of
+// this step, evaluated per step]; v += sum(delta); refractory holds V; spike
+// iff v >= V_th.
 template <typename T>
 struct DeltaP {
-  T EL, decay, cI, Vth, Vreset, Vmin;
+  T EL, decay, cI, Vth, Vreset, Vmin, Cm, taum, one_m_decay;
   int refn, refin, has_vmin;
   const double* P;  // refractory decay table (refractory_input)
 };
@@ -50,6 +52,9 @@ __device__ __forceinline__ DeltaP<T> load_delta(const double* P) {
   p.Vth = (T)P[D_VTH];
   p.Vreset = (T)P[D_VRESET];
   p.Vmin = (T)P[D_VMIN];
+  p.Cm = (T)P[32];
+  p.taum = (T)P[33];
+  p.one_m_decay = (T)1 - p.decay;
   p.refn = (int)P[D_REFN];
   p.refin = P[D_REFIN] != 0.0;
   p.has_vmin = P[D_VMIN] > -1e300;
@@ -57,9 +62,11 @@ __device__ __forceinline__ DeltaP<T> load_delta(const double* P) {
   return p;
 }
 template <typename T>
-__device__ __forceinline__ bool step_delta(const DeltaP<T>& p, T* s, int& r, T din) {
+__device__ __forceinline__ bool step_delta(const DeltaP<T>& p, T* s, int& r, T din, T i_total,
+                                           bool has_cur) {
   const T last_v = s[0];
-  T v = p.EL + (last_v - p.EL) * p.decay + p.cI;
+  const T cI = has_cur ? (i_total / p.Cm) * p.taum * p.one_m_decay : p.cI;
+  T v = p.EL + (last_v - p.EL) * p.decay + cI;
   v = v + din;
   if (p.refin) v = v + s[1];
   if (p.has_vmin) v = v > p.Vmin ? v : p.Vmin;
@@ -101,7 +108,7 @@ __device__ __forceinline__ ExpP<T> load_exp(const double* P) {
   return p;
 }
 template <typename T>
-__device__ __forceinline__ bool step_exp(const ExpP<T>& p, T* s, int& r, T w_all) {
+__device__ __forceinline__ bool step_exp(const ExpP<T>& p, T* s, int& r, T w_all, T cur) {
   T V_rel = s[0] - p.EL;
   T iex = s[1], iin = s[2];
   const T i0 = s[3], i1 = s[4];
@@ -122,7 +129,7 @@ __device__ __forceinline__ bool step_exp(const ExpP<T>& p, T* s, int& r, T w_all
   s[0] = V_rel + p.EL;
   s[1] = iex;
   s[2] = iin;
-  s[3] = (T)0;  // next-step latched current: x + sum_current_inputs = 0 on this path
+  s[3] = cur;  // i_0 for the next step: x + sum_current_inputs (x = 0)
   s[4] = (T)0;
   return spike;
 }
@@ -161,7 +168,7 @@ __device__ __forceinline__ AlphaP<T> load_alpha(const double* P) {
   return p;
 }
 template <typename T>
-__device__ __forceinline__ bool step_alpha(const AlphaP<T>& p, T* s, int& r, T w_all) {
+__device__ __forceinline__ bool step_alpha(const AlphaP<T>& p, T* s, int& r, T w_all, T cur) {
   const T wex = w_all > (T)0 ? w_all : (T)0;
   const T win = w_all < (T)0 ? w_all : (T)0;
   T y3 = s[0] - p.EL;
@@ -187,7 +194,7 @@ __device__ __forceinline__ bool step_alpha(const AlphaP<T>& p, T* s, int& r, T w
   s[2] = Iin;
   s[3] = dIex;
   s[4] = dIin;
-  s[5] = (T)0;
+  s[5] = cur;  // y0 for the next step: x + sum_current_inputs (x = 0)
   return spike;
 }
 
@@ -250,7 +257,7 @@ __device__ __forceinline__ double nanmax(double a, double b) {
 // One simulation step of aeif_cond_alpha for one neuron. Returns spike flag;
 // sets *unstable when the reference's guard would fire.
 __device__ __forceinline__ bool step_aeif(const double* __restrict__ P, double* s, int& r,
-                                          double wex_in, double win_in, int* unstable) {
+                                          double wex_in, double win_in, double cur, int* unstable) {
   const double A10 = 1.0 / 4.0;
   const double A20 = 3.0 / 32.0, A21 = 9.0 / 32.0;
   const double A30 = 1932.0 / 2197.0, A31 = -7200.0 / 2197.0, A32 = 7296.0 / 2197.0;
@@ -357,7 +364,7 @@ __device__ __forceinline__ bool step_aeif(const double* __restrict__ P, double* 
   s[4] = y.gin;
   s[5] = y.w;
   s[6] = h;
-  s[7] = 0.0;  // I_stim for the next step: x + sum_current_inputs = 0 on this path
+  s[7] = cur;  // I_stim for the next step: x + sum_current_inputs (x = 0)
   return spiked;
 }
 

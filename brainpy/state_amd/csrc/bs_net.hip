@@ -84,7 +84,7 @@ struct DevPop {
   int n_state;       // state variables used by the model
   int gp_begin, gp_count;  // incoming static projections (gather list)
   int tile0;         // first tile of this population
-  int pad;
+  int n_cur;         // current-input (dc_generator) channels
   long long tab_bytes;  // bytes of tables staged into LDS by k_window
   double P[BS_NPARAM];
 };
@@ -114,6 +114,7 @@ struct Tile {
   int pop, start, count, tile_in_pop;
   int ncw, nfx, ngp, nch;
   int nrec, blob_words, gp_begin, pois;  // pois: the population has a drawn-ahead channel
+  int ncur, pad2;    // ncur: 1 if the population has current-input channels
   int pre_pop[4];   // pre populations of the first four gather sources
   long long gid0, blob_off, tab_bytes;
 };
@@ -305,6 +306,7 @@ struct UpdCtx {
   const T* inp;
   int nrec, Wm;
   const unsigned short* pc;  // [Wm][TS] drawn-ahead Poisson counts of this tile
+  int ncur;                  // 1: row nrec of inp holds the summed current input
 };
 
 // Input of one channel at window step w for lane tid (neuron j).
@@ -328,6 +330,11 @@ __device__ __forceinline__ T chan_value(const UpdCtx<T>& u, const ChReg<T>& c, i
     if (e >= 0 && sgs >= 0 && sgs > c.tmin && sgs <= c.tmax)
       cn = poisson_inv(c.seed, (uint32_t)sgs, (uint32_t)e, c.cdf, c.guide, c.cdf_len, c.parts);
     return (T)cn * c.weight;
+  }
+  if (c.kind == BS_CH_CURRENT) {  // dc_generator: amplitude * weight inside [tmin, tmax)
+    const int j = u.start + lane;
+    const int e = c.gen_index ? c.gen_index[j] : j;
+    return (e >= 0 && (long long)step >= c.tmin && (long long)step < c.tmax) ? c.weight : (T)0;
   }
   long long q;  // BS_CH_FIXED
   if (c.pacc) {  // plastic: this step's deposits (windows of one step), consumed here
@@ -358,7 +365,8 @@ __device__ __forceinline__ void update_loop(const UpdCtx<T>& u, const ChReg<T>* 
     const int step = a.n0 + w;
     const T acc0 = u.inp[w * u.TS + u.tid];
     const T acc1 = u.nrec > 1 ? u.inp[(u.Wm + w) * u.TS + u.tid] : (T)0;
-    const bool spike = stepf(s, r, acc0, acc1);
+    const T cur = u.ncur ? u.inp[(u.nrec * u.Wm + w) * u.TS + u.tid] : (T)0;
+    const bool spike = stepf(s, r, acc0, acc1, cur);
     if (vmc >= 0) a.vm_buf[(long long)(step - a.vm_step0) * a.n_vm + vmc] = (double)s[0];
     if (a.spk_hist) a.spk_hist[(long long)(step % a.Ls) * a.Ntot + u.gid] = spike ? 1 : 0;
     if (spike) {
@@ -435,8 +443,8 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   const int SP = a.max_span + 1;
   const int nrec = tl.nrec;
   const int H = a.H;
-  const WinLayout Lo = win_layout(Wm, H, TS, ncw, nfx, ngp, SP, nch, nrec, (int)sizeof(T), tl.tab_bytes,
-                                  tl.pois);
+  const WinLayout Lo = win_layout(Wm, H, TS, ncw, nfx, ngp, SP, nch, nrec + tl.ncur, (int)sizeof(T),
+                                  tl.tab_bytes, tl.pois);
   const BlobLayout Bl = blob_layout(nch, ngp, tl.tab_bytes);
   uint32_t* cnt = (uint32_t*)(lds + Lo.cnt);
   long long* fxa = (long long*)(lds + Lo.fxa);
@@ -734,11 +742,14 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   {
     const T* tables = (const T*)a.tables;
     UpdCtx<T> u{a, cnt, fxa, spl, spn, chs, tab, tables, TS, ncw, nfx, nch, tl.start, tid,
-                tid < tl.count, (long long)gpop.gid0 + tl.start + tid, inp, nrec, Wm, pc};
+                tid < tl.count, (long long)gpop.gid0 + tl.start + tid, inp, nrec, Wm, pc, tl.ncur};
     constexpr int kIT = 4;
     const int nit = W * TS;
+    // current inputs are summed apart from the delta inputs, starting from
+    // I_e + x (x = 0) for iaf_psc_delta and from x = 0 for the latching models
+    const T cur0 = gpop.model == BS_IAF_PSC_DELTA ? (T)P[34] + (T)0 : (T)0;
     if (nit <= 256 * kIT) {
-      T acc0[kIT], acc1[kIT];
+      T acc0[kIT], acc1[kIT], accC[kIT];
       bool ok[kIT];
 #pragma unroll
       for (int i = 0; i < kIT; ++i) {
@@ -746,6 +757,7 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
         ok[i] = idx < nit && (idx & (TS - 1)) < tl.count;
         acc0[i] = (T)0;
         acc1[i] = (T)0;
+        accC[i] = cur0;
       }
       for (int c = 0; c < nch; ++c) {
         const ChReg<T> cc = load_ch<T>(chs[c], tab, tables);
@@ -755,7 +767,10 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
           const int idx = tid + 256 * i, w = idx >> lgTS, l = idx & (TS - 1);
           v[i] = ok[i] ? chan_value<T>(u, cc, w, a.n0 + w, l) : (T)0;
         }
-        if (cc.receptor == 0) {
+        if (cc.kind == BS_CH_CURRENT) {
+#pragma unroll
+          for (int i = 0; i < kIT; ++i) accC[i] = accC[i] + v[i];
+        } else if (cc.receptor == 0) {
 #pragma unroll
           for (int i = 0; i < kIT; ++i) acc0[i] = acc0[i] + v[i];
         } else {
@@ -769,20 +784,24 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
         if (ok[i]) {
           inp[w * TS + l] = acc0[i];
           if (nrec > 1) inp[(Wm + w) * TS + l] = acc1[i];
+          if (tl.ncur) inp[(nrec * Wm + w) * TS + l] = accC[i];
         }
       }
     } else {
       for (int idx = tid; idx < nit; idx += 256) {
         const int w = idx >> lgTS, l = idx & (TS - 1);
         if (l >= tl.count) continue;
-        T acc0 = (T)0, acc1 = (T)0;
+        T acc0 = (T)0, acc1 = (T)0, accC = cur0;
         for (int c = 0; c < nch; ++c) {
           const ChReg<T> cc = load_ch<T>(chs[c], tab, tables);
           const T v = chan_value<T>(u, cc, w, a.n0 + w, l);
-          if (cc.receptor == 0) acc0 = acc0 + v; else acc1 = acc1 + v;
+          if (cc.kind == BS_CH_CURRENT) accC = accC + v;
+          else if (cc.receptor == 0) acc0 = acc0 + v;
+          else acc1 = acc1 + v;
         }
         inp[w * TS + l] = acc0;
         if (nrec > 1) inp[(Wm + w) * TS + l] = acc1;
+        if (tl.ncur) inp[(nrec * Wm + w) * TS + l] = accC;
       }
     }
   }
@@ -793,28 +812,28 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
     const T* tables = (const T*)a.tables;
     ChReg<T> cr[4];
     UpdCtx<T> u{a, cnt, fxa, spl, spn, chs, tab, tables, TS, ncw, nfx, nch, tl.start, tid,
-                tid < tl.count, (long long)gpop.gid0 + tl.start + tid, inp, nrec, Wm, pc};
+                tid < tl.count, (long long)gpop.gid0 + tl.start + tid, inp, nrec, Wm, pc, tl.ncur};
     switch (gpop.model) {
       case BS_IAF_PSC_DELTA: {
         const DeltaP<T> mp = load_delta<T>(P);
         update_loop<T>(u, cr, gpop.n_state,
-                       [&](T* s, int& r, T a0, T) { return step_delta<T>(mp, s, r, a0); });
+                       [&](T* s, int& r, T a0, T, T cur) { return step_delta<T>(mp, s, r, a0, cur, tl.ncur != 0); });
       } break;
       case BS_IAF_PSC_EXP: {
         const ExpP<T> mp = load_exp<T>(P);
         update_loop<T>(u, cr, gpop.n_state,
-                       [&](T* s, int& r, T a0, T) { return step_exp<T>(mp, s, r, a0); });
+                       [&](T* s, int& r, T a0, T, T cur) { return step_exp<T>(mp, s, r, a0, cur); });
       } break;
       case BS_IAF_PSC_ALPHA: {
         const AlphaP<T> mp = load_alpha<T>(P);
         update_loop<T>(u, cr, gpop.n_state,
-                       [&](T* s, int& r, T a0, T) { return step_alpha<T>(mp, s, r, a0); });
+                       [&](T* s, int& r, T a0, T, T cur) { return step_alpha<T>(mp, s, r, a0, cur); });
       } break;
       case BS_AEIF_COND_ALPHA: {
         if constexpr (std::is_same<T, double>::value) {
-          update_loop<T>(u, cr, gpop.n_state, [&](T* s, int& r, T a0, T a1) {
+          update_loop<T>(u, cr, gpop.n_state, [&](T* s, int& r, T a0, T a1, T cur) {
             int unstable = 0;
-            const bool sp = step_aeif(P, s, r, a0, a1, &unstable);
+            const bool sp = step_aeif(P, s, r, a0, a1, cur, &unstable);
             if (unstable) atomicOr(a.flags, FLAG_UNSTABLE);
             return sp;
           });
@@ -1296,6 +1315,18 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
           CHK(upload(net, &g, ch.gen_index, (size_t)sp.n));
           dc.gen_index = g;
         }
+      } else if (ch.kind == BS_CH_CURRENT) {
+        if (!(ch.weight == ch.weight)) BAD("current channel value must not be NaN");
+        if (ch.receptor != 0) BAD("current channels feed the current-input seam (receptor 0)");
+        dc.weight = ch.weight;
+        dc.tmin = ch.t_min_step;
+        dc.tmax = ch.t_max_step;
+        if (ch.gen_index) {
+          int* g = nullptr;
+          CHK(upload(net, &g, ch.gen_index, (size_t)sp.n));
+          dc.gen_index = g;
+        }
+        ++dp.n_cur;
       } else {
         BAD("unknown channel kind");
       }
@@ -1570,7 +1601,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     for (size_t q = 0; q < net->pops.size(); ++q) {
       const DevPop& p = net->pops[q];
       best = std::max(best, win_layout(w, hz(w), TS, p.n_cw, p.n_fx, p.gp_count, SP, p.ch_count,
-                                       p.model == BS_AEIF_COND_ALPHA ? 2 : 1, (int)net->tsize, 0,
+                                       (p.model == BS_AEIF_COND_ALPHA ? 2 : 1) + (p.n_cur > 0 ? 1 : 0), (int)net->tsize, 0,
                                        net->pop_pc[q]).end);
     }
     return best;
@@ -1588,7 +1619,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     for (size_t q = 0; q < net->pops.size(); ++q) {
       DevPop& p = net->pops[q];
       const size_t base_end = win_layout(W, net->H, TS, p.n_cw, p.n_fx, p.gp_count, SP, p.ch_count,
-                                         p.model == BS_AEIF_COND_ALPHA ? 2 : 1, (int)net->tsize, 0,
+                                         (p.model == BS_AEIF_COND_ALPHA ? 2 : 1) + (p.n_cur > 0 ? 1 : 0), (int)net->tsize, 0,
                                          net->pop_pc[q]).end;
       size_t off = 0;
       for (int c = p.ch_begin; c < p.ch_begin + p.ch_count; ++c) {
@@ -1604,7 +1635,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
       }
       p.tab_bytes = (long long)off;
       need = std::max(need, win_layout(W, net->H, TS, p.n_cw, p.n_fx, p.gp_count, SP, p.ch_count,
-                                       p.model == BS_AEIF_COND_ALPHA ? 2 : 1, (int)net->tsize,
+                                       (p.model == BS_AEIF_COND_ALPHA ? 2 : 1) + (p.n_cur > 0 ? 1 : 0), (int)net->tsize,
                                        (size_t)off, net->pop_pc[q]).end);
     }
     net->lds_bytes = need;
@@ -1672,6 +1703,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
       T.blob_words = (int)(blob_layout(dp.ch_count, dp.gp_count, (size_t)dp.tab_bytes).end / 16);
       T.gp_begin = dp.gp_begin;
       T.pois = net->pop_pc[T.pop];
+      T.ncur = dp.n_cur > 0 ? 1 : 0;
       for (int g = 0; g < 4; ++g) T.pre_pop[g] = g < dp.gp_count ? gp_sorted[dp.gp_begin + g].pre_pop : 0;
       T.gid0 = dp.gid0;
       T.blob_off = boff[T.pop];

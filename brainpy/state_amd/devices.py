@@ -5,6 +5,9 @@
   independent Poisson train of mean ``rate * dt / 1000`` per step, active iff
   ``t_min_step < step <= t_max_step``. Draws come from the engine's
   counter-based stream (Threefry-2x32 keyed by the derived seed and the step).
+* ``dc_generator`` (``_nest_device/dc_generator.py:31-262``): a constant
+  current (pA) inside the half-open window ``[origin + start, origin + stop)``,
+  injected into the neuron's current-input seam (``simulator.py:1612-1745``).
 * ``spike_recorder`` / ``voltmeter`` / ``multimeter``: tapped by the Simulator
   every step after the update (``simulator.py:1858-1867``).
 """
@@ -16,7 +19,7 @@ import numpy as np
 
 from . import units as u
 
-__all__ = ['poisson_generator', 'spike_recorder', 'voltmeter', 'multimeter']
+__all__ = ['poisson_generator', 'dc_generator', 'spike_recorder', 'voltmeter', 'multimeter']
 
 
 class poisson_generator:
@@ -61,6 +64,39 @@ class poisson_generator:
 
     def lam(self, dt_ms):
         return self.rate * dt_ms / 1000.0
+
+
+class dc_generator:
+    """DC current generator (NEST ``dc_generator``): ``amplitude`` (pA) while
+    ``origin + start <= t < origin + stop`` (``stop=None``: no end), else 0,
+    with ``t`` the step's time ``n * dt``."""
+    __module__ = 'brainpy.state_amd'
+    _is_generator = True
+    _injects_current = True
+
+    def __init__(self, in_size=1, amplitude=0.0, start=0.0, stop=None, origin=0.0, name=None):
+        self.in_size = int(in_size)
+        self.amplitude = u.scalar(amplitude, 'current')
+        self.start = u.scalar(start, 'time')
+        self.stop = None if stop is None else u.scalar(stop, 'time')
+        self.origin = u.scalar(origin, 'time')
+        self.name = name
+
+    def step_window(self, dt_ms):
+        """(n_start, n_stop): active at steps n_start <= n < n_stop, i.e. where
+        ``n * dt >= origin + start`` and ``n * dt < origin + stop``, both
+        compared in float64 as the reference compares ``t`` (dc_generator.py:252-262)."""
+        def first_at_or_after(t):
+            n = max(0, int(math.floor(t / dt_ms)) - 2)
+            while float(n) * dt_ms < t:
+                n += 1
+            return n
+        t_start = self.origin + self.start
+        n_start = first_at_or_after(t_start)
+        if self.stop is None:
+            return n_start, np.iinfo(np.int64).max
+        n_stop = first_at_or_after(self.origin + self.stop)
+        return n_start, max(n_stop, n_start)     # stop <= start: never active
 
 
 class spike_recorder:

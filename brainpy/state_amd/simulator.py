@@ -18,7 +18,7 @@ import numpy as np
 
 from . import units as u
 from . import _lib as L
-from .devices import multimeter, poisson_generator, spike_recorder
+from .devices import dc_generator, multimeter, poisson_generator, spike_recorder
 from .dist import Distribution
 from .models import Neuron, aeif_cond_alpha, iaf_psc_delta
 from .nodeview import NodeView, _Segment, _flat_size
@@ -121,6 +121,19 @@ class PlasticProjection(StaticProjection):
     def __init__(self, *args, rule=None, **kw):
         super().__init__(*args, **kw)
         self.rule = rule
+
+
+class _CurChannel:
+    """A current generator wired into a population's current-input seam
+    (simulator.py:1612-1642): no projection, no delay."""
+
+    def __init__(self, ordinal, gen, post_pop, post_idx, weight):
+        self.ordinal = ordinal
+        self.gen = gen
+        self.post_pop = post_pop
+        self.post_idx = post_idx
+        self.weight = weight            # None: the generator's current as is
+        self.receptor = 0
 
 
 class _GenChannel:
@@ -332,6 +345,22 @@ class Simulator:
         post_pop = post_seg.population
         if post_pop is None or not isinstance(post_pop, Neuron):
             raise TypeError('connect target must be a neuron population')
+        if isinstance(pre_seg, _GenSegment) and getattr(pre_seg.spec.model_cls, '_injects_current', False):
+            if synapse is not None:
+                raise NotImplementedError('a current generator has no synapse')
+            if receptor_type is not None:
+                raise ValueError(f'receptor_type={receptor_type} was given for current injection into '
+                                 f'{type(post_pop).__name__}, which has no current receptors.')
+            post_idx = np.asarray(post_seg.indices, dtype=np.int32)
+            gen = pre_seg.spec.model_cls(len(post_idx), **dict(pre_seg.spec.params))
+            if isinstance(weight, u.Quantity):
+                raise TypeError('a current generator connection weight is a plain scale factor')
+            w = None if weight is None else np.asarray(weight, dtype=np.float64)
+            if w is not None and np.ndim(w) != 0:
+                raise NotImplementedError('current generator connections take a scalar weight')
+            rec = _CurChannel(ordinal, gen, post_pop, post_idx, None if w is None else float(w))
+            self._records.append(rec)
+            return rec
         receptor = self._receptor(post_pop, receptor_type)
         wdim = _weight_dim(post_pop)
         post_idx = np.asarray(post_seg.indices, dtype=np.int32)
@@ -637,7 +666,22 @@ class _NetBuilder:
             for r in per_pop[p]:
                 ch = chans[c]
                 ch.receptor = r.receptor
-                if isinstance(r, _GenChannel):
+                if isinstance(r, _CurChannel):
+                    g = r.gen
+                    ch.kind = L.BS_CH_CURRENT
+                    # device.update() * weight (simulator.py:1738-1740), one product
+                    ch.weight = g.amplitude if r.weight is None else g.amplitude * r.weight
+                    n0, n1 = g.step_window(dt)
+                    ch.t_min_step = int(n0)
+                    ch.t_max_step = int(min(n1, np.iinfo(np.int64).max))
+                    if len(r.post_idx) == pop.n and np.array_equal(r.post_idx, np.arange(pop.n)):
+                        ch.gen_index = None
+                    else:
+                        gi = np.full(pop.n, -1, dtype=np.int32)
+                        gi[r.post_idx] = np.arange(len(r.post_idx), dtype=np.int32)
+                        ch.gen_index = D.arr(gi, C.c_int32)
+                    table_off.append(-1)
+                elif isinstance(r, _GenChannel):
                     g = r.gen
                     ch.kind = L.BS_CH_POISSON
                     ch.weight = r.weight

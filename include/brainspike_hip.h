@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define BS_ABI_VERSION 2
+#define BS_ABI_VERSION 3
 
 /* ---- return codes ------------------------------------------------------ */
 #define BS_OK 0
@@ -58,8 +58,16 @@ enum bs_channel_kind {
                         sum of k copies of the weight (CSR event matmul,
                         projections.py:36-59, with per-edge equal weights)       */
   BS_CH_FIXED = 2,   /* heterogeneous weights: exact 2^-F fixed-point sum       */
-  BS_CH_POISSON = 3  /* one_to_one poisson_generator: value = count * weight
+  BS_CH_POISSON = 3, /* one_to_one poisson_generator: value = count * weight
                         (event_proj.py:311-312, poisson_generator.py:501-576)    */
+  BS_CH_CURRENT = 4  /* dc_generator current (pA) into the neuron's current-input
+                        seam (simulator.py:1612-1745): value = weight (amplitude
+                        times the connection weight) at steps t_min_step <= step
+                        < t_max_step, else 0; summed in list order with the
+                        other current channels, separately from the delta
+                        inputs. iaf_psc_delta adds the sum to I_e in the same
+                        step; iaf_psc_exp / iaf_psc_alpha / aeif_cond_alpha
+                        latch it for the next step (i_0 / y0 / I_stim)          */
 };
 
 /* Parameter block lengths (doubles) per model; layouts are documented in
@@ -92,12 +100,13 @@ typedef struct bs_channel {
   int32_t fixed_shift;     /* FIXED: F, value = sum(q_e) * 2^-F                  */
   /* POISSON only: */
   double lam;              /* rate * dt / 1000                                   */
-  int64_t t_min_step;      /* active iff t_min_step < step <= t_max_step        */
+  int64_t t_min_step;      /* POISSON: active iff t_min_step < step <= t_max_step;
+                              CURRENT: active iff t_min_step <= step < t_max_step */
   int64_t t_max_step;
   uint32_t seed;           /* Simulator._derive_seed(rng_seed, ordinal)          */
   int32_t delay_steps;     /* delay / dt                                         */
   const int32_t* gen_index;/* [n_post_pop] generator element feeding each post,
-                              -1 if none; NULL = identity                        */
+                              -1 if none; NULL = identity (POISSON, CURRENT)     */
 } bs_channel;
 
 /* STDP parameters (stdp_synapse.py:115-179): additive/multiplicative pair rule. */

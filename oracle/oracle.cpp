@@ -237,11 +237,11 @@ extern "C" int orc_create(const bs_net_desc* d, orc_net** out) {
     for (int k = 0; k < sp.n_vm; ++k) net->taps.emplace_back(p, sp.vm_idx[k]);
     for (int c = sp.ch_begin; c < sp.ch_begin + sp.ch_count; ++c) {
       const bs_channel& ch = d->channels[c];
-      if (ch.kind == BS_CH_POISSON) {
+      if (ch.kind == BS_CH_POISSON || ch.kind == BS_CH_CURRENT) {
         std::vector<int> gi(sp.n);
         for (int i = 0; i < sp.n; ++i) gi[i] = ch.gen_index ? ch.gen_index[i] : i;
         net->gen_index[c] = gi;
-        net->ptab[c] = make_table(ch.lam);
+        if (ch.kind == BS_CH_POISSON) net->ptab[c] = make_table(ch.lam);
       }
     }
   }
@@ -306,6 +306,8 @@ extern "C" int orc_create(const bs_net_desc* d, orc_net** out) {
       net->chan_pop[c] = p;
       if (d->channels[c].kind == BS_CH_POISSON)
         net->ghist[c].assign(net->Lh, std::vector<double>(sp.n, 0.0));
+      else if (d->channels[c].kind == BS_CH_CURRENT)
+        continue;  // no arrival ring: the current enters in its own step
       else if (d->channels[c].kind == BS_CH_COUNT)
         net->cring[c].assign(net->Lh, std::vector<double>(sp.n, 0.0));
       else
@@ -321,14 +323,15 @@ extern "C" int orc_destroy(orc_net* net) {
 }
 
 // ----------------------------------------------------------- model updates
-// iaf_psc_delta.update (_nest_neuron/iaf_psc_delta.py:478-521)
+// iaf_psc_delta.update (_nest_neuron/iaf_psc_delta.py:478-521);
+// i_total = sum_current_inputs(I_e + x): (I_e + 0) plus this step's current
+// inputs in registration order (simulator.py:1737-1745).
 template <typename T>
-static bool upd_delta(const std::vector<double>& P, T& V, T& buf, int& r, T delta_v) {
+static bool upd_delta(const std::vector<double>& P, T& V, T& buf, int& r, T delta_v, T i_total) {
   const T E_L = (T)P[0], decay = (T)P[1];
-  const T C_m = (T)P[32], tau_m = (T)P[33], I_e = (T)P[34];
+  const T C_m = (T)P[32], tau_m = (T)P[33];
   const T last_v = V;
   const int ref_steps = r;
-  const T i_total = I_e + (T)0;  // sum_current_inputs(I_e + x), x = 0 pA
   T v_candidate = E_L + (last_v - E_L) * decay + (i_total / C_m) * tau_m * ((T)1 - decay);
   v_candidate = v_candidate + delta_v;
   const bool refractory_input = P[7] != 0.0;
@@ -349,7 +352,7 @@ static bool upd_delta(const std::vector<double>& P, T& V, T& buf, int& r, T delt
 
 // iaf_psc_exp.update (_nest_neuron/iaf_psc_exp.py:604-682), delta = 0
 template <typename T>
-static bool upd_exp(const std::vector<double>& P, T* s, int& r, T w_all) {
+static bool upd_exp(const std::vector<double>& P, T* s, int& r, T w_all, T cur) {
   const T E_L = (T)P[0];
   T V_rel = s[0] - E_L;
   const T theta = (T)P[1];
@@ -373,7 +376,7 @@ static bool upd_exp(const std::vector<double>& P, T* s, int& r, T w_all) {
   s[0] = V_rel + E_L;
   s[1] = i_syn_ex;
   s[2] = i_syn_in;
-  s[3] = (T)0 + (T)0;
+  s[3] = cur + (T)0;  // new i_0 = sum_current_inputs(x) + 0 pA, applied next step
   s[4] = (T)0;
   r = rr;
   return spike_cond;
@@ -381,7 +384,7 @@ static bool upd_exp(const std::vector<double>& P, T* s, int& r, T w_all) {
 
 // iaf_psc_alpha.update (_nest_neuron/iaf_psc_alpha.py:604-684)
 template <typename T>
-static bool upd_alpha(const std::vector<double>& P, T* s, int& r, T w_all) {
+static bool upd_alpha(const std::vector<double>& P, T* s, int& r, T w_all, T cur) {
   const T w_ex = w_all > (T)0 ? w_all : (T)0;
   const T w_in = w_all < (T)0 ? w_all : (T)0;
   const T E_L = (T)P[0];
@@ -408,7 +411,7 @@ static bool upd_alpha(const std::vector<double>& P, T* s, int& r, T w_all) {
   s[2] = I_in;
   s[3] = dI_ex;
   s[4] = dI_in;
-  s[5] = (T)0;
+  s[5] = cur + (T)0;  // y0 for the next step: sum_current_inputs(x) + 0 pA
   r = rr;
   return spike_cond;
 }
@@ -476,7 +479,7 @@ static double nanmax(double a, double b) {
 }
 
 static bool update(const std::vector<double>& P, double* st, int& r, double w_ex, double w_in,
-                   bool* unstable) {
+                   double cur, bool* unstable) {
   double y[NV] = {st[0], st[1], st[2], st[3], st[4], st[5]};
   double h = st[6];
   const double i_stim = st[7];
@@ -546,7 +549,7 @@ static bool update(const std::vector<double>& P, double* st, int& r, double w_ex
   y[3] = y[3] + P[PSCIN] * w_in;
   for (int v = 0; v < NV; ++v) st[v] = y[v];
   st[6] = h;
-  st[7] = 0.0;
+  st[7] = cur + 0.0;  // I_stim for the next step: sum_current_inputs(x) + 0 pA
   return spike_mask;
 }
 }  // namespace aeif
@@ -623,6 +626,13 @@ static int run_t(orc_net* net, int64_t n_steps, double* vm_out) {
         if (src < 0) continue;
         const std::vector<double>& x = net->ghist[c][(int)(src % Lh)];
         for (int j = 0; j < npost; ++j) yin[c][j] = (T)x[j] * (T)ch.weight;
+      } else if (ch.kind == BS_CH_CURRENT) {
+        // phase 1b: dc_generator current, amplitude * weight inside its window
+        // [t_min_step, t_max_step), scattered onto its targets
+        // (dc_generator.py:219-262, simulator.py:1737-1745)
+        const bool active = ch.t_min_step <= n && n < ch.t_max_step;
+        const std::vector<int>& gi = net->gen_index[c];
+        for (int j = 0; j < npost; ++j) yin[c][j] = (active && gi[j] >= 0) ? (T)ch.weight : (T)0;
       }
     }
     // Static projections, in the kernel's DelayScatter form (delay_scatter.py:
@@ -687,23 +697,29 @@ static int run_t(orc_net* net, int64_t n_steps, double* vm_out) {
       std::vector<double>& H = net->hist[p][hs];
       for (int j = 0; j < sp.n; ++j) {
         T acc[2] = {(T)0, (T)0};  // sum_delta_inputs(zeros) per receptor
-        for (int c = sp.ch_begin; c < sp.ch_begin + sp.ch_count; ++c)
-          acc[net->chans[c].receptor] = acc[net->chans[c].receptor] + yin[c][j];
+        // sum_current_inputs(init): I_e + x for iaf_psc_delta, x for the others (x = 0)
+        T cur = sp.model == BS_IAF_PSC_DELTA ? (T)P[34] + (T)0 : (T)0;
+        for (int c = sp.ch_begin; c < sp.ch_begin + sp.ch_count; ++c) {
+          if (net->chans[c].kind == BS_CH_CURRENT)
+            cur = cur + yin[c][j];
+          else
+            acc[net->chans[c].receptor] = acc[net->chans[c].receptor] + yin[c][j];
+        }
         T s[BS_NSTATE];
         for (int k = 0; k < BS_NSTATE; ++k) s[k] = (T)S[(size_t)k * sp.n + j];
         int& r = net->ref[p][j];
         bool spk = false;
         if (sp.model == BS_IAF_PSC_DELTA) {
-          spk = upd_delta<T>(P, s[0], s[1], r, acc[0]);
+          spk = upd_delta<T>(P, s[0], s[1], r, acc[0], cur);
         } else if (sp.model == BS_IAF_PSC_EXP) {
-          spk = upd_exp<T>(P, s, r, acc[0]);
+          spk = upd_exp<T>(P, s, r, acc[0], cur);
         } else if (sp.model == BS_IAF_PSC_ALPHA) {
-          spk = upd_alpha<T>(P, s, r, acc[0]);
+          spk = upd_alpha<T>(P, s, r, acc[0], cur);
         } else {
           double sd[BS_NSTATE];
           for (int k = 0; k < BS_NSTATE; ++k) sd[k] = (double)s[k];
           bool unstable = false;
-          spk = aeif::update(P, sd, r, (double)acc[0], (double)acc[1], &unstable);
+          spk = aeif::update(P, sd, r, (double)acc[0], (double)acc[1], (double)cur, &unstable);
           if (unstable) return fail(BS_ENUMERIC, "Numerical instability in aeif_cond_alpha dynamics.");
           for (int k = 0; k < BS_NSTATE; ++k) s[k] = (T)sd[k];
         }
@@ -775,20 +791,20 @@ extern "C" int orc_neuron_step(int model, const double* params, double* state, i
     float s[BS_NSTATE];
     for (int k = 0; k < BS_NSTATE; ++k) s[k] = (float)state[k];
     bool spk = false;
-    if (model == BS_IAF_PSC_DELTA) spk = upd_delta<float>(P, s[0], s[1], *ref, (float)in0);
-    else if (model == BS_IAF_PSC_EXP) spk = upd_exp<float>(P, s, *ref, (float)in0);
-    else if (model == BS_IAF_PSC_ALPHA) spk = upd_alpha<float>(P, s, *ref, (float)in0);
+    if (model == BS_IAF_PSC_DELTA) spk = upd_delta<float>(P, s[0], s[1], *ref, (float)in0, (float)P[34] + 0.0f);
+    else if (model == BS_IAF_PSC_EXP) spk = upd_exp<float>(P, s, *ref, (float)in0, 0.0f);
+    else if (model == BS_IAF_PSC_ALPHA) spk = upd_alpha<float>(P, s, *ref, (float)in0, 0.0f);
     else return -1;
     for (int k = 0; k < BS_NSTATE; ++k) state[k] = s[k];
     return spk ? 1 : 0;
   }
   bool spk = false;
-  if (model == BS_IAF_PSC_DELTA) spk = upd_delta<double>(P, state[0], state[1], *ref, in0);
-  else if (model == BS_IAF_PSC_EXP) spk = upd_exp<double>(P, state, *ref, in0);
-  else if (model == BS_IAF_PSC_ALPHA) spk = upd_alpha<double>(P, state, *ref, in0);
+  if (model == BS_IAF_PSC_DELTA) spk = upd_delta<double>(P, state[0], state[1], *ref, in0, P[34] + 0.0);
+  else if (model == BS_IAF_PSC_EXP) spk = upd_exp<double>(P, state, *ref, in0, 0.0);
+  else if (model == BS_IAF_PSC_ALPHA) spk = upd_alpha<double>(P, state, *ref, in0, 0.0);
   else if (model == BS_AEIF_COND_ALPHA) {
     bool unstable = false;
-    spk = aeif::update(P, state, *ref, in0, in1, &unstable);
+    spk = aeif::update(P, state, *ref, in0, in1, 0.0, &unstable);
     if (unstable) return -5;
   } else {
     return -1;
