@@ -16,7 +16,8 @@
 //     exact and order independent); inputs (per step and lane, the channels in
 //     the reference's summation order: S[k] tables for homogeneous weights,
 //     exact 2^-F fixed-point sums for per-edge weights, count * weight for
-//     Poisson drive drawn from a counter-based Threefry stream); update (the
+//     Poisson drive drawn from a counter-based Threefry stream; dc_generator
+//     currents summed apart, for the models' current-input seam); update (the
 //     model step for W steps with the state in registers); emit (per-step spike
 //     buckets for the following windows, the spike record, voltmeter taps).
 //   * When the tiles leave CUs idle, the launch carries extra producer blocks

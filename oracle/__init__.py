@@ -7,7 +7,7 @@ with g++ -ffp-contract=off) and runs the reference's algorithm on one CPU
 thread from the same ``bs_net_desc`` the engine consumes.
 
 Pinning of each restated function (tests/test_oracle_kat.py,
-tests/test_stdp_oracle.py, tests/test_bptt_oracle.py):
+tests/test_stdp_oracle.py, tests/test_bptt_oracle.py, tests/test_dc_generator.py):
 
 * threefry2x32 -- Random123 published known-answer vectors;
 * exp / log / pow -- agreement with libm to about one ulp;
@@ -21,6 +21,10 @@ tests/test_stdp_oracle.py, tests/test_bptt_oracle.py):
   aeif_cond_alpha_test.py:40-200, and the one-step current known answer;
 * delivery -- latency s + 1 + d, event conservation, count tables equal to
   sequential sums;
+* current inputs (dc_generator) -- the reference's dc_generator tests: the
+  same trace as I_e, the closed-form iaf_psc_delta recurrence inside
+  start/stop/origin windows, steady state, additivity, and the one-step latch
+  of iaf_psc_exp / iaf_psc_alpha (the I_e trace one step later);
 * stdp_synapse -- stdp_synapse_rule_test.py:65-131 known answers and a
   closed-form two-neuron trajectory;
 * bptt_ref -- DC / refractory / PSC closed forms for the forward, central
