@@ -61,11 +61,13 @@ __device__ __forceinline__ DeltaP<T> load_delta(const double* P) {
   p.P = P;
   return p;
 }
-template <typename T>
-__device__ __forceinline__ bool step_delta(const DeltaP<T>& p, T* s, int& r, T din, T i_total,
-                                           bool has_cur) {
+// HAS_CUR: the population has current-input channels (a compile-time branch,
+// so the common case carries no per-step division).
+template <typename T, bool HAS_CUR>
+__device__ __forceinline__ bool step_delta(const DeltaP<T>& p, T* s, int& r, T din, T i_total) {
   const T last_v = s[0];
-  const T cI = has_cur ? (i_total / p.Cm) * p.taum * p.one_m_decay : p.cI;
+  T cI = p.cI;
+  if constexpr (HAS_CUR) cI = (i_total / p.Cm) * p.taum * p.one_m_decay;
   T v = p.EL + (last_v - p.EL) * p.decay + cI;
   v = v + din;
   if (p.refin) v = v + s[1];

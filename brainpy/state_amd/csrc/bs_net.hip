@@ -117,8 +117,9 @@ struct Tile {
   int nrec, blob_words, gp_begin, pois;  // pois: the population has a drawn-ahead channel
   int ncur, pad2;    // ncur: 1 if the population has current-input channels
   int pre_pop[4];   // pre populations of the first four gather sources
-  long long gid0, blob_off, tab_bytes;
+  long long gid0, blob_off, tab_bytes;  // 96 bytes: records stay 16-byte aligned
 };
+static_assert(sizeof(Tile) % 16 == 0, "tile records are read as 16-byte words");
 
 // An incoming static projection of a post population. Its edges are grouped
 // by (pre neuron, post tile): seg[i * (nt + 1) + t] .. seg[i * (nt + 1) + t + 1]
@@ -349,8 +350,9 @@ __device__ __forceinline__ T chan_value(const UpdCtx<T>& u, const ChReg<T>& c, i
 }
 
 // The W-step loop of one lane: inputs summed in channel order, model step,
-// voltmeter tap, spike buffered in LDS.
-template <typename T, typename F>
+// voltmeter tap, spike buffered in LDS. HAS_CUR: the population has current
+// inputs (row nrec of inp); without them the model sees a zero current.
+template <typename T, bool HAS_CUR, typename F>
 __device__ __forceinline__ void update_loop(const UpdCtx<T>& u, const ChReg<T>* cr, int ns, F stepf) {
   const WinArgs& a = u.a;
   T* st = (T*)a.state;
@@ -366,7 +368,8 @@ __device__ __forceinline__ void update_loop(const UpdCtx<T>& u, const ChReg<T>* 
     const int step = a.n0 + w;
     const T acc0 = u.inp[w * u.TS + u.tid];
     const T acc1 = u.nrec > 1 ? u.inp[(u.Wm + w) * u.TS + u.tid] : (T)0;
-    const T cur = u.ncur ? u.inp[(u.nrec * u.Wm + w) * u.TS + u.tid] : (T)0;
+    T cur = (T)0;
+    if constexpr (HAS_CUR) cur = u.inp[(u.nrec * u.Wm + w) * u.TS + u.tid];
     const bool spike = stepf(s, r, acc0, acc1, cur);
     if (vmc >= 0) a.vm_buf[(long long)(step - a.vm_step0) * a.n_vm + vmc] = (double)s[0];
     if (a.spk_hist) a.spk_hist[(long long)(step % a.Ls) * a.Ntot + u.gid] = spike ? 1 : 0;
@@ -814,33 +817,42 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
     ChReg<T> cr[4];
     UpdCtx<T> u{a, cnt, fxa, spl, spn, chs, tab, tables, TS, ncw, nfx, nch, tl.start, tid,
                 tid < tl.count, (long long)gpop.gid0 + tl.start + tid, inp, nrec, Wm, pc, tl.ncur};
-    switch (gpop.model) {
-      case BS_IAF_PSC_DELTA: {
-        const DeltaP<T> mp = load_delta<T>(P);
-        update_loop<T>(u, cr, gpop.n_state,
-                       [&](T* s, int& r, T a0, T, T cur) { return step_delta<T>(mp, s, r, a0, cur, tl.ncur != 0); });
-      } break;
-      case BS_IAF_PSC_EXP: {
-        const ExpP<T> mp = load_exp<T>(P);
-        update_loop<T>(u, cr, gpop.n_state,
-                       [&](T* s, int& r, T a0, T, T cur) { return step_exp<T>(mp, s, r, a0, cur); });
-      } break;
-      case BS_IAF_PSC_ALPHA: {
-        const AlphaP<T> mp = load_alpha<T>(P);
-        update_loop<T>(u, cr, gpop.n_state,
-                       [&](T* s, int& r, T a0, T, T cur) { return step_alpha<T>(mp, s, r, a0, cur); });
-      } break;
-      case BS_AEIF_COND_ALPHA: {
-        if constexpr (std::is_same<T, double>::value) {
-          update_loop<T>(u, cr, gpop.n_state, [&](T* s, int& r, T a0, T a1, T cur) {
-            int unstable = 0;
-            const bool sp = step_aeif(P, s, r, a0, a1, cur, &unstable);
-            if (unstable) atomicOr(a.flags, FLAG_UNSTABLE);
-            return sp;
-          });
-        }
-      } break;
+    // each model with and without current inputs (a compile-time branch)
+#define BS_UPDATE(CUR)                                                                              \
+    switch (gpop.model) {                                                                          \
+      case BS_IAF_PSC_DELTA: {                                                                     \
+        const DeltaP<T> mp = load_delta<T>(P);                                                     \
+        update_loop<T, CUR>(u, cr, gpop.n_state, [&](T* s, int& r, T a0, T, T cur) {               \
+          return step_delta<T, CUR>(mp, s, r, a0, cur);                                            \
+        });                                                                                        \
+      } break;                                                                                     \
+      case BS_IAF_PSC_EXP: {                                                                       \
+        const ExpP<T> mp = load_exp<T>(P);                                                         \
+        update_loop<T, CUR>(u, cr, gpop.n_state,                                                   \
+                            [&](T* s, int& r, T a0, T, T cur) { return step_exp<T>(mp, s, r, a0, cur); }); \
+      } break;                                                                                     \
+      case BS_IAF_PSC_ALPHA: {                                                                     \
+        const AlphaP<T> mp = load_alpha<T>(P);                                                     \
+        update_loop<T, CUR>(u, cr, gpop.n_state,                                                   \
+                            [&](T* s, int& r, T a0, T, T cur) { return step_alpha<T>(mp, s, r, a0, cur); }); \
+      } break;                                                                                     \
+      case BS_AEIF_COND_ALPHA: {                                                                   \
+        if constexpr (std::is_same<T, double>::value) {                                            \
+          update_loop<T, CUR>(u, cr, gpop.n_state, [&](T* s, int& r, T a0, T a1, T cur) {           \
+            int unstable = 0;                                                                      \
+            const bool sp = step_aeif(P, s, r, a0, a1, cur, &unstable);                            \
+            if (unstable) atomicOr(a.flags, FLAG_UNSTABLE);                                        \
+            return sp;                                                                             \
+          });                                                                                      \
+        }                                                                                          \
+      } break;                                                                                     \
     }
+    if (tl.ncur) {
+      BS_UPDATE(true)
+    } else {
+      BS_UPDATE(false)
+    }
+#undef BS_UPDATE
   }
   __syncthreads();
 
