@@ -353,7 +353,7 @@ __device__ __forceinline__ T chan_value(const UpdCtx<T>& u, const ChReg<T>& c, i
 // voltmeter tap, spike buffered in LDS. HAS_CUR: the population has current
 // inputs (row nrec of inp); without them the model sees a zero current.
 template <typename T, bool HAS_CUR, typename F>
-__device__ __forceinline__ void update_loop(const UpdCtx<T>& u, const ChReg<T>* cr, int ns, F stepf) {
+__device__ __forceinline__ void update_loop(const UpdCtx<T>& u, int ns, F stepf) {
   const WinArgs& a = u.a;
   T* st = (T*)a.state;
   T s[BS_NSTATE];
@@ -814,7 +814,6 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   // ---- 3. update: the W-step loop of each neuron
   if (tid < TS) {
     const T* tables = (const T*)a.tables;
-    ChReg<T> cr[4];
     UpdCtx<T> u{a, cnt, fxa, spl, spn, chs, tab, tables, TS, ncw, nfx, nch, tl.start, tid,
                 tid < tl.count, (long long)gpop.gid0 + tl.start + tid, inp, nrec, Wm, pc, tl.ncur};
     // each model with and without current inputs (a compile-time branch)
@@ -822,23 +821,23 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
     switch (gpop.model) {                                                                          \
       case BS_IAF_PSC_DELTA: {                                                                     \
         const DeltaP<T> mp = load_delta<T>(P);                                                     \
-        update_loop<T, CUR>(u, cr, gpop.n_state, [&](T* s, int& r, T a0, T, T cur) {               \
+        update_loop<T, CUR>(u, gpop.n_state, [&](T* s, int& r, T a0, T, T cur) {               \
           return step_delta<T, CUR>(mp, s, r, a0, cur);                                            \
         });                                                                                        \
       } break;                                                                                     \
       case BS_IAF_PSC_EXP: {                                                                       \
         const ExpP<T> mp = load_exp<T>(P);                                                         \
-        update_loop<T, CUR>(u, cr, gpop.n_state,                                                   \
+        update_loop<T, CUR>(u, gpop.n_state,                                                   \
                             [&](T* s, int& r, T a0, T, T cur) { return step_exp<T>(mp, s, r, a0, cur); }); \
       } break;                                                                                     \
       case BS_IAF_PSC_ALPHA: {                                                                     \
         const AlphaP<T> mp = load_alpha<T>(P);                                                     \
-        update_loop<T, CUR>(u, cr, gpop.n_state,                                                   \
+        update_loop<T, CUR>(u, gpop.n_state,                                                   \
                             [&](T* s, int& r, T a0, T, T cur) { return step_alpha<T>(mp, s, r, a0, cur); }); \
       } break;                                                                                     \
       case BS_AEIF_COND_ALPHA: {                                                                   \
         if constexpr (std::is_same<T, double>::value) {                                            \
-          update_loop<T, CUR>(u, cr, gpop.n_state, [&](T* s, int& r, T a0, T a1, T cur) {           \
+          update_loop<T, CUR>(u, gpop.n_state, [&](T* s, int& r, T a0, T a1, T cur) {           \
             int unstable = 0;                                                                      \
             const bool sp = step_aeif(P, s, r, a0, a1, cur, &unstable);                            \
             if (unstable) atomicOr(a.flags, FLAG_UNSTABLE);                                        \

@@ -234,3 +234,26 @@ def test_brunel_11250_fp32_statistics_within_distributional_band():
     assert rep['rate_rel_dev'] < 0.05 and rep['cv_isi_rel_dev'] < 0.05, rep
     assert rep['first_divergence_step'] is not None and rep['first_divergence_step'] > 10
     assert rep['vm_max_abs_err_mV_before_divergence'] < 1e-3, rep
+
+
+def test_hub_population_with_five_outgoing_projections():
+    """One population feeds five others (five gather sources reading the same
+    spike buckets, different delays) and receives from all of them."""
+    from brainpy.state_amd import (Simulator, fixed_indegree, iaf_psc_delta, poisson_generator,
+                                   spike_recorder)
+    from brainpy.state_amd.init import Uniform
+    sim = Simulator(dt=0.1 * u.ms)
+    kw = dict(E_L=0., V_th=20., V_reset=10., tau_m=20., t_ref=2., V_initializer=Uniform(0., 19., seed=3))
+    hub = sim.create(iaf_psc_delta, 400, **kw)
+    posts = [sim.create(iaf_psc_delta, 90 + 30 * k, **kw) for k in range(5)]
+    noise = sim.create(poisson_generator, rate=20000. * u.Hz)
+    sim.connect(noise, hub, weight=0.1, delay=1.5)
+    for k, p in enumerate(posts):
+        sim.connect(noise, p, weight=0.1, delay=1.5)
+        sim.connect(hub, p, weight=0.1, delay=1.0 + 0.5 * k, rule=fixed_indegree(40), seed=10 + k)
+        sim.connect(p, hub, weight=-0.4, delay=1.2, rule=fixed_indegree(5), seed=20 + k)
+    for pop in [hub] + posts:
+        sim.connect(pop, sim.create(spike_recorder))
+    gpu, orc = _both(sim, 2000)
+    assert len(orc[1]) > 500
+    _assert_same(gpu, orc)
