@@ -39,9 +39,11 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -1364,6 +1366,85 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   std::vector<std::vector<int>> pop_gp(d->n_pops);
   std::vector<int> chan_proj(d->n_channels, -1);
   int dmin = 1 << 30, dmax = 0;
+  // Static projections: the per-(pre, tile) segments are a counting sort of
+  // each projection's edges, built for all projections in parallel on host
+  // threads (the order inside a segment is the edge order, as before).
+  struct StaticSeg {
+    std::string err;
+    int pdmin = 0, pdmax = 0, nt = 0;
+    std::vector<uint32_t> seg, ew;
+    std::vector<long long> eq;
+  };
+  std::vector<StaticSeg> sseg(d->n_projs);
+  std::vector<long long> gid0_of(d->n_pops);
+  for (int p = 0; p < d->n_pops; ++p) gid0_of[p] = net->pops[p].gid0;
+  {
+    auto build_one = [&](int q) {
+      const bs_projection& pj = d->projs[q];
+      StaticSeg& S = sseg[q];
+      if (pj.plastic || pj.pre_pop < 0 || pj.pre_pop >= d->n_pops || pj.post_pop < 0 ||
+          pj.post_pop >= d->n_pops || pj.channel < 0 || pj.channel >= d->n_channels)
+        return;  // plastic, or rejected with its message by the serial pass
+      const int kind = d->channels[pj.channel].kind;
+      if ((kind != BS_CH_COUNT && kind != BS_CH_FIXED) || (kind == BS_CH_FIXED && !pj.weight) ||
+          pj.n_edges < 0 || pj.n_edges >= (1ll << 32))
+        return;
+      const int npre = d->pops[pj.pre_pop].n, npost = d->pops[pj.post_pop].n;
+      const int nt = (npost + TS - 1) / TS;
+      const long long g0 = gid0_of[pj.pre_pop];
+      int pdmin = 1 << 30, pdmax = 0;
+      std::vector<uint32_t> cnt((size_t)npre * (nt + 1) + 1, 0);
+      for (long long e = 0; e < pj.n_edges; ++e) {
+        const int a = pj.pre[e], b = pj.post[e];
+        if (a < 0 || a >= npre || b < 0 || b >= npost) {
+          S.err = "edge index out of range";
+          return;
+        }
+        const int dd = pj.delay ? pj.delay[e] : pj.delay_steps;
+        if (dd < 0 || dd >= (1 << 24)) {
+          S.err = "delay must be in [0, 2^24) steps";
+          return;
+        }
+        pdmin = std::min(pdmin, dd);
+        pdmax = std::max(pdmax, dd);
+        cnt[(size_t)a * (nt + 1) + b / TS + 1]++;
+        __atomic_fetch_add(&net->outdeg[g0 + a], 1ll, __ATOMIC_RELAXED);
+      }
+      if (pj.n_edges == 0) pdmin = pdmax = 0;
+      // exclusive prefix over (pre, tile) in row-major order; entry (a, nt) is the row end
+      S.seg.assign((size_t)npre * (nt + 1), 0);
+      uint32_t acc = 0;
+      for (int a = 0; a < npre; ++a) {
+        for (int t = 0; t < nt; ++t) {
+          S.seg[(size_t)a * (nt + 1) + t] = acc;
+          acc += cnt[(size_t)a * (nt + 1) + t + 1];
+        }
+        S.seg[(size_t)a * (nt + 1) + nt] = acc;
+      }
+      std::vector<uint32_t> pos(S.seg);
+      S.ew.assign(pj.n_edges + 4, 0u);  // the gather reads whole 16-byte chunks past a segment's end
+      if (kind == BS_CH_FIXED) S.eq.assign(pj.n_edges, 0);
+      const int F = d->channels[pj.channel].fixed_shift;
+      for (long long e = 0; e < pj.n_edges; ++e) {
+        const int a = pj.pre[e], b = pj.post[e];
+        const int dd = pj.delay ? pj.delay[e] : pj.delay_steps;
+        const uint32_t k = pos[(size_t)a * (nt + 1) + b / TS]++;
+        S.ew[k] = (uint32_t)(b % TS) | ((uint32_t)dd << 8);
+        if (kind == BS_CH_FIXED) S.eq[k] = (long long)std::llrint(std::ldexp(pj.weight[e], F));
+      }
+      S.pdmin = pdmin;
+      S.pdmax = pdmax;
+      S.nt = nt;
+    };
+    const int nthr = (int)std::max(1u, std::min<unsigned>(16u, std::thread::hardware_concurrency()));
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < std::min(nthr, d->n_projs); ++t)
+      pool.emplace_back([&] {
+        for (int q = next++; q < d->n_projs; q = next++) build_one(q);
+      });
+    for (std::thread& th : pool) th.join();
+  }
   for (int q = 0; q < d->n_projs; ++q) {
     const bs_projection& pj = d->projs[q];
     if (pj.pre_pop < 0 || pj.pre_pop >= d->n_pops || pj.post_pop < 0 || pj.post_pop >= d->n_pops)
@@ -1488,45 +1569,9 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     if (kind != BS_CH_COUNT && kind != BS_CH_FIXED) BAD("projection channel must be COUNT or FIXED");
     if (kind == BS_CH_FIXED && !pj.weight) BAD("FIXED projection needs per-edge weights");
     if (pj.n_edges < 0 || pj.n_edges >= (1ll << 32)) BAD("projection edge count out of range");
-    const int npre = d->pops[pj.pre_pop].n, npost = d->pops[pj.post_pop].n;
-    const DevPop& post = net->pops[pj.post_pop];
-    const int nt = (npost + TS - 1) / TS;
-    int pdmin = 1 << 30, pdmax = 0;
-    std::vector<uint32_t> cnt((size_t)npre * (nt + 1) + 1, 0);
-    for (long long e = 0; e < pj.n_edges; ++e) {
-      const int a = pj.pre[e], b = pj.post[e];
-      if (a < 0 || a >= npre || b < 0 || b >= npost) BAD("edge index out of range");
-      const int dd = pj.delay ? pj.delay[e] : pj.delay_steps;
-      if (dd < 0 || dd >= (1 << 24)) BAD("delay must be in [0, 2^24) steps");
-      pdmin = std::min(pdmin, dd);
-      pdmax = std::max(pdmax, dd);
-      cnt[(size_t)a * (nt + 1) + b / TS + 1]++;
-      net->outdeg[net->pops[pj.pre_pop].gid0 + a]++;
-    }
-    if (pj.n_edges == 0) pdmin = pdmax = 0;
-    // exclusive prefix over (pre, tile) in row-major order; entry (a, nt) is the row end
-    std::vector<uint32_t> seg((size_t)npre * (nt + 1), 0);
-    {
-      uint32_t acc = 0;
-      for (int a = 0; a < npre; ++a) {
-        for (int t = 0; t < nt; ++t) {
-          seg[(size_t)a * (nt + 1) + t] = acc;
-          acc += cnt[(size_t)a * (nt + 1) + t + 1];
-        }
-        seg[(size_t)a * (nt + 1) + nt] = acc;
-      }
-    }
-    std::vector<uint32_t> pos(seg);
-    std::vector<uint32_t> ew(pj.n_edges);
-    std::vector<long long> eq(kind == BS_CH_FIXED ? pj.n_edges : 0);
-    const int F = d->channels[pj.channel].fixed_shift;
-    for (long long e = 0; e < pj.n_edges; ++e) {
-      const int a = pj.pre[e], b = pj.post[e];
-      const int dd = pj.delay ? pj.delay[e] : pj.delay_steps;
-      const uint32_t k = pos[(size_t)a * (nt + 1) + b / TS]++;
-      ew[k] = (uint32_t)(b % TS) | ((uint32_t)dd << 8);
-      if (kind == BS_CH_FIXED) eq[k] = (long long)std::llrint(std::ldexp(pj.weight[e], F));
-    }
+    StaticSeg& S = sseg[q];
+    if (!S.err.empty()) BAD(S.err);
+    const int nt = S.nt, pdmin = S.pdmin, pdmax = S.pdmax;
     GProj G;
     memset(&G, 0, sizeof(G));
     G.pre_pop = pj.pre_pop;
@@ -1536,10 +1581,12 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     G.nt = nt;
     uint32_t *dseg = nullptr, *dew = nullptr;
     long long* deq = nullptr;
-    CHK(upload(net, &dseg, seg.data(), seg.size()));
-    ew.resize(ew.size() + 4, 0u);  // the gather reads whole 16-byte chunks past a segment's end
-    CHK(upload(net, &dew, ew.data(), ew.size()));
-    if (kind == BS_CH_FIXED) CHK(upload(net, &deq, eq.data(), eq.size()));
+    CHK(upload(net, &dseg, S.seg.data(), S.seg.size()));
+    CHK(upload(net, &dew, S.ew.data(), S.ew.size()));
+    if (kind == BS_CH_FIXED) CHK(upload(net, &deq, S.eq.data(), S.eq.size()));
+    std::vector<uint32_t>().swap(S.seg);  // host copies no longer needed
+    std::vector<uint32_t>().swap(S.ew);
+    std::vector<long long>().swap(S.eq);
     G.seg = dseg;
     G.edges = dew;
     G.q = deq;
@@ -1549,7 +1596,6 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
       dmin = std::min(dmin, pdmin);
       dmax = std::max(dmax, pdmax);
     }
-    (void)post;
   }
   if (dmin > dmax) dmin = dmax = 0;
   for (int c = 0; c < d->n_channels; ++c) {
