@@ -55,6 +55,16 @@ def _weight_dim(pop):
     return 'current'
 
 
+def _map_local(sel, seg_idx):
+    """Population-local indices of segment-local ones (``sel[seg_idx]``, int32);
+    a segment that is the whole population in order maps to itself."""
+    seg_idx = np.asarray(seg_idx)
+    n = sel.shape[0]
+    if n and sel[0] == 0 and sel[-1] == n - 1 and (n < 3 or np.array_equal(sel, np.arange(n))):
+        return np.asarray(seg_idx, dtype=np.int32)
+    return sel[np.asarray(seg_idx, dtype=np.int64)].astype(np.int32, copy=False)
+
+
 def delay_to_steps(delay, dt):
     """Delay (ms, scalar or array) -> integer steps; delay/dt must be integral.
 
@@ -394,8 +404,8 @@ class Simulator:
                 raise TypeError(f'rule must be a connection rule, got {rule!r}')
             spec = rule.sample(n_pre, n_post, seed=dseed, pre_is_post=(pre_pop is post_pop),
                                allow_autapses=allow_autapses, allow_multapses=allow_multapses)
-        pre_l = pre_sel[np.asarray(spec.pre_idx, dtype=np.int64)]
-        post_l = post_idx[np.asarray(spec.post_idx, dtype=np.int64)]
+        pre_l = _map_local(pre_sel, spec.pre_idx)
+        post_l = _map_local(post_idx, spec.post_idx)
         E = int(spec.n_edges)
         # weights (resolve_param, connectivity.py:385-405)
         if synapse is not None:
@@ -728,8 +738,8 @@ class _NetBuilder:
                     pj.channel = c
                     pj.plastic = 1 if r.plastic else 0
                     pj.n_edges = E
-                    pj.pre = D.arr(r.pre_idx.astype(np.int32), C.c_int32)
-                    pj.post = D.arr(r.post_idx.astype(np.int32), C.c_int32)
+                    pj.pre = D.arr(np.asarray(r.pre_idx, dtype=np.int32), C.c_int32)
+                    pj.post = D.arr(np.asarray(r.post_idx, dtype=np.int32), C.c_int32)
                     if np.ndim(r.delay) == 0:
                         pj.delay = None
                         pj.delay_steps = int(r.delay)
