@@ -254,7 +254,7 @@ def run_ensemble(args):
         f'built in {time.time() - t0:.1f}s')
     rng = np.random.default_rng(1)
     sample = sorted(rng.choice(M, size=min(16, M), replace=False).tolist())
-    best, table = Ensemble.autotune([sims[k] for k in sample], steps=300)
+    best, table = Ensemble.autotune([sims[k] for k in sample])
     log('autotune (tile_size, max_window) -> ms/step: ' +
         ', '.join(f'{k}: {v * 1e3:.3f}' for k, v in sorted(table.items())))
     t1 = time.time()

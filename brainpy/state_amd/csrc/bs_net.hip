@@ -25,7 +25,10 @@
 //     of each population) into a two-window ring; the tiles then read counts
 //     in the stage phase instead of drawing them in the inputs phase.
 //   * Full chunks of windows are captured once into a HIP graph and replayed;
-//     the kernels read the chunk's first step from device memory.
+//     the kernels read the chunk's first step from device memory. The spike
+//     records of a chunk go to one of two buffers: while the device runs
+//     chunk c, a second stream sorts chunk c - 1's records and the host
+//     decodes them from pinned memory.
 //   * Plastic (stdp_synapse) projections force W = 1 and add two kernels per
 //     step: k_stdp_traces (pre-trace history and post traces) and
 //     k_stdp_events (facilitation of the incoming edges of neurons that fired
@@ -1034,13 +1037,28 @@ struct bs_net {
   int* d_ref = nullptr;
   uint32_t* d_sb = nullptr;
   uint32_t* d_sb_count = nullptr;
-  unsigned long long* d_rec = nullptr;      // record keys of the chunk, emit order
+  // Spike records are double-buffered: chunk c writes buffer c & 1 while the
+  // host sorts, copies and decodes chunk c - 1 (second stream, pinned memory).
+  unsigned long long* d_rec[2] = {nullptr, nullptr};  // record keys of a chunk, emit order
   unsigned long long* d_rec_sorted = nullptr;
   void* d_sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   int gid_bits = 1, step_bits = 1;
-  unsigned long long* d_rec_count = nullptr;
+  unsigned long long* d_rec_count = nullptr;  // [2], one per record buffer
   long long rec_cap = 0;
+  int rec_par = 0;                             // record buffer the next launches write
+  hipStream_t stream2 = nullptr;               // record sorts and their copies
+  hipEvent_t ev_chunk[2] = {nullptr, nullptr};
+  // pinned host memory of the run loop: per-buffer count, flags, chunk base
+  // step and previous window length, then the sorted records and V_m rows
+  unsigned long long* h_cnt = nullptr;
+  int* h_flags = nullptr;
+  long long* h_base_p = nullptr;
+  int* h_prev_p = nullptr;
+  unsigned long long* h_rec = nullptr;
+  size_t h_rec_cap = 0;
+  double* h_vm_p = nullptr;                    // [2][vm_cap_steps * n_vm_dev]
+  size_t h_vm_cap = 0;
   int* d_flags = nullptr;
   int* d_vm_col = nullptr;
   int n_vm_dev = 0;
@@ -1054,7 +1072,6 @@ struct bs_net {
   std::vector<std::vector<double>> w0;        // initial weights, (pre, delay) order
   std::vector<std::vector<long long>> w_ref;  // reference CSR position -> stored position
   long long* d_base = nullptr;     // first step of the chunk a graph replay advances
-  long long h_base = 0;
   // arrival ring (events landing after their delivery window), per tile slice
   uint32_t* d_ring_cnt = nullptr;
   long long* d_ring_fx = nullptr;
@@ -1065,9 +1082,8 @@ struct bs_net {
   int Lr = 1, NCW = 1, NFX = 0;
   int H = 1;                       // arrival offsets held in LDS by k_window
   int* d_prevlen = nullptr;        // graph replays: length of the window before the chunk
-  int h_prevlen = 0;
   int last_win = 0;                // length of the last window run (0 after reset)
-  hipGraphExec_t graph = nullptr;  // one full chunk of window launches
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // one full chunk of window launches per record buffer
   // Poisson producer blocks (when the tiles leave CUs idle)
   std::vector<int> pop_pc;         // population has a drawn-ahead Poisson channel
   std::vector<PJob> pjobs;
@@ -1083,7 +1099,15 @@ struct bs_net {
   std::vector<int> gid_pop;       // population of each gid
   std::vector<int> vm_tap_col;    // per tap (in caller order) -> device column
   long long step = 0;
-  std::vector<int32_t> out_gid, out_step;
+  // decoded spikes of the last run, one block per chunk; blocks keep their
+  // capacity across runs so a steady run decodes into memory already touched
+  struct SpikeBlock {
+    std::vector<int32_t> gid, step;
+    size_t n = 0;
+  };
+  std::vector<SpikeBlock> out_blocks;
+  size_t n_out_blocks = 0;
+  long long n_out = 0;
   // timing / accounting of the last run
   bool timing = false;
   double t_deliver = 0, t_update = 0, t_total = 0;
@@ -1107,7 +1131,8 @@ static void free_net(bs_net* net) {
     if (p) (void)hipFree(p);
   };
   F(net->d_tiles); F(net->d_pops); F(net->d_chans); F(net->d_gproj); F(net->d_tables);
-  F(net->d_state); F(net->d_ref); F(net->d_sb); F(net->d_sb_count); F(net->d_rec); F(net->d_rec_sorted); F(net->d_sort_tmp);
+  F(net->d_state); F(net->d_ref); F(net->d_sb); F(net->d_sb_count);
+  F(net->d_rec[0]); F(net->d_rec[1]); F(net->d_rec_sorted); F(net->d_sort_tmp);
   F(net->d_rec_count); F(net->d_flags); F(net->d_vm_col); F(net->d_vm);
   for (void* p : net->owned) F(p);
   F(net->d_base); F(net->d_blob); F(net->d_ring_cnt); F(net->d_ring_fx); F(net->d_prevlen);
@@ -1115,7 +1140,15 @@ static void free_net(bs_net* net) {
   F(net->d_spk_hist);
   F(net->d_pjobs);
   F(net->d_pcnt);
-  if (net->graph) (void)hipGraphExecDestroy(net->graph);
+  for (int k = 0; k < 2; ++k) {
+    if (net->graph[k]) (void)hipGraphExecDestroy(net->graph[k]);
+    if (net->ev_chunk[k]) (void)hipEventDestroy(net->ev_chunk[k]);
+  }
+  auto FH = [](void* p) {
+    if (p) (void)hipHostFree(p);
+  };
+  FH(net->h_cnt); FH(net->h_flags); FH(net->h_base_p); FH(net->h_prev_p); FH(net->h_rec); FH(net->h_vm_p);
+  if (net->stream2) (void)hipStreamDestroy(net->stream2);
   if (net->stream) (void)hipStreamDestroy(net->stream);
   delete net;
 }
@@ -1226,6 +1259,12 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   net->dt = d->dt_ms;
   HCHK(hipGetDevice(&net->device));
   HCHK(hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking));
+  HCHK(hipStreamCreateWithFlags(&net->stream2, hipStreamNonBlocking));
+  for (int k = 0; k < 2; ++k) HCHK(hipEventCreateWithFlags(&net->ev_chunk[k], hipEventDisableTiming));
+  HCHK(hipHostMalloc((void**)&net->h_cnt, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+  HCHK(hipHostMalloc((void**)&net->h_flags, 2 * sizeof(int), hipHostMallocDefault));
+  HCHK(hipHostMalloc((void**)&net->h_base_p, 2 * sizeof(long long), hipHostMallocDefault));
+  HCHK(hipHostMalloc((void**)&net->h_prev_p, 2 * sizeof(int), hipHostMallocDefault));
 
   // ---- tile size: the largest of 256/128/64 neurons that still gives >= 1024
   // tiles (four per CU), so small networks spread over the whole GPU. With
@@ -1805,7 +1844,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   CHK(dmalloc(&net->d_ref, gid));
   CHK(dmalloc(&net->d_sb, (size_t)net->Ls * gid));
   CHK(dmalloc(&net->d_sb_count, (size_t)net->Ls * d->n_pops));
-  CHK(dmalloc(&net->d_rec_count, 1));
+  CHK(dmalloc(&net->d_rec_count, 2));
   CHK(dmalloc(&net->d_flags, 1));
   CHK(dmalloc(&net->d_base, 1));
   CHK(dmalloc(&net->d_prevlen, 1));
@@ -1844,12 +1883,13 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     if (k < 1) k = 1;
     if (k > 512) k = 512;
     net->rec_cap = per_win * k;
-    CHK(dmalloc(&net->d_rec, net->rec_cap));
+    CHK(dmalloc(&net->d_rec[0], net->rec_cap));
+    CHK(dmalloc(&net->d_rec[1], net->rec_cap));
     CHK(dmalloc(&net->d_rec_sorted, net->rec_cap));
     net->vm_cap_steps = k * W;
     while ((1ll << net->gid_bits) < gid) ++net->gid_bits;
     while ((1ll << net->step_bits) < net->vm_cap_steps) ++net->step_bits;
-    HCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, net->sort_tmp_bytes, net->d_rec, net->d_rec_sorted,
+    HCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, net->sort_tmp_bytes, net->d_rec[0], net->d_rec_sorted,
                                            (int)net->rec_cap, 0, net->gid_bits + net->step_bits));
     {
       void* t = nullptr;
@@ -1921,8 +1961,8 @@ static int upload_state(bs_net* net) {
 
 int bs_net_reset(bs_net* net) {
   if (!net) return fail(BS_EINVAL, "bs_net_reset: null handle");
-  net->out_gid.clear();
-  net->out_step.clear();
+  net->n_out_blocks = 0;
+  net->n_out = 0;
   return upload_state(net);
 }
 
@@ -1965,9 +2005,9 @@ static void launch_window(bs_net* net, int n0, int W, long long vm_step0, int pr
   a.Wmax = net->W;
   a.max_span = net->max_span;
   a.TS = net->TS;
-  a.rec = net->d_rec;
+  a.rec = net->d_rec[net->rec_par];
   a.gid_bits = net->gid_bits;
-  a.rec_count = net->d_rec_count;
+  a.rec_count = net->d_rec_count + net->rec_par;
   a.rec_cap = net->rec_cap;
   a.flags = net->d_flags;
   a.vm_col = net->d_vm_col;
@@ -2024,21 +2064,225 @@ int bs_net_set_timing(bs_net* net, int on) {
   return BS_OK;
 }
 
+// One chunk of windows queued on the stream and not yet drained.
+struct ChunkJob {
+  long long c0 = 0, chunk = 0, done = 0;
+  int par = 0, nwin = 0;
+};
+
+// Queues one chunk of windows writing record buffer j.par (a graph replay when
+// the chunk is full), then the copies of its record count, the flags and, with
+// pin_vm, its V_m rows into pinned memory, and marks its end with
+// ev_chunk[j.par]. Returns after queueing; nothing here waits for the device.
+static int enqueue_chunk(bs_net* net, const ChunkJob& j, bool pin_vm, std::vector<hipEvent_t>& evs,
+                         double state_bytes) {
+  net->rec_par = j.par;
+  HIPCHK(hipMemsetAsync(net->d_rec_count + j.par, 0, sizeof(unsigned long long), net->stream));
+  if (net->timing) {
+    for (size_t i = evs.size(); i < (size_t)j.nwin * 2; ++i) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      evs.push_back(e);
+    }
+  }
+  const bool full = j.chunk == net->vm_cap_steps;
+  if (full && !net->timing) {
+    // replay the captured chunk: K window launches reading the base step
+    hipGraphExec_t& G = net->graph[j.par];
+    if (!G) {
+      hipGraph_t g;
+      HIPCHK(hipStreamBeginCapture(net->stream, hipStreamCaptureModeThreadLocal));
+      for (int w = 0; w < j.nwin; ++w) {
+        if (!net->stdp.empty()) launch_stdp(net, 0, net->d_base, w);
+        if (net->tsize == 8)
+          launch_window<double>(net, 0, net->W, 0, net->W, net->d_base, w);
+        else
+          launch_window<float>(net, 0, net->W, 0, net->W, net->d_base, w);
+      }
+      HIPCHK(hipStreamEndCapture(net->stream, &g));
+      hipError_t e = hipGraphInstantiate(&G, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return fail(BS_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    // pinned slots of this buffer: rewritten two chunks later, after this
+    // chunk's event has been waited on
+    net->h_base_p[j.par] = j.c0;
+    net->h_prev_p[j.par] = net->last_win;
+    HIPCHK(hipMemcpyAsync(net->d_base, net->h_base_p + j.par, sizeof(long long), hipMemcpyHostToDevice,
+                          net->stream));
+    HIPCHK(hipMemcpyAsync(net->d_prevlen, net->h_prev_p + j.par, sizeof(int), hipMemcpyHostToDevice,
+                          net->stream));
+    HIPCHK(hipGraphLaunch(G, net->stream));
+    net->last_win = net->W;
+    net->n_update += j.nwin;
+    net->b_update += state_bytes * j.chunk;
+  } else {
+    for (int w = 0; w < j.nwin; ++w) {
+      const long long n0 = j.c0 + (long long)w * net->W;
+      const int Wc = (int)std::min<long long>(net->W, j.c0 + j.chunk - n0);
+      if (net->timing) HIPCHK(hipEventRecord(evs[2 * w], net->stream));
+      if (!net->stdp.empty()) launch_stdp(net, (int)n0, nullptr, 0);
+      if (net->tsize == 8)
+        launch_window<double>(net, (int)n0, Wc, j.c0, net->last_win);
+      else
+        launch_window<float>(net, (int)n0, Wc, j.c0, net->last_win);
+      net->last_win = Wc;
+      if (net->timing) HIPCHK(hipEventRecord(evs[2 * w + 1], net->stream));
+      HIPCHK(hipGetLastError());
+      net->n_update++;
+      net->b_update += state_bytes * Wc;
+    }
+  }
+  HIPCHK(hipMemcpyAsync(net->h_cnt + j.par, net->d_rec_count + j.par, sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, net->stream));
+  HIPCHK(hipMemcpyAsync(net->h_flags + j.par, net->d_flags, sizeof(int), hipMemcpyDeviceToHost, net->stream));
+  if (pin_vm)
+    HIPCHK(hipMemcpyAsync(net->h_vm_p + (size_t)j.par * net->vm_cap_steps * net->n_vm_dev, net->d_vm,
+                          sizeof(double) * (size_t)j.chunk * net->n_vm_dev, hipMemcpyDeviceToHost, net->stream));
+  HIPCHK(hipEventRecord(net->ev_chunk[j.par], net->stream));
+  return BS_OK;
+}
+
+// Waits for chunk j, sorts its records by (step, gid) on the second stream,
+// copies them into pinned memory and decodes them into the next spike block;
+// fills the chunk's rows of vm_out. Runs while the next chunk computes.
+static int drain_chunk(bs_net* net, const ChunkJob& j, bool pin_vm, double* vm_out,
+                       std::vector<hipEvent_t>& evs) {
+  HIPCHK(hipEventSynchronize(net->ev_chunk[j.par]));
+  const int flags = net->h_flags[j.par];
+  if (flags & FLAG_OVERFLOW) return fail(BS_ECAPACITY, "spike list overflow");
+  if (flags & FLAG_COUNT) return fail(BS_ECAPACITY, "arrival event count exceeded the channel's in-degree");
+  if (flags & FLAG_UNSTABLE) return fail(BS_ENUMERIC, "Numerical instability in aeif_cond_alpha dynamics.");
+  if (net->timing) {
+    for (int w = 0; w < j.nwin; ++w) {
+      float a = 0;
+      HIPCHK(hipEventElapsedTime(&a, evs[2 * w], evs[2 * w + 1]));
+      net->t_update += a;
+    }
+  }
+  const unsigned long long cnt = net->h_cnt[j.par];
+  if (cnt) {
+    if (cnt > net->h_rec_cap) {
+      const size_t want = std::max<size_t>((size_t)cnt, 2 * net->h_rec_cap);
+      if (net->h_rec) HIPCHK(hipHostFree(net->h_rec));
+      net->h_rec = nullptr;
+      net->h_rec_cap = 0;
+      HIPCHK(hipHostMalloc((void**)&net->h_rec, sizeof(unsigned long long) * want, hipHostMallocDefault));
+      net->h_rec_cap = want;
+    }
+    size_t tb = net->sort_tmp_bytes;
+    HIPCHK(hipcub::DeviceRadixSort::SortKeys(net->d_sort_tmp, tb, net->d_rec[j.par], net->d_rec_sorted,
+                                             (int)cnt, 0, net->gid_bits + net->step_bits, net->stream2));
+    HIPCHK(hipMemcpyAsync(net->h_rec, net->d_rec_sorted, sizeof(unsigned long long) * cnt,
+                          hipMemcpyDeviceToHost, net->stream2));
+    HIPCHK(hipStreamSynchronize(net->stream2));
+  }
+  const unsigned long long gmask = (1ull << net->gid_bits) - 1;
+  if (net->n_out_blocks == net->out_blocks.size()) net->out_blocks.emplace_back();
+  bs_net::SpikeBlock& blk = net->out_blocks[net->n_out_blocks++];
+  if (blk.gid.size() < cnt) {
+    blk.gid.resize(cnt);
+    blk.step.resize(cnt);
+  }
+  size_t w = 0;
+  long long ev = 0;
+  const int gb = net->gid_bits;
+  const long long* od = net->outdeg.data();
+  const unsigned long long* hr = net->h_rec;
+  int32_t* og = blk.gid.data();
+  int32_t* os = blk.step.data();
+  if (net->all_recorded) {
+    // every record is kept: a straight decode, split over host threads when
+    // the chunk is large (the decode must finish inside the next chunk)
+    const long long c0 = j.c0;
+    auto decode = [&](size_t lo, size_t hi, long long* evp) {
+      long long e = 0;
+      for (size_t i = lo; i < hi; ++i) {
+        const unsigned long long k = hr[i];
+        const int g = (int)(k & gmask);
+        e += od[g];
+        og[i] = g;
+        os[i] = (int)(c0 + (long long)(k >> gb));
+      }
+      *evp = e;
+    };
+    const unsigned hc = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    const unsigned nth = cnt >= (1ull << 18) ? hc : 1u;
+    std::vector<long long> evt(nth, 0);
+    std::vector<std::thread> pool;
+    const size_t per = (size_t)((cnt + nth - 1) / nth);
+    for (unsigned t = 1; t < nth; ++t) {
+      const size_t lo = std::min<size_t>(cnt, per * t), hi = std::min<size_t>(cnt, per * (t + 1));
+      pool.emplace_back(decode, lo, hi, &evt[t]);
+    }
+    decode(0, std::min<size_t>(cnt, per), &evt[0]);
+    for (std::thread& th : pool) th.join();
+    for (long long e : evt) ev += e;
+    w = cnt;
+  } else {
+    for (unsigned long long i = 0; i < cnt; ++i) {
+      const unsigned long long k = hr[i];
+      const int g = (int)(k & gmask);
+      ev += od[g];
+      if (net->pop_record[net->gid_pop[g]]) {
+        og[w] = g;
+        os[w] = (int)(j.c0 + (long long)(k >> gb));
+        ++w;
+      }
+    }
+  }
+  blk.n = w;
+  net->n_out += (long long)w;
+  net->n_events += ev;
+  // gather: each spike's bucket entry and segment bounds, 4 B per edge; 8 B record
+  net->b_deliver += 8.0 * (double)cnt + 4.0 * (double)ev;
+  net->b_update += 8.0 * (double)cnt;
+  const int ntaps = (int)net->vm_tap_col.size();
+  if (vm_out && ntaps > 0) {
+    const double* hv;
+    std::vector<double> hvm;
+    if (pin_vm) {
+      hv = net->h_vm_p + (size_t)j.par * net->vm_cap_steps * net->n_vm_dev;
+    } else {
+      hvm.resize((size_t)j.chunk * std::max(1, net->n_vm_dev));
+      HIPCHK(hipMemcpy(hvm.data(), net->d_vm, 8 * hvm.size(), hipMemcpyDeviceToHost));
+      hv = hvm.data();
+    }
+    for (long long t = 0; t < j.chunk; ++t)
+      for (int k = 0; k < ntaps; ++k)
+        vm_out[(j.done + t) * ntaps + k] = hv[(size_t)t * net->n_vm_dev + net->vm_tap_col[k]];
+  }
+  return BS_OK;
+}
+
 int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
   if (!net) return fail(BS_EINVAL, "bs_net_run: null handle");
   if (n_steps < 0) return fail(BS_EINVAL, "bs_net_run: n_steps must be >= 0");
   if (net->step + n_steps >= (1ll << 31)) return fail(BS_EINVAL, "bs_net_run: step counter overflow");
   HIPCHK(hipSetDevice(net->device));
-  net->out_gid.clear();
-  net->out_step.clear();
+  net->n_out_blocks = 0;
+  net->n_out = 0;
   net->t_deliver = net->t_update = net->t_total = 0;
   net->n_deliver = net->n_update = 0;
   net->b_deliver = net->b_update = 0;
   net->n_events = 0;
   const int ntaps = (int)net->vm_tap_col.size();
   const long long chunk_max = net->vm_cap_steps;  // = K * W
-  std::vector<unsigned long long> hrec;
-  std::vector<double> hvm;
+  // Pipelined drain: chunk c + 1 is queued before chunk c is drained, so the
+  // device never waits on the host's sort, copy and decode. Timed runs keep
+  // one chunk in flight (their per-window events are read at each drain), and
+  // so do runs whose V_m rows would need more than 256 MiB of pinned memory.
+  const size_t vm_bytes = sizeof(double) * (size_t)chunk_max * std::max(1, net->n_vm_dev);
+  const bool want_vm = vm_out && ntaps > 0;
+  const bool pipelined = !net->timing && n_steps > chunk_max && (!want_vm || vm_bytes <= (256u << 20));
+  const bool pin_vm = pipelined && want_vm;
+  if (pin_vm && net->h_vm_cap < 2 * vm_bytes) {
+    if (net->h_vm_p) HIPCHK(hipHostFree(net->h_vm_p));
+    net->h_vm_p = nullptr;
+    net->h_vm_cap = 0;
+    HIPCHK(hipHostMalloc((void**)&net->h_vm_p, 2 * vm_bytes, hipHostMallocDefault));
+    net->h_vm_cap = 2 * vm_bytes;
+  }
   hipEvent_t ev_all0, ev_all1;
   HIPCHK(hipEventCreate(&ev_all0));
   HIPCHK(hipEventCreate(&ev_all1));
@@ -2048,151 +2292,47 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
     launch_pois_fill(net, (int)net->step);
     HIPCHK(hipGetLastError());
   }
-  long long done = 0;
-  int rc = BS_OK;
   // Algorithmic update bytes per network step: every neuron reads and writes
   // its state variables and its int32 refractory counter and reads one
   // arrival slot (4 B count), plus 8 B per recorded V_m sample.
   double state_bytes = 8.0 * net->n_vm_dev;
   for (const DevPop& p : net->pops) state_bytes += (double)p.n * (2.0 * p.n_state * net->tsize + 8.0 + 4.0);
-  while (done < n_steps) {
-    const long long chunk = std::min<long long>(chunk_max, n_steps - done);
-    const long long c0 = net->step;
-    HIPCHK(hipMemsetAsync(net->d_rec_count, 0, sizeof(unsigned long long), net->stream));
-    const int nwin = (int)((chunk + net->W - 1) / net->W);
-    if (net->timing) {
-      for (size_t i = evs.size(); i < (size_t)nwin * 2; ++i) {
-        hipEvent_t e;
-        HIPCHK(hipEventCreate(&e));
-        evs.push_back(e);
+  long long done = 0;
+  int rc = BS_OK;
+  int par = 0;
+  ChunkJob pending;
+  bool have_pending = false;
+  while (rc == BS_OK && (done < n_steps || have_pending)) {
+    ChunkJob cur;
+    bool queued = false;
+    if (done < n_steps) {
+      cur.c0 = net->step;
+      cur.chunk = std::min<long long>(chunk_max, n_steps - done);
+      cur.done = done;
+      cur.par = par;
+      cur.nwin = (int)((cur.chunk + net->W - 1) / net->W);
+      rc = enqueue_chunk(net, cur, pin_vm, evs, state_bytes);
+      if (rc != BS_OK) break;
+      queued = true;
+      net->step += cur.chunk;
+      done += cur.chunk;
+      par ^= 1;
+    }
+    if (have_pending) {
+      rc = drain_chunk(net, pending, pin_vm, vm_out, evs);
+      have_pending = false;
+    }
+    if (queued) {
+      if (pipelined) {
+        pending = cur;
+        have_pending = true;
+      } else if (rc == BS_OK) {
+        rc = drain_chunk(net, cur, pin_vm, vm_out, evs);
       }
     }
-    const bool full = chunk == chunk_max;
-    if (full && !net->timing) {
-      // replay the captured chunk: K window launches reading the base step
-      if (!net->graph) {
-        hipGraph_t g;
-        HIPCHK(hipStreamBeginCapture(net->stream, hipStreamCaptureModeThreadLocal));
-        for (int w = 0; w < nwin; ++w) {
-          if (!net->stdp.empty()) launch_stdp(net, 0, net->d_base, w);
-          if (net->tsize == 8)
-            launch_window<double>(net, 0, net->W, 0, net->W, net->d_base, w);
-          else
-            launch_window<float>(net, 0, net->W, 0, net->W, net->d_base, w);
-        }
-        HIPCHK(hipStreamEndCapture(net->stream, &g));
-        hipError_t e = hipGraphInstantiate(&net->graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) return fail(BS_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-      }
-      net->h_base = c0;  // stays valid until the chunk's synchronisation below
-      net->h_prevlen = net->last_win;
-      HIPCHK(hipMemcpyAsync(net->d_base, &net->h_base, sizeof(long long), hipMemcpyHostToDevice,
-                            net->stream));
-      HIPCHK(hipMemcpyAsync(net->d_prevlen, &net->h_prevlen, sizeof(int), hipMemcpyHostToDevice,
-                            net->stream));
-      HIPCHK(hipGraphLaunch(net->graph, net->stream));
-      net->last_win = net->W;
-      net->n_update += nwin;
-      net->b_update += state_bytes * chunk;
-    } else {
-      for (int w = 0; w < nwin; ++w) {
-        const long long n0 = c0 + (long long)w * net->W;
-        const int Wc = (int)std::min<long long>(net->W, c0 + chunk - n0);
-        if (net->timing) HIPCHK(hipEventRecord(evs[2 * w], net->stream));
-        if (!net->stdp.empty()) launch_stdp(net, (int)n0, nullptr, 0);
-        if (net->tsize == 8)
-          launch_window<double>(net, (int)n0, Wc, c0, net->last_win);
-        else
-          launch_window<float>(net, (int)n0, Wc, c0, net->last_win);
-        net->last_win = Wc;
-        if (net->timing) HIPCHK(hipEventRecord(evs[2 * w + 1], net->stream));
-        HIPCHK(hipGetLastError());
-        net->n_update++;
-        net->b_update += state_bytes * Wc;
-      }
-    }
-    unsigned long long cnt = 0;
-    HIPCHK(hipMemcpyAsync(&cnt, net->d_rec_count, sizeof(cnt), hipMemcpyDeviceToHost, net->stream));
-    int flags = 0;
-    HIPCHK(hipMemcpyAsync(&flags, net->d_flags, sizeof(int), hipMemcpyDeviceToHost, net->stream));
-    HIPCHK(hipStreamSynchronize(net->stream));
-    if (flags & FLAG_OVERFLOW) {
-      rc = fail(BS_ECAPACITY, "spike list overflow");
-      break;
-    }
-    if (flags & FLAG_COUNT) {
-      rc = fail(BS_ECAPACITY, "arrival event count exceeded the channel's in-degree");
-      break;
-    }
-    if (flags & FLAG_UNSTABLE) {
-      rc = fail(BS_ENUMERIC, "Numerical instability in aeif_cond_alpha dynamics.");
-      break;
-    }
-    if (net->timing) {
-      for (int w = 0; w < nwin; ++w) {
-        float a = 0;
-        HIPCHK(hipEventElapsedTime(&a, evs[2 * w], evs[2 * w + 1]));
-        net->t_update += a;
-      }
-    }
-    // the chunk's records sorted by (step, gid) on the device, then decoded
-    hrec.resize(cnt);
-    if (cnt) {
-      size_t tb = net->sort_tmp_bytes;
-      HIPCHK(hipcub::DeviceRadixSort::SortKeys(net->d_sort_tmp, tb, net->d_rec, net->d_rec_sorted, (int)cnt,
-                                               0, net->gid_bits + net->step_bits, net->stream));
-      HIPCHK(hipMemcpyAsync(hrec.data(), net->d_rec_sorted, sizeof(unsigned long long) * cnt,
-                            hipMemcpyDeviceToHost, net->stream));
-      HIPCHK(hipStreamSynchronize(net->stream));
-    }
-    const unsigned long long gmask = (1ull << net->gid_bits) - 1;
-    const size_t base = net->out_gid.size();
-    net->out_gid.resize(base + cnt);
-    net->out_step.resize(base + cnt);
-    size_t w = base;
-    long long ev = 0;
-    const int gb = net->gid_bits;
-    const long long* od = net->outdeg.data();
-    if (net->all_recorded) {
-      int32_t* og = net->out_gid.data();
-      int32_t* os = net->out_step.data();
-      for (unsigned long long i = 0; i < cnt; ++i) {
-        const unsigned long long k = hrec[i];
-        const int g = (int)(k & gmask);
-        ev += od[g];
-        og[w] = g;
-        os[w] = (int)(c0 + (long long)(k >> gb));
-        ++w;
-      }
-    } else {
-      for (unsigned long long i = 0; i < cnt; ++i) {
-        const unsigned long long k = hrec[i];
-        const int g = (int)(k & gmask);
-        ev += od[g];
-        if (net->pop_record[net->gid_pop[g]]) {
-          net->out_gid[w] = g;
-          net->out_step[w] = (int)(c0 + (long long)(k >> gb));
-          ++w;
-        }
-      }
-    }
-    net->out_gid.resize(w);
-    net->out_step.resize(w);
-    net->n_events += ev;
-    // gather: each spike's bucket entry and segment bounds, 4 B per edge; 8 B record
-    net->b_deliver += 8.0 * (double)cnt + 4.0 * (double)ev;
-    net->b_update += 8.0 * (double)cnt;
-    if (vm_out && ntaps > 0) {
-      hvm.resize((size_t)chunk * std::max(1, net->n_vm_dev));
-      HIPCHK(hipMemcpy(hvm.data(), net->d_vm, 8 * hvm.size(), hipMemcpyDeviceToHost));
-      for (long long t = 0; t < chunk; ++t)
-        for (int k = 0; k < ntaps; ++k)
-          vm_out[(done + t) * ntaps + k] = hvm[(size_t)t * net->n_vm_dev + net->vm_tap_col[k]];
-    }
-    net->step += chunk;
-    done += chunk;
   }
+  // an error leaves at most one chunk queued: let it finish before returning
+  (void)hipStreamSynchronize(net->stream);
   HIPCHK(hipEventRecord(ev_all1, net->stream));
   HIPCHK(hipEventSynchronize(ev_all1));
   float tot = 0;
@@ -2208,15 +2348,20 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
 
 int bs_net_spike_count(bs_net* net, int64_t* n) {
   if (!net || !n) return fail(BS_EINVAL, "bs_net_spike_count: null argument");
-  *n = (int64_t)net->out_gid.size();
+  *n = (int64_t)net->n_out;
   return BS_OK;
 }
 
 int bs_net_copy_spikes(bs_net* net, int32_t* gid, int32_t* step, int64_t cap) {
   if (!net || !gid || !step) return fail(BS_EINVAL, "bs_net_copy_spikes: null argument");
-  if (cap < (int64_t)net->out_gid.size()) return fail(BS_ECAPACITY, "bs_net_copy_spikes: cap too small");
-  memcpy(gid, net->out_gid.data(), 4 * net->out_gid.size());
-  memcpy(step, net->out_step.data(), 4 * net->out_step.size());
+  if (cap < (int64_t)net->n_out) return fail(BS_ECAPACITY, "bs_net_copy_spikes: cap too small");
+  for (size_t b = 0; b < net->n_out_blocks; ++b) {
+    const bs_net::SpikeBlock& blk = net->out_blocks[b];
+    memcpy(gid, blk.gid.data(), 4 * blk.n);
+    memcpy(step, blk.step.data(), 4 * blk.n);
+    gid += blk.n;
+    step += blk.n;
+  }
   return BS_OK;
 }
 

@@ -112,6 +112,23 @@ def test_brunel_1000_delta_fp32_bitexact_vs_fp32_oracle():
     _assert_same(gpu, orc, rtol=0)
 
 
+def test_pipelined_chunks_and_split_runs():
+    """Runs of several chunks (both record buffers and both captured graphs in
+    use, then a partial chunk), split across run calls at points inside a
+    chunk: the raster and V_m rows equal the oracle's single run."""
+    sim, h = brunel_delta(order=50, n_vm=6, dtype='float64')
+    desc = sim.build_desc()
+    eng = sim._ensure_engine()
+    eng.reset()
+    parts = [eng.run(n) for n in (30011, 9000, 1)]
+    vm_g = np.concatenate([p[0] for p in parts])
+    g_g = np.concatenate([p[1][0] for p in parts])
+    s_g = np.concatenate([p[1][1] for p in parts])
+    vm_o, g_o, s_o = oracle.OracleNet(desc).run(39012)
+    assert len(g_o) > 5000
+    _assert_same((vm_g, g_g, s_g), (vm_o, g_o, s_o))
+
+
 def test_brunel_per_edge_delays_small():
     sim, h = brunel_delta(order=100, delay=('uniform', 1.0, 3.0), n_vm=5)
     gpu, orc = _both(sim, 3000)
