@@ -55,6 +55,11 @@ def main():
     ap.add_argument('--ensemble', type=int, default=0, metavar='M',
                     help='instead: the batched-ensemble workload with M networks (256 = full size)')
     ap.add_argument('--ensemble-steps', type=int, default=5000)
+    ap.add_argument('--connectivity', default='host', choices=['host', 'device'],
+                    help='where the ensemble samples its random edges (the networks differ)')
+    ap.add_argument('--build-bench', action='store_true',
+                    help='instead: network + engine build times of Brunel-11250 and the '
+                         '256-network ensemble with host and with device connectivity')
     ap.add_argument('--aeif-stdp', type=int, default=0, metavar='STEPS',
                     help='instead: the conductance-based aeif_cond_alpha + stdp_synapse network '
                          '(50,000 neurons, p = 0.01) for STEPS steps')
@@ -75,6 +80,9 @@ def main():
     import numpy as np
     from brainpy.state_amd.networks import brunel_delta
 
+    if args.build_bench:
+        print(json.dumps(run_build_bench(args)), flush=True)
+        return
     if args.ensemble:
         print(json.dumps(run_ensemble(args)), flush=True)
         return
@@ -238,6 +246,57 @@ def pmc_traffic(kernel, skip=50):
                                  f'scale), mean of {len(f)} dispatches after the first {skip}')
 
 
+def run_build_bench(args):
+    """Seconds to sample the network (Simulator.connect) and to create the
+    engine (bs_net_create: gather segments, tables, state), for Brunel-11250
+    (12.7M edges, per-edge delays) and the 256-network ensemble (1.76e9
+    edges), with the random rules drawn on the host (NumPy PCG64, segments by
+    a host counting sort) and on the device (Threefry samplers, segments by a
+    device radix sort). The two modes draw different networks of the same
+    distribution."""
+    import torch
+    from brainpy.state_amd import Ensemble
+    from brainpy.state_amd.networks import brunel_delta, brunel_exp_ensemble
+    out = {'metric': 'network build seconds (sampling + engine creation)', 'unit': 's',
+           'higher_is_better': False, 'dtype': 'int32 edges', 'cases': {}}
+    for mode in ('host', 'device'):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim, h = brunel_delta(order=2250, delay=('uniform', 1.0, 3.0), n_vm=10, connectivity=mode)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        eng = sim._ensure_engine()
+        t2 = time.perf_counter()
+        E = sum(int(p.n_edges) for p in eng.desc.c.projs[:eng.desc.c.n_projs])
+        eng.close()
+        sim._engine = None
+        out['cases'][f'brunel11250/{mode}'] = {'edges': E, 'sample_s': round(t1 - t0, 3),
+                                               'engine_s': round(t2 - t1, 3),
+                                               'edges_per_s': round(E / (t2 - t0), 1)}
+        log(f'brunel-11250 {mode}: {E} edges, sampled in {t1 - t0:.2f}s, engine {t2 - t1:.2f}s')
+        del sim, h
+    for mode in ('host', 'device'):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sims, prm = brunel_exp_ensemble(256, seed=0, connectivity=mode)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ens = Ensemble(sims, tile_size=128)
+        t2 = time.perf_counter()
+        E = sum(int(p.n_edges) for p in ens._engine.desc.c.projs[:ens._engine.desc.c.n_projs])
+        ens.reset()
+        _, (g, _) = ens.run_raw(500)
+        ens.close()
+        out['cases'][f'ensemble256/{mode}'] = {'edges': E, 'sample_s': round(t1 - t0, 3),
+                                               'engine_s': round(t2 - t1, 3),
+                                               'edges_per_s': round(E / (t2 - t0), 1),
+                                               'spikes_500_steps': int(len(g))}
+        log(f'ensemble-256 {mode}: {E} edges, sampled in {t1 - t0:.2f}s, engine {t2 - t1:.2f}s, '
+            f'{len(g)} spikes in 500 steps')
+        del sims, ens
+    return out
+
+
 def run_ensemble(args):
     """Independent Brunel iaf_psc_exp networks (N log-uniform 500-20000, 10 %
     in-degree, g ~ U[3,7], nu_ext/nu_thr ~ U[0.9,4], tau_syn 0.5 ms, float32)
@@ -248,7 +307,7 @@ def run_ensemble(args):
     from brainpy.state_amd.networks import brunel_exp_ensemble
     M, steps = args.ensemble, args.ensemble_steps
     t0 = time.time()
-    sims, prm = brunel_exp_ensemble(M, seed=0)
+    sims, prm = brunel_exp_ensemble(M, seed=0, connectivity=args.connectivity)
     syn = int(sum(0.1 * n * n for n in prm['N']))
     log(f'ensemble: {M} networks, {int(prm["N"].sum())} neurons, ~{syn:.3e} synapses, '
         f'built in {time.time() - t0:.1f}s')
@@ -297,7 +356,7 @@ def run_ensemble(args):
     return {'metric': 'synaptic events/s (batched ensemble, 1 GPU)', 'value': round(by['n_events'] / wall, 1),
             'unit': 'events/s', 'steps': steps, 'ms_per_step': round(wall / steps * 1e3, 4),
             'rtf': round(wall / (steps * sims[0].dt * 1e-3), 4), 'dtype': 'f32',
-            'config': {'workload': 'brunel iaf_psc_exp ensemble', 'networks': M,
+            'config': {'workload': 'brunel iaf_psc_exp ensemble', 'networks': M, 'connectivity': args.connectivity,
                        'neurons': int(prm['N'].sum()), 'synapses_approx': syn, 'tile_size': best[0],
                        'max_window': best[1]},
             'autotune_ms_per_step': {f'{k[0]}/{k[1]}': round(v * 1e3, 4) for k, v in table.items()},

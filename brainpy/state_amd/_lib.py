@@ -15,7 +15,7 @@ __all__ = ['lib', 'EngineUnavailable', 'EngineError', 'check', 'BS_NPARAM', 'BS_
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libbrainspike_hip.so')
 
-BS_ABI_VERSION = 3
+BS_ABI_VERSION = 4
 BS_NPARAM = 40
 BS_NSTATE = 9
 BS_F64, BS_F32 = 0, 1
@@ -64,7 +64,7 @@ class Projection(C.Structure):
                 ('plastic', C.c_int32), ('n_edges', C.c_int64),
                 ('pre', C.POINTER(C.c_int32)), ('post', C.POINTER(C.c_int32)),
                 ('delay', C.POINTER(C.c_int32)), ('delay_steps', C.c_int32),
-                ('_pad', C.c_int32), ('weight', C.POINTER(C.c_double)), ('stdp', Stdp)]
+                ('on_device', C.c_int32), ('weight', C.POINTER(C.c_double)), ('stdp', Stdp)]
 
 
 class NetDesc(C.Structure):
@@ -110,6 +110,12 @@ _SIGS = {
     'bs_bptt_forward': (C.c_int, [_P, _P, _P, _P, _P, _P, C.POINTER(C.c_double)]),
     'bs_bptt_backward': (C.c_int, [_P, _P, _P]),
     'bs_bptt_spikes': (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_float)]),
+    'bs_conn_fixed_indegree': (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                         _P, _P, _P, _P]),
+    'bs_conn_bernoulli_count': (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_int32, _P,
+                                          C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    'bs_conn_bernoulli_fill': (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_int32, _P, _P, _P,
+                                         _P, _P]),
     'bs_debug_math': (C.c_int, [C.c_int32, C.c_int64, C.POINTER(C.c_double),
                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     'bs_debug_threefry': (C.c_int, [C.c_int64] + [C.POINTER(C.c_uint32)] * 6),

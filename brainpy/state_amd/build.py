@@ -5,7 +5,7 @@ import os
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = [os.path.join(HERE, 'csrc', 'bs_net.hip'), os.path.join(HERE, 'csrc', 'bs_bptt.hip')]
+SRC = [os.path.join(HERE, 'csrc', f) for f in ('bs_net.hip', 'bs_bptt.hip', 'bs_conn.hip')]
 OUT = os.path.join(HERE, 'libbrainspike_hip.so')
 
 # -ffp-contract=off: every arithmetic op is one IEEE-rounded op (no FMA

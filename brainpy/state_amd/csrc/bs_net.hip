@@ -1240,6 +1240,193 @@ static void poisson_table(double lam, std::vector<double>& cdf, std::vector<unsi
   }
 }
 
+// ---- gather segments of a projection whose edges are already in device
+// memory: the layout of the host counting sort (segments in (pre, tile)
+// row-major order, each holding its edges in edge order), from a stable radix
+// sort of the edge indices by (pre, tile).
+__global__ void k_seg_keys(const int32_t* __restrict__ pre, const int32_t* __restrict__ post,
+                           const int32_t* __restrict__ delay, int delay_steps, long long E, int npre, int npost,
+                           int TS, int nt, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                           int* __restrict__ stats) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int dmn = 0x7FFFFFFF, dmx = -1;
+  if (e < E) {
+    const int a = pre[e], b = post[e];
+    const int dd = delay ? delay[e] : delay_steps;
+    uint32_t key = 0;
+    if (a < 0 || a >= npre || b < 0 || b >= npost) {
+      atomicOr(&stats[0], 1);
+    } else if (dd < 0 || dd >= (1 << 24)) {
+      atomicOr(&stats[0], 2);
+    } else {
+      key = (uint32_t)a * (uint32_t)nt + (uint32_t)(b / TS);
+      dmn = dmx = dd;
+    }
+    keys[e] = key;
+    vals[e] = (uint32_t)e;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    dmn = min(dmn, __shfl_xor(dmn, o, 64));
+    dmx = max(dmx, __shfl_xor(dmx, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0 && dmx >= 0) {
+    atomicMin(&stats[1], dmn);
+    atomicMax(&stats[2], dmx);
+  }
+}
+
+// seg[a * (nt + 1) + t] = first sorted position with key >= a * nt + t; the
+// entry t = nt is the row end.
+__global__ void k_seg_bounds(const uint32_t* __restrict__ keys, long long E, int npre, int nt,
+                             uint32_t* __restrict__ seg, int* __restrict__ deg) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = (long long)npre * (nt + 1);
+  if (idx >= n) return;
+  const long long a = idx / (nt + 1), t = idx - a * (nt + 1);
+  const unsigned long long target = (unsigned long long)a * nt + t;
+  long long lo = 0, hi = E;
+  while (lo < hi) {
+    const long long mid = (lo + hi) >> 1;
+    if ((unsigned long long)keys[mid] < target) lo = mid + 1;
+    else hi = mid;
+  }
+  seg[idx] = (uint32_t)lo;
+  if (t == nt) {  // the row is complete once its end is known: one thread per row
+    long long lo0 = 0, hi0 = E;
+    const unsigned long long t0 = (unsigned long long)a * nt;
+    while (lo0 < hi0) {
+      const long long mid = (lo0 + hi0) >> 1;
+      if ((unsigned long long)keys[mid] < t0) lo0 = mid + 1;
+      else hi0 = mid;
+    }
+    deg[a] = (int)(lo - lo0);
+  }
+}
+
+__global__ void k_seg_edges(const uint32_t* __restrict__ order, long long E, const int32_t* __restrict__ post,
+                            const int32_t* __restrict__ delay, int delay_steps, const double* __restrict__ w,
+                            int F, int TS, uint32_t* __restrict__ ew, long long* __restrict__ eq) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= E + 4) return;
+  if (k >= E) {  // the gather reads whole 16-byte chunks past a segment's end
+    ew[k] = 0u;
+    return;
+  }
+  const uint32_t e = order[k];
+  const int dd = delay ? delay[e] : delay_steps;
+  ew[k] = (uint32_t)(post[e] % TS) | ((uint32_t)dd << 8);
+  if (eq) eq[k] = (long long)llrint(ldexp(w[e], F));
+}
+
+// Scratch of the device segment build, grown to the largest projection.
+struct SegScratch {
+  uint32_t* kv[4] = {nullptr, nullptr, nullptr, nullptr};  // keys in/out, values in/out
+  long long cap = 0;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  int* stats = nullptr;
+  int* deg = nullptr;
+  long long deg_cap = 0;
+  ~SegScratch() {
+    for (uint32_t* p : kv) (void)hipFree(p);
+    (void)hipFree(tmp);
+    (void)hipFree(stats);
+    (void)hipFree(deg);
+  }
+};
+
+// Builds G.seg / G.edges / G.q of a static projection with device-resident
+// edges on net->stream; adds the out-degrees to net->outdeg and returns the
+// projection's delay range. Messages match the host build's.
+static int build_segments_device(bs_net* net, const bs_projection& pj, int kind, int F, int npre, int npost,
+                                 long long g0, SegScratch& S, GProj& G, int& pdmin, int& pdmax) {
+  const int TS = net->TS;
+  const long long E = pj.n_edges;
+  const int nt = (npost + TS - 1) / TS;
+  if ((unsigned long long)npre * (unsigned long long)nt >= (1ull << 32))
+    return fail(BS_EINVAL, "device edges: more than 2^32 (pre, tile) segments in one projection");
+  if (E >= (1ll << 31)) return fail(BS_EINVAL, "device edges: more than 2^31 - 1 edges in one projection");
+  if (E > 0 && (!pj.pre || !pj.post)) return fail(BS_EINVAL, "device edges: null edge array");
+  hipStream_t st = net->stream;
+  if (E > S.cap) {
+    for (uint32_t*& p : S.kv) {
+      (void)hipFree(p);
+      p = nullptr;
+    }
+    S.cap = 0;
+    for (uint32_t*& p : S.kv) HIPCHK(hipMalloc(&p, sizeof(uint32_t) * (size_t)E));
+    S.cap = E;
+    size_t tb = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, S.kv[0], S.kv[1], S.kv[2], S.kv[3], (int)E, 0, 32, st));
+    if (tb > S.tmp_bytes) {
+      (void)hipFree(S.tmp);
+      S.tmp = nullptr;
+      S.tmp_bytes = 0;
+      HIPCHK(hipMalloc(&S.tmp, tb));
+      S.tmp_bytes = tb;
+    }
+  }
+  if (!S.stats) HIPCHK(hipMalloc(&S.stats, 4 * sizeof(int)));
+  if (npre > S.deg_cap) {
+    (void)hipFree(S.deg);
+    S.deg = nullptr;
+    HIPCHK(hipMalloc(&S.deg, sizeof(int) * (size_t)std::max(1, npre)));
+    S.deg_cap = npre;
+  }
+  const int stats0[4] = {0, 0x7FFFFFFF, -1, 0};
+  HIPCHK(hipMemcpyAsync(S.stats, stats0, sizeof(stats0), hipMemcpyHostToDevice, st));
+  int bits = 1;
+  while (bits < 32 && (1ull << bits) < (unsigned long long)npre * nt) ++bits;
+  if (E > 0) {
+    hipLaunchKernelGGL(k_seg_keys, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, st, pj.pre, pj.post, pj.delay,
+                       pj.delay_steps, E, npre, npost, TS, nt, S.kv[0], S.kv[2], S.stats);
+    HIPCHK(hipGetLastError());
+    size_t tb = S.tmp_bytes;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(S.tmp, tb, S.kv[0], S.kv[1], S.kv[2], S.kv[3], (int)E, 0, bits, st));
+  }
+  uint32_t *dseg = nullptr, *dew = nullptr;
+  long long* deq = nullptr;
+  const long long nseg = (long long)npre * (nt + 1);
+  int rc = dmalloc(&dseg, (size_t)std::max<long long>(1, nseg));
+  if (rc) return rc;
+  net->owned.push_back(dseg);
+  rc = dmalloc(&dew, (size_t)E + 4);
+  if (rc) return rc;
+  net->owned.push_back(dew);
+  if (kind == BS_CH_FIXED) {
+    rc = dmalloc(&deq, (size_t)std::max<long long>(1, E));
+    if (rc) return rc;
+    net->owned.push_back(deq);
+  }
+  if (nseg > 0) {
+    hipLaunchKernelGGL(k_seg_bounds, dim3((unsigned)((nseg + 255) / 256)), dim3(256), 0, st, S.kv[1], E, npre, nt,
+                       dseg, S.deg);
+    HIPCHK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_seg_edges, dim3((unsigned)((E + 4 + 255) / 256)), dim3(256), 0, st, S.kv[3], E, pj.post,
+                     pj.delay, pj.delay_steps, kind == BS_CH_FIXED ? pj.weight : nullptr, F, TS, dew, deq);
+  HIPCHK(hipGetLastError());
+  int stats[4];
+  std::vector<int> deg((size_t)npre);
+  HIPCHK(hipMemcpyAsync(stats, S.stats, sizeof(stats), hipMemcpyDeviceToHost, st));
+  if (npre > 0) HIPCHK(hipMemcpyAsync(deg.data(), S.deg, sizeof(int) * (size_t)npre, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (stats[0] & 1) return fail(BS_EINVAL, "edge index out of range");
+  if (stats[0] & 2) return fail(BS_EINVAL, "delay must be in [0, 2^24) steps");
+  for (int a = 0; a < npre; ++a) net->outdeg[g0 + a] += deg[a];
+  if (E == 0) {
+    pdmin = pdmax = 0;
+  } else {
+    pdmin = stats[1];
+    pdmax = stats[2];
+  }
+  G.seg = dseg;
+  G.edges = dew;
+  G.q = deq;
+  G.nt = nt;
+  return BS_OK;
+}
+
 int bs_net_create(const bs_net_desc* d, bs_net** out) {
   if (!d || !out) return fail(BS_EINVAL, "bs_net_create: null argument");
   *out = nullptr;
@@ -1421,9 +1608,9 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     auto build_one = [&](int q) {
       const bs_projection& pj = d->projs[q];
       StaticSeg& S = sseg[q];
-      if (pj.plastic || pj.pre_pop < 0 || pj.pre_pop >= d->n_pops || pj.post_pop < 0 ||
+      if (pj.plastic || pj.on_device || pj.pre_pop < 0 || pj.pre_pop >= d->n_pops || pj.post_pop < 0 ||
           pj.post_pop >= d->n_pops || pj.channel < 0 || pj.channel >= d->n_channels)
-        return;  // plastic, or rejected with its message by the serial pass
+        return;  // plastic, device edges, or rejected with its message by the serial pass
       const int kind = d->channels[pj.channel].kind;
       if ((kind != BS_CH_COUNT && kind != BS_CH_FIXED) || (kind == BS_CH_FIXED && !pj.weight) ||
           pj.n_edges < 0 || pj.n_edges >= (1ll << 32))
@@ -1484,6 +1671,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
       });
     for (std::thread& th : pool) th.join();
   }
+  SegScratch seg_scratch;  // device segment builds (projections with on_device edges)
   for (int q = 0; q < d->n_projs; ++q) {
     const bs_projection& pj = d->projs[q];
     if (pj.pre_pop < 0 || pj.pre_pop >= d->n_pops || pj.post_pop < 0 || pj.post_pop >= d->n_pops)
@@ -1495,6 +1683,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     net->proj_stdp.push_back(-1);
     const int kind = d->channels[pj.channel].kind;
     if (pj.plastic) {
+      if (pj.on_device) BAD("a plastic projection takes host edge arrays (on_device = 0)");
       if (kind != BS_CH_FIXED) BAD("a plastic projection needs a FIXED channel");
       if (!pj.weight) BAD("a plastic projection needs per-edge initial weights");
       if (!(pj.stdp.Wmax != 0.0) || !(pj.stdp.tau_plus > 0) || !(pj.stdp.tau_minus > 0))
@@ -1608,27 +1797,34 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     if (kind != BS_CH_COUNT && kind != BS_CH_FIXED) BAD("projection channel must be COUNT or FIXED");
     if (kind == BS_CH_FIXED && !pj.weight) BAD("FIXED projection needs per-edge weights");
     if (pj.n_edges < 0 || pj.n_edges >= (1ll << 32)) BAD("projection edge count out of range");
-    StaticSeg& S = sseg[q];
-    if (!S.err.empty()) BAD(S.err);
-    const int nt = S.nt, pdmin = S.pdmin, pdmax = S.pdmax;
     GProj G;
     memset(&G, 0, sizeof(G));
     G.pre_pop = pj.pre_pop;
     G.chan = pj.channel;
+    int pdmin = 0, pdmax = 0;
+    if (pj.on_device) {
+      CHK(build_segments_device(net, pj, kind, d->channels[pj.channel].fixed_shift, d->pops[pj.pre_pop].n,
+                                d->pops[pj.post_pop].n, gid0_of[pj.pre_pop], seg_scratch, G, pdmin, pdmax));
+    } else {
+      StaticSeg& S = sseg[q];
+      if (!S.err.empty()) BAD(S.err);
+      pdmin = S.pdmin;
+      pdmax = S.pdmax;
+      G.nt = S.nt;
+      uint32_t *dseg = nullptr, *dew = nullptr;
+      long long* deq = nullptr;
+      CHK(upload(net, &dseg, S.seg.data(), S.seg.size()));
+      CHK(upload(net, &dew, S.ew.data(), S.ew.size()));
+      if (kind == BS_CH_FIXED) CHK(upload(net, &deq, S.eq.data(), S.eq.size()));
+      std::vector<uint32_t>().swap(S.seg);  // host copies no longer needed
+      std::vector<uint32_t>().swap(S.ew);
+      std::vector<long long>().swap(S.eq);
+      G.seg = dseg;
+      G.edges = dew;
+      G.q = deq;
+    }
     G.d_min = pdmin;
     G.d_max = pdmax;
-    G.nt = nt;
-    uint32_t *dseg = nullptr, *dew = nullptr;
-    long long* deq = nullptr;
-    CHK(upload(net, &dseg, S.seg.data(), S.seg.size()));
-    CHK(upload(net, &dew, S.ew.data(), S.ew.size()));
-    if (kind == BS_CH_FIXED) CHK(upload(net, &deq, S.eq.data(), S.eq.size()));
-    std::vector<uint32_t>().swap(S.seg);  // host copies no longer needed
-    std::vector<uint32_t>().swap(S.ew);
-    std::vector<long long>().swap(S.eq);
-    G.seg = dseg;
-    G.edges = dew;
-    G.q = deq;
     pop_gp[pj.post_pop].push_back((int)gp_h.size());
     gp_h.push_back(G);
     if (pj.n_edges) {

@@ -37,8 +37,9 @@ def _edge_delays(sim, rule_K, n_post, lo, hi, seed):
 def brunel_delta(order=200, *, NE=None, NI=None, epsilon=0.1, g=5.0, eta=2.0, J=0.1,
                  delay=1.5, dt=0.1, tau_m=20.0, V_th=20.0, V_reset=10.0, t_ref=2.0, C_m=250.0,
                  seed_conn=(1, 2), seed_delay=3, n_vm=10, record_all=True, dtype='float64',
-                 max_window=0):
-    """Brunel-A network of iaf_psc_delta neurons; returns ``(sim, handles)``."""
+                 max_window=0, connectivity='host'):
+    """Brunel-A network of iaf_psc_delta neurons; returns ``(sim, handles)``.
+    ``connectivity='device'`` samples the fixed in-degree edges on the device."""
     NE = 4 * order if NE is None else int(NE)
     NI = order if NI is None else int(NI)
     CE, CI = int(epsilon * NE), int(epsilon * NI)
@@ -46,7 +47,7 @@ def brunel_delta(order=200, *, NE=None, NI=None, epsilon=0.1, g=5.0, eta=2.0, J=
     p_rate = 1000.0 * (eta * nu_th) * CE
     npar = dict(C_m=C_m * u.pF, tau_m=tau_m * u.ms, t_ref=t_ref * u.ms, E_L=0. * u.mV,
                 V_reset=V_reset * u.mV, V_th=V_th * u.mV, V_initializer=Constant(0. * u.mV))
-    sim = Simulator(dt=dt * u.ms, dtype=dtype, max_window=max_window)
+    sim = Simulator(dt=dt * u.ms, dtype=dtype, max_window=max_window, connectivity=connectivity)
     ne = sim.create(iaf_psc_delta, NE, params=npar)
     ni = sim.create(iaf_psc_delta, NI, params=npar)
     noise = sim.create(poisson_generator, rate=p_rate * u.Hz)
@@ -92,7 +93,7 @@ def brunel_delta(order=200, *, NE=None, NI=None, epsilon=0.1, g=5.0, eta=2.0, J=
 
 def brunel_exp(N, *, frac_e=0.8, epsilon=0.1, g=5.0, eta=2.0, J=20.0, delay=1.5, dt=0.1,
                tau_m=20.0, tau_syn=0.5, V_th=-50.0, E_L=-70.0, V_reset=-70.0, t_ref=2.0,
-               C_m=250.0, seed=1, dtype='float32', record=True, sim=None):
+               C_m=250.0, seed=1, dtype='float32', record=True, sim=None, connectivity='host'):
     """Brunel network of iaf_psc_exp neurons (PSC amplitudes in pA).
 
     The external rate is ``eta * nu_th`` with ``nu_th`` the rate that drives the
@@ -109,7 +110,7 @@ def brunel_exp(N, *, frac_e=0.8, epsilon=0.1, g=5.0, eta=2.0, J=20.0, delay=1.5,
     npar = dict(C_m=C_m, tau_m=tau_m, t_ref=t_ref, E_L=E_L, V_reset=V_reset, V_th=V_th,
                 tau_syn_ex=tau_syn, tau_syn_in=tau_syn, V_initializer=Constant(E_L))
     if sim is None:
-        sim = Simulator(dt=dt * u.ms, dtype=dtype)
+        sim = Simulator(dt=dt * u.ms, dtype=dtype, connectivity=connectivity)
     ne = sim.create(iaf_psc_exp, NE, params=npar)
     ni = sim.create(iaf_psc_exp, NI, params=npar)
     noise = sim.create(poisson_generator, rate=p_rate * u.Hz)
@@ -148,7 +149,8 @@ def brunel_exp_ensemble(n_networks=256, *, n_min=500, n_max=20000, g_range=(3.0,
 
 def aeif_stdp(n=50000, *, frac_e=0.8, p=0.01, g_E=1.0, g_I=4.0, rate=8000.0, w_noise=1.0,
               delay=(1.0, 5.0), dt=0.1, seed=1, n_vm=10, plastic=True, tau_plus=20.0,
-              lambda_=0.01, alpha=1.05, Wmax=3.0, mu=0.0, dtype='float64', record=True):
+              lambda_=0.01, alpha=1.05, Wmax=3.0, mu=0.0, dtype='float64', record=True,
+              connectivity='host'):
     """Conductance-based network of aeif_cond_alpha neurons (NEST defaults:
     a = 4 nS, b = 80.5 pA, tau_w = 144 ms, Delta_T = 2 mV; adaptive RKF45),
     random connectivity with probability ``p``, g_E / g_I in nS, per-edge delays
@@ -157,7 +159,7 @@ def aeif_stdp(n=50000, *, frac_e=0.8, p=0.01, g_E=1.0, g_I=4.0, rate=8000.0, w_n
     neuron. Returns ``(sim, handles)``."""
     NE = int(round(frac_e * n))
     NI = int(n) - NE
-    sim = Simulator(dt=dt * u.ms, dtype=dtype)
+    sim = Simulator(dt=dt * u.ms, dtype=dtype, connectivity=connectivity)
     ne = sim.create(aeif_cond_alpha, NE)
     ni = sim.create(aeif_cond_alpha, NI)
     noise = sim.create(poisson_generator, rate=rate * u.Hz)

@@ -30,11 +30,20 @@ def _rng(seed):
 
 class ConnRule:
     """Base class: ``sample(n_pre, n_post, *, seed, pre_is_post, allow_autapses,
-    allow_multapses) -> ConnSpec`` of segment-local indices."""
+    allow_multapses) -> ConnSpec`` of segment-local indices. Random rules with
+    a device sampler also implement ``sample_device(...)`` returning
+    :class:`~brainpy.state_amd.device_conn.DeviceEdges` of population-local
+    indices (``pre_map`` / ``post_map`` translate the segment-local ones)."""
     __module__ = 'brainpy.state_amd'
+    random = False           # draws random numbers (deterministic rules build on the host)
 
     def sample(self, n_pre, n_post, *, seed, pre_is_post, allow_autapses, allow_multapses):
         raise NotImplementedError
+
+    def sample_device(self, n_pre, n_post, *, seed, pre_is_post, allow_autapses, allow_multapses,
+                      pre_map=None, post_map=None):
+        raise NotImplementedError(f'{type(self).__name__.lstrip("_")} has no device sampler; '
+                                  "use Simulator(connectivity='host')")
 
 
 class _AllToAll(ConnRule):
@@ -57,6 +66,7 @@ class _OneToOne(ConnRule):
 
 class _FixedIndegree(ConnRule):
     """Each post neuron receives exactly ``K`` edges (connectivity.py:101-140)."""
+    random = True
 
     def __init__(self, K: int):
         self.K = int(K)
@@ -86,9 +96,24 @@ class _FixedIndegree(ConnRule):
             pre = pre + (pre >= np.arange(n_post)[:, None])
         return ConnSpec(pre.reshape(-1).astype(np.int32), post, int(K * n_post))
 
+    def sample_device(self, n_pre, n_post, *, seed, pre_is_post, allow_autapses, allow_multapses,
+                      pre_map=None, post_map=None):
+        from .device_conn import sample_fixed_indegree
+        if self.K < 0:
+            raise ValueError(f'K must be >= 0, got {self.K}')
+        if not allow_multapses:
+            raise NotImplementedError('the device fixed_indegree sampler draws with replacement; '
+                                      "use allow_multapses=True or Simulator(connectivity='host')")
+        excl = pre_is_post and not allow_autapses
+        if self.K > 0 and n_post > 0 and (n_pre - 1 if excl else n_pre) <= 0:
+            raise ValueError('no candidate presynaptic neurons')
+        return sample_fixed_indegree(seed, n_pre, n_post, self.K, exclude_self=excl,
+                                     pre_map=pre_map, post_map=post_map)
+
 
 class _PairwiseBernoulli(ConnRule):
     """Each ordered pair connected with probability ``p`` (connectivity.py:81-99)."""
+    random = True
 
     def __init__(self, p: float):
         self.p = float(p)
@@ -119,9 +144,17 @@ class _PairwiseBernoulli(ConnRule):
             pre, post = pre[keep], post[keep]
         return ConnSpec(pre, post, int(pre.shape[0]))
 
+    def sample_device(self, n_pre, n_post, *, seed, pre_is_post, allow_autapses, allow_multapses,
+                      pre_map=None, post_map=None):
+        from .device_conn import sample_pairwise_bernoulli
+        return sample_pairwise_bernoulli(seed, n_pre, n_post, self.p,
+                                         exclude_self=pre_is_post and not allow_autapses,
+                                         pre_map=pre_map, post_map=post_map)
+
 
 class _FixedTotalNumber(ConnRule):
     """Exactly ``N`` edges drawn uniformly over the (pre, post) grid."""
+    random = True
 
     def __init__(self, N: int):
         self.N = int(N)

@@ -102,21 +102,40 @@ def seq_sum_table(w, kmax, dtype):
 
 # ---------------------------------------------------------------- projections
 class StaticProjection:
-    """Handle of a realised static projection (reference: EventProjection)."""
+    """Handle of a realised static projection (reference: EventProjection).
+
+    ``pre_idx`` / ``post_idx`` are population-local indices in the sampled
+    order. Edges sampled on the device (``device_edges``) stay there; the
+    host arrays are downloaded the first time they are read."""
     __module__ = 'brainpy.state_amd'
     plastic = False
 
-    def __init__(self, ordinal, pre_pop, post_pop, pre, post, weight, delay, receptor):
+    def __init__(self, ordinal, pre_pop, post_pop, pre, post, weight, delay, receptor, device_edges=None):
         self.ordinal = ordinal
         self.pre_pop, self.post_pop = pre_pop, post_pop
-        self.pre_idx, self.post_idx = pre, post      # population-local, sampled order
+        self._pre, self._post = pre, post
+        self.device_edges = device_edges
         self.weight = weight                         # float or (E,) array
         self.delay = delay                           # int steps or (E,) int array
         self.receptor = receptor
 
     @property
+    def pre_idx(self):
+        if self._pre is None:
+            self._pre, self._post = self.device_edges.host()
+        return self._pre
+
+    @property
+    def post_idx(self):
+        if self._post is None:
+            self._pre, self._post = self.device_edges.host()
+        return self._post
+
+    @property
     def n_edges(self):
-        return int(self.pre_idx.shape[0])
+        if self._pre is None:
+            return self.device_edges.n_edges
+        return int(self._pre.shape[0])
 
     @property
     def homogeneous(self):
@@ -226,10 +245,15 @@ class Simulator:
     tile_size : int
         Post neurons per block of the window kernel: 64, 128 or 256 (0 =
         automatic). Results do not depend on it; ``Ensemble.autotune`` picks it.
+    connectivity : {'host', 'device'}
+        Where the random rules (fixed_indegree, pairwise_bernoulli) draw their
+        edges: the host's NumPy streams, or the engine's counter-based device
+        samplers, whose edges stay in device memory through engine creation.
+        The two give different (equally distributed) networks for one seed.
     """
     __module__ = 'brainpy.state_amd'
 
-    def __init__(self, *, dt, dtype='float64', max_window=0, tile_size=0):
+    def __init__(self, *, dt, dtype='float64', max_window=0, tile_size=0, connectivity='host'):
         self._dt_q = dt
         self.dt = u.scalar(dt, 'time')
         if self.dt <= 0:
@@ -244,6 +268,9 @@ class Simulator:
         if int(tile_size) not in (0, 64, 128, 256):
             raise ValueError('tile_size must be 0, 64, 128 or 256')
         self.tile_size = int(tile_size)
+        if connectivity not in ('host', 'device'):
+            raise ValueError(f"connectivity must be 'host' or 'device', got {connectivity!r}")
+        self.connectivity = connectivity
         self._pops = []              # neuron populations, creation order
         self._taps = {}              # id(spike_recorder) -> (pop, idx)
         self._analog_taps = {}       # id(multimeter) -> (pop, idx, names)
@@ -396,17 +423,27 @@ class Simulator:
         pre_sel = np.asarray(pre_seg.indices, dtype=np.int32)
         n_pre, n_post = len(pre_sel), len(post_idx)
         dseed = self._derive_seed(seed, ordinal)
+        dev = None
         if isinstance(rule, _OneToOne):
             spec = rule.sample(n_pre, n_post, seed=dseed, pre_is_post=False,
                                allow_autapses=True, allow_multapses=True)
         else:
             if not isinstance(rule, ConnRule):
                 raise TypeError(f'rule must be a connection rule, got {rule!r}')
-            spec = rule.sample(n_pre, n_post, seed=dseed, pre_is_post=(pre_pop is post_pop),
-                               allow_autapses=allow_autapses, allow_multapses=allow_multapses)
-        pre_l = _map_local(pre_sel, spec.pre_idx)
-        post_l = _map_local(post_idx, spec.post_idx)
-        E = int(spec.n_edges)
+            if self.connectivity == 'device' and rule.random:
+                dev = rule.sample_device(n_pre, n_post, seed=dseed, pre_is_post=(pre_pop is post_pop),
+                                         allow_autapses=allow_autapses, allow_multapses=allow_multapses,
+                                         pre_map=pre_sel, post_map=post_idx)
+            else:
+                spec = rule.sample(n_pre, n_post, seed=dseed, pre_is_post=(pre_pop is post_pop),
+                                   allow_autapses=allow_autapses, allow_multapses=allow_multapses)
+        if dev is None:
+            pre_l = _map_local(pre_sel, spec.pre_idx)
+            post_l = _map_local(post_idx, spec.post_idx)
+            E = int(spec.n_edges)
+        else:
+            pre_l = post_l = None            # population-local already, in device memory
+            E = dev.n_edges
         # weights (resolve_param, connectivity.py:385-405)
         if synapse is not None:
             if not isinstance(synapse, stdp_synapse):
@@ -434,15 +471,20 @@ class Simulator:
             d = delay_to_steps(delay, self.dt)
             if np.ndim(d) == 1:
                 if d.shape[0] == _flat_size(pre_pop) and d.shape[0] != E:
+                    if pre_l is None:
+                        pre_l, post_l = dev.host()
                     d = d[pre_l]             # axonal: one delay per pre neuron
                 elif d.shape[0] != E:
                     raise ValueError('delay array must have one entry per pre neuron or per edge')
         if synapse is not None:
+            if pre_l is None:                # plastic projections keep host edge arrays
+                pre_l, post_l = dev.host()
             pr = PlasticProjection(ordinal, pre_pop, post_pop, pre_l, post_l,
                                    np.broadcast_to(np.asarray(w, np.float64), (E,)).copy(), d,
                                    receptor, rule=synapse)
         else:
-            pr = StaticProjection(ordinal, pre_pop, post_pop, pre_l, post_l, w, d, receptor)
+            pr = StaticProjection(ordinal, pre_pop, post_pop, pre_l, post_l, w, d, receptor,
+                                  device_edges=dev)
         self._records.append(pr)
         return pr
 
@@ -460,9 +502,11 @@ class Simulator:
         self._engine = None
         self._rollout_ready = False
 
-    def build_desc(self):
-        """Flatten the network into the C-ABI descriptor (``bs_net_desc``)."""
-        return _NetBuilder(self).build()
+    def build_desc(self, *, host_edges=False):
+        """Flatten the network into the C-ABI descriptor (``bs_net_desc``).
+        Edges sampled on the device are passed as device pointers unless
+        ``host_edges`` (a descriptor any host reader can consume)."""
+        return _NetBuilder(self, host_edges=host_edges).build()
 
     # -- run ---------------------------------------------------------------
     def _ensure_engine(self):
@@ -607,16 +651,23 @@ class _Desc:
         self.tap_cols = {}
         self.n_taps = 0
         self.proj_index = {}
+        self.device_edges = False        # some projection passes device pointers
 
     def arr(self, a, ctype):
         a = np.ascontiguousarray(a)
         self.keep.append(a)
         return a.ctypes.data_as(C.POINTER(ctype))
 
+    def dptr(self, t, ctype):
+        """A device tensor's address as a ctypes pointer (the tensor is kept)."""
+        self.keep.append(t)
+        return C.cast(C.c_void_p(t.data_ptr()), C.POINTER(ctype))
+
 
 class _NetBuilder:
-    def __init__(self, sim: Simulator):
+    def __init__(self, sim: Simulator, host_edges=False):
         self.sim = sim
+        self.host_edges = host_edges
 
     def build(self) -> _Desc:
         sim = self.sim
@@ -711,11 +762,16 @@ class _NetBuilder:
                 else:
                     pre_p = D.pop_index[id(r.pre_pop)]
                     E = r.n_edges
-                    indeg = np.bincount(r.post_idx, minlength=pop.n) if E else np.zeros(1, np.int64)
+                    dev = None if (self.host_edges or r.plastic) else getattr(r, 'device_edges', None)
                     if r.homogeneous and not r.plastic:
                         ch.kind = L.BS_CH_COUNT
                         ch.weight = float(r.weight)
-                        kmax = int(indeg.max()) if E else 0
+                        if not E:
+                            kmax = 0
+                        elif dev is not None:
+                            kmax = dev.max_indegree
+                        else:
+                            kmax = int(np.bincount(r.post_idx, minlength=pop.n).max())
                         if kmax > 65535:
                             raise NotImplementedError('more than 65535 edges into one neuron '
                                                       'from one projection')
@@ -738,18 +794,35 @@ class _NetBuilder:
                     pj.channel = c
                     pj.plastic = 1 if r.plastic else 0
                     pj.n_edges = E
-                    pj.pre = D.arr(np.asarray(r.pre_idx, dtype=np.int32), C.c_int32)
-                    pj.post = D.arr(np.asarray(r.post_idx, dtype=np.int32), C.c_int32)
-                    if np.ndim(r.delay) == 0:
-                        pj.delay = None
-                        pj.delay_steps = int(r.delay)
+                    if dev is not None:
+                        # edges stay in device memory; per-edge delays and
+                        # weights follow them there (bs_projection.on_device)
+                        import torch
+                        pj.on_device = 1
+                        pj.pre = D.dptr(dev.pre, C.c_int32)
+                        pj.post = D.dptr(dev.post, C.c_int32)
+                        up = lambda a, dt_: torch.from_numpy(np.ascontiguousarray(a, dtype=dt_)).cuda()  # noqa: E731
+                        if np.ndim(r.delay) == 0:
+                            pj.delay = None
+                            pj.delay_steps = int(r.delay)
+                        else:
+                            pj.delay = D.dptr(up(r.delay, np.int32), C.c_int32)
+                            pj.delay_steps = 0
+                        pj.weight = None if np.ndim(r.weight) == 0 else D.dptr(up(r.weight, np.float64), C.c_double)
+                        D.device_edges = True
                     else:
-                        pj.delay = D.arr(np.asarray(r.delay, np.int32), C.c_int32)
-                        pj.delay_steps = 0
-                    if np.ndim(r.weight) != 0:
-                        pj.weight = D.arr(np.asarray(r.weight, np.float64), C.c_double)
-                    else:
-                        pj.weight = None
+                        pj.pre = D.arr(np.asarray(r.pre_idx, dtype=np.int32), C.c_int32)
+                        pj.post = D.arr(np.asarray(r.post_idx, dtype=np.int32), C.c_int32)
+                        if np.ndim(r.delay) == 0:
+                            pj.delay = None
+                            pj.delay_steps = int(r.delay)
+                        else:
+                            pj.delay = D.arr(np.asarray(r.delay, np.int32), C.c_int32)
+                            pj.delay_steps = 0
+                        if np.ndim(r.weight) != 0:
+                            pj.weight = D.arr(np.asarray(r.weight, np.float64), C.c_double)
+                        else:
+                            pj.weight = None
                     if r.plastic:
                         s = r.rule
                         pj.stdp.tau_plus, pj.stdp.tau_minus = s.tau_plus, s.tau_minus
@@ -783,6 +856,9 @@ class _NetBuilder:
         D.table_off_np = np.asarray(table_off or [-1], dtype=np.int64)
         D.c = d
         D.ntot = gid
+        if D.device_edges:
+            import torch
+            torch.cuda.synchronize()     # the uploads above are complete before the engine reads them
         return D
 
 

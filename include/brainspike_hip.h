@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define BS_ABI_VERSION 3
+#define BS_ABI_VERSION 4
 
 /* ---- return codes ------------------------------------------------------ */
 #define BS_OK 0
@@ -125,7 +125,9 @@ typedef struct bs_projection {
   const int32_t* post;       /* [n_edges] population-local post index            */
   const int32_t* delay;      /* [n_edges] delay in steps, or NULL                 */
   int32_t delay_steps;       /* homogeneous delay in steps when delay == NULL    */
-  int32_t _pad;
+  int32_t on_device;         /* 1: pre, post, delay and weight are device
+                                pointers (static projections only; e.g. from
+                                bs_conn_*), read in place by bs_net_create      */
   const double* weight;      /* [n_edges] initial weights (FIXED/plastic) or NULL */
   bs_stdp stdp;
 } bs_projection;
@@ -184,6 +186,36 @@ int bs_net_last_timing(bs_net* net, double* deliver_ms, double* update_ms,
 /* Algorithmic bytes moved by the last run (roofline numerators). */
 int bs_net_last_bytes(bs_net* net, double* deliver_bytes, double* update_bytes,
                       int64_t* n_events);
+
+/* ---- connectivity sampled on the device ------------------------------------
+ *
+ * Replaces the random rule samplers sample_fixed_indegree (with multapses) and
+ * sample_pairwise_bernoulli (_nest_network/connectivity.py:81-140), which the
+ * reference runs as a host loop of jax.random calls. Each edge is a pure
+ * function of the seed and its position: key = (seed & 0xFFFFFFFF,
+ * 0xE36F6E6E ^ (seed >> 32)); Threefry-2x32-20.
+ *   fixed in-degree: post j, slot k < K draws (x0, x1) = threefry(key, (j, k)),
+ *     c = floor((x1 * 2^32 + x0) * n_cand / 2^64); with exclude_self
+ *     n_cand = n_pre - 1 and pre = c + (c >= j), else n_cand = n_pre, pre = c.
+ *     Edge e = j * K + k (post-major, the reference's order).
+ *   pairwise Bernoulli: pair (i, j) is connected iff x0 of threefry(key, (i, j))
+ *     is below ceil(p * 2^32) (the uniform x0 * 2^-32 < p), i == j skipped with
+ *     exclude_self; edges in row-major (pre, post) order.
+ * pre_map / post_map (or NULL) translate the sampled segment-local indices to
+ * population-local ones. All arrays are DEVICE pointers on the current device;
+ * the calls return after the device work is done. */
+int bs_conn_fixed_indegree(int64_t seed, int32_t n_pre, int32_t n_post, int32_t K, int32_t exclude_self,
+                           const int32_t* pre_map, const int32_t* post_map, int32_t* pre, int32_t* post);
+/* Count pass: row_ptr (device, n_pre + 1 entries) receives the exclusive prefix
+ * of the per-row edge counts; n_edges and max_indegree (largest number of
+ * edges into one post) are host outputs. */
+int bs_conn_bernoulli_count(int64_t seed, int32_t n_pre, int32_t n_post, double p, int32_t exclude_self,
+                            int64_t* row_ptr, int64_t* n_edges, int32_t* max_indegree);
+/* Fill pass: the edges of the count pass with the same arguments, written
+ * from row_ptr into pre / post (n_edges entries each). */
+int bs_conn_bernoulli_fill(int64_t seed, int32_t n_pre, int32_t n_post, double p, int32_t exclude_self,
+                           const int64_t* row_ptr, const int32_t* pre_map, const int32_t* post_map,
+                           int32_t* pre, int32_t* post);
 
 /* ---- surrogate-gradient BPTT of a recurrent iaf_psc_exp network ----------
  *

@@ -28,7 +28,11 @@ tests/test_stdp_oracle.py, tests/test_bptt_oracle.py, tests/test_dc_generator.py
 * stdp_synapse -- stdp_synapse_rule_test.py:65-131 known answers and a
   closed-form two-neuron trajectory;
 * bptt_ref -- DC / refractory / PSC closed forms for the forward, central
-  finite differences and a closed-form surrogate gradient for autograd.
+  finite differences and a closed-form surrogate gradient for autograd;
+* connectivity samplers (tests/test_conn_oracle.py) -- Python big-integer
+  restatements of the index map on tiny grids (on the KAT-pinned Threefry),
+  exact in-degree, self-exclusion, row-major order, p = 0 / p = 1 grids, and
+  chi-square / binomial bounds on the index and edge-count distributions.
 
 Parity unpinned: the reference's JAX random streams (Poisson counts,
 connectivity), XLA's exp, brainevent's float accumulation order for per-edge
@@ -83,6 +87,10 @@ def lib():
             'orc_log': (C.c_double, [C.c_double]),
             'orc_pow': (C.c_double, [C.c_double, C.c_double]),
             'orc_stdp_edge': (C.c_double, [C.c_double] * 5 + [C.c_void_p]),
+            'orc_conn_fixed_indegree': (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                                  C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+            'orc_conn_bernoulli': (C.c_int64, [C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_int32,
+                                               C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int64]),
         }
         for k, (res, args) in sig.items():
             f = getattr(h, k)
@@ -117,6 +125,34 @@ def log(x):
 
 def pow(x, y):  # noqa: A001
     return lib().orc_pow(float(x), float(y))
+
+
+def conn_fixed_indegree(seed, n_pre, n_post, K, exclude_self=False):
+    """The device fixed in-degree sampler's edges ``(pre, post)`` (int32),
+    post-major."""
+    E = int(n_post) * int(K)
+    pre = np.zeros(max(1, E), np.int32)
+    post = np.zeros(max(1, E), np.int32)
+    P = C.POINTER(C.c_int32)
+    _check(lib().orc_conn_fixed_indegree(int(seed), int(n_pre), int(n_post), int(K), int(bool(exclude_self)),
+                                         pre.ctypes.data_as(P), post.ctypes.data_as(P)))
+    return pre[:E], post[:E]
+
+
+def conn_bernoulli(seed, n_pre, n_post, p, exclude_self=False):
+    """The device pairwise-Bernoulli sampler's edges ``(pre, post)`` (int32),
+    row-major."""
+    P = C.POINTER(C.c_int32)
+    dummy = np.zeros(1, np.int32)
+    n = lib().orc_conn_bernoulli(int(seed), int(n_pre), int(n_post), float(p), int(bool(exclude_self)),
+                                 dummy.ctypes.data_as(P), dummy.ctypes.data_as(P), 0)
+    if n < 0:
+        _check(n)
+    pre = np.zeros(max(1, n), np.int32)
+    post = np.zeros(max(1, n), np.int32)
+    lib().orc_conn_bernoulli(int(seed), int(n_pre), int(n_post), float(p), int(bool(exclude_self)),
+                             pre.ctypes.data_as(P), post.ctypes.data_as(P), n)
+    return pre[:n], post[:n]
 
 
 def stdp_edge(w, pre_spike, post_spike, pre_trace, post_trace, Wmax=100., lambda_=0.01,

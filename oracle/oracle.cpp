@@ -203,6 +203,69 @@ struct orc_net {
   std::vector<std::pair<int, int>> taps;    // (pop, local)
 };
 
+// ------------------------------------------------------- connectivity
+// The device samplers' definition (bs_conn.hip), restated with plain integer
+// arithmetic. Reference rules: sample_fixed_indegree with multapses
+// (connectivity.py:101-136, candidates arange(0, j) ++ arange(j + 1, n_pre)
+// under self-exclusion, :117-121) and sample_pairwise_bernoulli
+// (connectivity.py:81-99: uniform < p over the pair grid, the diagonal masked,
+// edges in row-major order). Random bits: Threefry-2x32-20 with key
+// (seed & 0xFFFFFFFF, 0xE36F6E6E ^ (seed >> 32)).
+static void conn_key(int64_t seed, uint32_t* k0, uint32_t* k1) {
+  *k0 = (uint32_t)((uint64_t)seed & 0xFFFFFFFFull);
+  *k1 = 0xE36F6E6Eu ^ (uint32_t)((uint64_t)seed >> 32);
+}
+
+// Post j, slot k: c = floor((x1 2^32 + x0) n_cand / 2^64), shifted past j
+// under self-exclusion. Edge j * K + k.
+extern "C" int orc_conn_fixed_indegree(int64_t seed, int32_t n_pre, int32_t n_post, int32_t K,
+                                       int32_t exclude_self, int32_t* pre, int32_t* post) {
+  if (n_pre < 0 || n_post < 0 || K < 0) return fail(BS_EINVAL, "orc_conn_fixed_indegree: negative size");
+  const int64_t n_cand = exclude_self ? (int64_t)n_pre - 1 : (int64_t)n_pre;
+  if ((int64_t)n_post * K > 0 && n_cand <= 0) return fail(BS_EINVAL, "no candidate presynaptic neurons");
+  uint32_t k0, k1;
+  conn_key(seed, &k0, &k1);
+  for (int64_t j = 0; j < n_post; ++j)
+    for (int64_t k = 0; k < K; ++k) {
+      uint32_t x0, x1;
+      orc_threefry2x32(k0, k1, (uint32_t)j, (uint32_t)k, &x0, &x1);
+      const unsigned __int128 r = ((unsigned __int128)(((uint64_t)x1 << 32) | x0)) * (uint64_t)n_cand;
+      uint32_t c = (uint32_t)(uint64_t)(r >> 64);
+      if (exclude_self && c >= (uint32_t)j) c += 1;
+      pre[j * K + k] = (int32_t)c;
+      post[j * K + k] = (int32_t)j;
+    }
+  return BS_OK;
+}
+
+// Pair (i, j) is an edge iff x0 < ceil(p 2^32) (x0 * 2^-32 < p), i == j
+// skipped under self-exclusion; row-major order. Writes at most cap edges and
+// returns the number found (negative on a bad argument).
+extern "C" int64_t orc_conn_bernoulli(int64_t seed, int32_t n_pre, int32_t n_post, double p,
+                                      int32_t exclude_self, int32_t* pre, int32_t* post, int64_t cap) {
+  if (n_pre < 0 || n_post < 0 || !(p >= 0.0)) return fail(BS_EINVAL, "orc_conn_bernoulli: bad sizes or p");
+  uint64_t thr = 0;
+  if (p >= 1.0) thr = 1ull << 32;
+  else if (p > 0.0) thr = (uint64_t)ceil(p * 4294967296.0);
+  uint32_t k0, k1;
+  conn_key(seed, &k0, &k1);
+  int64_t n = 0;
+  for (int64_t i = 0; i < n_pre; ++i)
+    for (int64_t j = 0; j < n_post; ++j) {
+      if (exclude_self && i == j) continue;
+      uint32_t x0, x1;
+      orc_threefry2x32(k0, k1, (uint32_t)i, (uint32_t)j, &x0, &x1);
+      if ((uint64_t)x0 < thr) {
+        if (n < cap) {
+          pre[n] = (int32_t)i;
+          post[n] = (int32_t)j;
+        }
+        ++n;
+      }
+    }
+  return n;
+}
+
 static int nstate_of(int model) {
   switch (model) {
     case BS_IAF_PSC_DELTA: return 2;
@@ -216,6 +279,8 @@ static int nstate_of(int model) {
 extern "C" int orc_create(const bs_net_desc* d, orc_net** out) {
   if (!d || !out) return fail(BS_EINVAL, "orc_create: null argument");
   if (d->abi_version != BS_ABI_VERSION) return fail(BS_EINVAL, "orc_create: ABI mismatch");
+  for (int q = 0; q < d->n_projs; ++q)
+    if (d->projs[q].on_device) return fail(BS_EINVAL, "orc_create: the oracle reads host edge arrays only");
   orc_net* net = new orc_net();
   net->dtype = d->dtype;
   net->dt = d->dt_ms;

@@ -56,6 +56,11 @@ uint32_t orc_poisson(uint32_t seed, uint32_t step, uint32_t elem, double lam);
 double orc_exp(double x);
 double orc_log(double x);
 double orc_pow(double x, double y);
+/* connectivity samplers of the device path (definition in bs_conn.hip) */
+int orc_conn_fixed_indegree(int64_t seed, int32_t n_pre, int32_t n_post, int32_t K, int32_t exclude_self,
+                            int32_t* pre, int32_t* post);
+int64_t orc_conn_bernoulli(int64_t seed, int32_t n_pre, int32_t n_post, double p, int32_t exclude_self,
+                           int32_t* pre, int32_t* post, int64_t cap);
 
 #ifdef __cplusplus
 }

@@ -51,3 +51,18 @@ def test_autotune_returns_a_timed_candidate():
     best, table = Ensemble.autotune(sims, candidates=[(64, 0), (256, 0), (128, 4)], steps=100)
     assert best in table and len(table) == 3
     assert all(v > 0 for v in table.values())
+
+
+def test_device_connectivity_ensemble_equals_plain_runs():
+    """Networks whose edges were sampled on the device: one ensemble engine
+    built from the device arrays equals each network's own plain run."""
+    sims, _ = brunel_exp_ensemble(5, n_min=300, n_max=1200, seed=11, connectivity='device')
+    ens = Ensemble(sims)
+    parts = ens.run(1500)
+    ens.close()
+    total = 0
+    for sim, (vm, g, s) in zip(sims, parts):
+        g0, s0 = _plain(sim, 1500)
+        assert np.array_equal(g, g0) and np.array_equal(s, s0)
+        total += len(g)
+    assert total > 1000
