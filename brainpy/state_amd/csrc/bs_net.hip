@@ -1248,9 +1248,10 @@ __global__ void k_seg_keys(const int32_t* __restrict__ pre, const int32_t* __res
                            const int32_t* __restrict__ delay, int delay_steps, long long E, int npre, int npost,
                            int TS, int nt, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
                            int* __restrict__ stats) {
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  // grid-stride; with per-edge delays one min/max atomic pair per block
   int dmn = 0x7FFFFFFF, dmx = -1;
-  if (e < E) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += stride) {
     const int a = pre[e], b = post[e];
     const int dd = delay ? delay[e] : delay_steps;
     uint32_t key = 0;
@@ -1260,18 +1261,30 @@ __global__ void k_seg_keys(const int32_t* __restrict__ pre, const int32_t* __res
       atomicOr(&stats[0], 2);
     } else {
       key = (uint32_t)a * (uint32_t)nt + (uint32_t)(b / TS);
-      dmn = dmx = dd;
+      dmn = min(dmn, dd);
+      dmx = max(dmx, dd);
     }
     keys[e] = key;
     vals[e] = (uint32_t)e;
   }
+  if (!delay) return;  // a homogeneous delay is known on the host
+  __shared__ int red[2][4];
   for (int o = 32; o > 0; o >>= 1) {
     dmn = min(dmn, __shfl_xor(dmn, o, 64));
     dmx = max(dmx, __shfl_xor(dmx, o, 64));
   }
-  if ((threadIdx.x & 63) == 0 && dmx >= 0) {
-    atomicMin(&stats[1], dmn);
-    atomicMax(&stats[2], dmx);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = dmn;
+    red[1][threadIdx.x >> 6] = dmx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int mn = min(min(red[0][0], red[0][1]), min(red[0][2], red[0][3]));
+    const int mx = max(max(red[1][0], red[1][1]), max(red[1][2], red[1][3]));
+    if (mx >= 0) {
+      atomicMin(&stats[1], mn);
+      atomicMax(&stats[2], mx);
+    }
   }
 }
 
@@ -1378,7 +1391,8 @@ static int build_segments_device(bs_net* net, const bs_projection& pj, int kind,
   int bits = 1;
   while (bits < 32 && (1ull << bits) < (unsigned long long)npre * nt) ++bits;
   if (E > 0) {
-    hipLaunchKernelGGL(k_seg_keys, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, st, pj.pre, pj.post, pj.delay,
+    const long long nb = std::min<long long>((E + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_seg_keys, dim3((unsigned)nb), dim3(256), 0, st, pj.pre, pj.post, pj.delay,
                        pj.delay_steps, E, npre, npost, TS, nt, S.kv[0], S.kv[2], S.stats);
     HIPCHK(hipGetLastError());
     size_t tb = S.tmp_bytes;
@@ -1416,6 +1430,8 @@ static int build_segments_device(bs_net* net, const bs_projection& pj, int kind,
   for (int a = 0; a < npre; ++a) net->outdeg[g0 + a] += deg[a];
   if (E == 0) {
     pdmin = pdmax = 0;
+  } else if (!pj.delay) {
+    pdmin = pdmax = pj.delay_steps;
   } else {
     pdmin = stats[1];
     pdmax = stats[2];

@@ -20,6 +20,9 @@
  *   spikes/traces   <- SimulationResult.spikes / .trace (simulator.py:306-414)
  *   weights         <- EventPlasticProj.weight State, record_weight
  *                      (event_plastic.py:305, simulator.py:884-910)
+ *   bs_conn_*       <- sample_fixed_indegree / sample_pairwise_bernoulli
+ *                      (_nest_network/connectivity.py:81-140); with
+ *                      bs_projection.on_device, csr_structure (:39-54)
  */
 #ifndef BRAINSPIKE_HIP_H
 #define BRAINSPIKE_HIP_H
