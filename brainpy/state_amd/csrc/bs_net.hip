@@ -2020,6 +2020,71 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     }
   }
 
+  // ---- launch order of the tiles. A launch with more tiles than two per CU
+  // runs in rounds. Block b runs on XCD b % 8 and k_window gives XCD x the
+  // x-th contiguous share of the tile array, so the shares are filled here
+  // with runs of up to 16 tiles of one population, heaviest first, each run
+  // on the XCD with the least estimated work so far: the XCDs finish
+  // together and their last blocks are light ones. Estimated work of a tile:
+  // its neurons times (64 + the population's in-degree). The order changes no
+  // result (every sum is order independent).
+  {
+    int n_cu = 0;
+    HCHK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, net->device));
+    const int G = (int)net->tiles.size();
+    if (G > 2 * n_cu) {
+      std::vector<double> indeg(net->pops.size(), 0.0);
+      for (int q = 0; q < d->n_projs; ++q) {
+        const bs_projection& pj = d->projs[q];
+        const int npost = net->pops[pj.post_pop].n;
+        if (npost > 0) indeg[pj.post_pop] += (double)pj.n_edges / npost;
+      }
+      struct Run {
+        int first, len;
+        double w;  // per tile
+      };
+      std::vector<Run> runs;
+      for (int t = 0; t < G;) {
+        const int p = net->tiles[t].pop;
+        int e = t;
+        while (e < G && e - t < 16 && net->tiles[e].pop == p) ++e;
+        runs.push_back(Run{t, e - t, (double)net->tiles[t].count * (64.0 + indeg[p])});
+        t = e;
+      }
+      std::stable_sort(runs.begin(), runs.end(), [](const Run& x, const Run& y) { return x.w > y.w; });
+      const int q8 = G / 8, rem = G % 8;
+      int cap[8];
+      double load[8];
+      std::vector<int> lists[8];
+      for (int x = 0; x < 8; ++x) {
+        cap[x] = q8 + (x < rem ? 1 : 0);
+        load[x] = 0.0;
+      }
+      for (const Run& r : runs) {
+        int first = r.first, left = r.len;
+        while (left > 0) {
+          int best = -1;
+          for (int x = 0; x < 8; ++x)
+            if (cap[x] > 0 && (best < 0 || load[x] < load[best])) best = x;
+          const int take = std::min(left, cap[best]);
+          for (int k = 0; k < take; ++k) {
+            const Tile& Tk = net->tiles[first + k];
+            lists[best].push_back(first + k);
+            load[best] += (double)Tk.count * (64.0 + indeg[Tk.pop]);
+          }
+          cap[best] -= take;
+          first += take;
+          left -= take;
+        }
+      }
+      std::vector<Tile> order;
+      order.reserve(G);
+      for (int x = 0; x < 8; ++x)
+        for (int t : lists[x]) order.push_back(net->tiles[t]);
+      net->tiles.swap(order);
+    }
+  }
+
   // ---- voltmeter columns
   std::vector<int> vm_col(gid, -1);
   int ncol = 0;
