@@ -107,7 +107,8 @@ struct DevChan {
   int cdf_len;
   int pre;             // POISSON: counts drawn ahead by producer blocks (pcnt)
   long long tmin, tmax;
-  unsigned seed, pad2;
+  unsigned seed;
+  int lds_len;         // COUNT: entries S[0..lds_len) staged in LDS (the rest read from the table)
   const int* gen_index;
   const double* cdf;             // POISSON: CDF table of Poisson(lam / parts)
   const unsigned short* guide;   // POISSON: 64-entry guide table
@@ -261,7 +262,9 @@ struct ChReg {
   unsigned seed;
   long long tmin, tmax;
   T weight;
-  const T* tab;                 // count table S[0..max_count] (LDS or global)
+  const T* tab;                 // count table S[0..tab_n) (LDS, or global when not staged)
+  const T* tabg;                // the whole count table in global memory
+  int tab_n;
   const double* cdf;            // Poisson CDF (LDS or global)
   const unsigned short* guide;  // Poisson guide table
   const int* gen_index;
@@ -291,6 +294,8 @@ __device__ __forceinline__ ChReg<T> load_ch(const DevChan& ch, const unsigned ch
   c.pacc = ch.pacc;
   const bool staged = ch.lds_off >= 0;
   c.tab = staged ? (const T*)(tab + ch.lds_off) : tables + ch.table_off;
+  c.tabg = tables + ch.table_off;
+  c.tab_n = staged ? ch.lds_len : ch.max_count + 1;
   c.cdf = staged ? (const double*)(tab + ch.lds_off) : ch.cdf;
   c.guide = staged ? (const unsigned short*)((const double*)(tab + ch.lds_off) + ch.cdf_len)
                    : ch.guide;
@@ -326,7 +331,7 @@ __device__ __forceinline__ T chan_value(const UpdCtx<T>& u, const ChReg<T>& c, i
       atomicOr(u.a.flags, FLAG_COUNT);
       k = (uint32_t)c.max_count;
     }
-    return c.tab[k];
+    return k < (uint32_t)c.tab_n ? c.tab[k] : c.tabg[k];
   }
   if (c.kind == BS_CH_POISSON) {
     if (c.pre) return (T)u.pc[w * u.TS + lane] * c.weight;
@@ -1213,6 +1218,7 @@ static int upload(bs_net* net, X** d, const X* h, size_t n) {
 
 static int upload_state(bs_net* net);
 static const size_t kLdsBudget = 64 * 1024;
+static const int kStageCount = 256;  // count-table entries staged in LDS per channel
 
 // CDF table of Poisson(mu), mu = lam / parts, parts = max(1, ceil(lam / 32)):
 // pmf_0 = exp(-mu), pmf_k = pmf_{k-1} * mu / k, cdf_k = cdf_{k-1} + pmf_k,
@@ -1935,7 +1941,12 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
         DevChan& dc = net->chans[c];
         dc.lds_off = -1;
         size_t sz = 0;
-        if (dc.kind == BS_CH_COUNT) sz = al8((size_t)(dc.max_count + 1) * net->tsize);
+        // a count table is staged up to S[255]: per step and neuron larger
+        // counts are rare, and those read the table in global memory
+        if (dc.kind == BS_CH_COUNT) {
+          dc.lds_len = std::min(dc.max_count + 1, kStageCount);
+          sz = al8((size_t)dc.lds_len * net->tsize);
+        }
         else if (dc.kind == BS_CH_POISSON) sz = al8((size_t)dc.cdf_len * 8 + 64 * 2);
         if (sz && base_end + off + sz + 16 <= kLdsBudget) {
           dc.lds_off = (long long)off;
@@ -1986,7 +1997,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
         if (ch.lds_off < 0) continue;
         unsigned char* dst = &B[o + L.tab + (size_t)ch.lds_off];
         if (ch.kind == BS_CH_COUNT) {
-          for (int k = 0; k <= ch.max_count; ++k) {
+          for (int k = 0; k < ch.lds_len; ++k) {
             const double v = d->tables[ch.table_off + k];
             if (net->tsize == 8) memcpy(dst + 8 * k, &v, 8);
             else {
