@@ -17,6 +17,7 @@ from .rules import (all_to_all, explicit_edges, fixed_indegree, fixed_total_numb
 from .simulator import SimulationResult, Simulator
 from .ensemble import Ensemble
 from .connections import SynapseCollection
+from .shards import Shards
 
 __version__ = '0.1.0'
 
@@ -24,4 +25,4 @@ __all__ = ['Simulator', 'SimulationResult', 'NodeView', 'all_to_all', 'one_to_on
            'fixed_indegree', 'pairwise_bernoulli', 'fixed_total_number', 'explicit_edges',
            'iaf_psc_delta', 'iaf_psc_exp', 'iaf_psc_alpha', 'aeif_cond_alpha',
            'poisson_generator', 'dc_generator', 'spike_recorder', 'voltmeter', 'multimeter', 'stdp_synapse',
-           'units', 'init', 'dist', 'Ensemble', 'SynapseCollection']
+           'units', 'init', 'dist', 'Ensemble', 'SynapseCollection', 'Shards']

@@ -15,7 +15,7 @@ __all__ = ['lib', 'EngineUnavailable', 'EngineError', 'check', 'BS_NPARAM', 'BS_
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libbrainspike_hip.so')
 
-BS_ABI_VERSION = 4
+BS_ABI_VERSION = 5
 BS_NPARAM = 40
 BS_NSTATE = 9
 BS_F64, BS_F32 = 0, 1
@@ -43,7 +43,7 @@ class Population(C.Structure):
                 ('state0', C.POINTER(C.c_double)), ('ch_begin', C.c_int32),
                 ('ch_count', C.c_int32), ('n_vm', C.c_int32),
                 ('vm_idx', C.POINTER(C.c_int32)), ('record_spikes', C.c_int32),
-                ('_pad', C.c_int32)]
+                ('remote', C.c_int32)]
 
 
 class Channel(C.Structure):
@@ -110,6 +110,8 @@ _SIGS = {
     'bs_bptt_forward': (C.c_int, [_P, _P, _P, _P, _P, _P, C.POINTER(C.c_double)]),
     'bs_bptt_backward': (C.c_int, [_P, _P, _P]),
     'bs_bptt_spikes': (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_float)]),
+    'bs_net_window': (C.c_int, [_P, C.POINTER(C.c_int32)]),
+    'bs_net_inject_spikes': (C.c_int, [_P, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     'bs_conn_fixed_indegree': (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                          _P, _P, _P, _P]),
     'bs_conn_bernoulli_count': (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_int32, _P,

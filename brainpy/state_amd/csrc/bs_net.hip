@@ -1031,6 +1031,7 @@ struct bs_net {
   std::vector<DevChan> chans;
   std::vector<Tile> tiles;
   std::vector<int> pop_record;
+  std::vector<int> pop_remote;  // 1: spikes injected by bs_net_inject_spikes
   bool all_recorded = true;  // every population keeps its spikes for the caller
   // device buffers
   Tile* d_tiles = nullptr;
@@ -1521,6 +1522,8 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     if (sp.model == BS_AEIF_COND_ALPHA && d->dtype != BS_F64)
       BAD("aeif_cond_alpha runs in float64 only");
     if (!sp.state0) BAD("population state0 must be provided");
+    if (sp.remote && (sp.ch_count != 0 || sp.n_vm != 0))
+      BAD("a remote population has no input channels and no voltmeter taps");
     dp.ch_begin = sp.ch_begin;
     dp.ch_count = sp.ch_count;
     if (sp.ch_begin < 0 || sp.ch_count < 0 || sp.ch_begin + sp.ch_count > d->n_channels)
@@ -1596,11 +1599,14 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     dp.n_cw = (ncount + 1) / 2;
     dp.n_fx = nfx;
     dp.tile0 = (int)net->tiles.size();
-    for (int s0 = 0; s0 < sp.n; s0 += TS)
-      net->tiles.push_back(Tile{p, s0, std::min(TS, sp.n - s0), 0});
+    if (!sp.remote)  // a remote population's neurons are updated by their own shard
+      for (int s0 = 0; s0 < sp.n; s0 += TS)
+        net->tiles.push_back(Tile{p, s0, std::min(TS, sp.n - s0), 0});
     gid += sp.n;
-    net->pop_record.push_back(sp.record_spikes);
-    if (!sp.record_spikes) net->all_recorded = false;
+    const int rec = sp.remote ? 0 : sp.record_spikes;
+    net->pop_record.push_back(rec);
+    net->pop_remote.push_back(sp.remote ? 1 : 0);
+    if (!rec) net->all_recorded = false;
   }
   net->Ntot = gid;
   if (gid >= (1ll << 31)) BAD("more than 2^31 neurons");
@@ -1706,6 +1712,8 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     const int kind = d->channels[pj.channel].kind;
     if (pj.plastic) {
       if (pj.on_device) BAD("a plastic projection takes host edge arrays (on_device = 0)");
+      if (d->pops[pj.pre_pop].remote || d->pops[pj.post_pop].remote)
+        BAD("a plastic projection needs both populations on this shard");
       if (kind != BS_CH_FIXED) BAD("a plastic projection needs a FIXED channel");
       if (!pj.weight) BAD("a plastic projection needs per-edge initial weights");
       if (!(pj.stdp.Wmax != 0.0) || !(pj.stdp.tau_plus > 0) || !(pj.stdp.tau_minus > 0))
@@ -2632,6 +2640,47 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
   (void)hipEventDestroy(ev_all0);
   (void)hipEventDestroy(ev_all1);
   return rc;
+}
+
+int bs_net_window(bs_net* net, int32_t* W) {
+  if (!net || !W) return fail(BS_EINVAL, "bs_net_window: null argument");
+  *W = net->W;
+  return BS_OK;
+}
+
+// A peer shard's window: its spikes become this handle's buckets of the
+// remote populations for the steps of the window just run, which the next
+// windows gather exactly like local buckets (the buckets of those steps were
+// cleared by the window before).
+int bs_net_inject_spikes(bs_net* net, int64_t n, const int32_t* gid, const int32_t* step) {
+  if (!net) return fail(BS_EINVAL, "bs_net_inject_spikes: null handle");
+  if (n < 0 || (n > 0 && (!gid || !step))) return fail(BS_EINVAL, "bs_net_inject_spikes: bad arguments");
+  if (n == 0) return BS_OK;
+  const long long lo = net->step - net->last_win, hi = net->step;
+  // group by (step, population), keeping the given order inside a group
+  std::vector<std::vector<uint32_t>> grp((size_t)std::max<long long>(1, hi - lo) * net->pops.size());
+  for (int64_t i = 0; i < n; ++i) {
+    const int g = gid[i];
+    if (g < 0 || g >= net->Ntot) return fail(BS_EINVAL, "bs_net_inject_spikes: gid out of range");
+    const int p = net->gid_pop[g];
+    if (!net->pop_remote[p]) return fail(BS_EINVAL, "bs_net_inject_spikes: spike of a population local to this shard");
+    if (step[i] < lo || step[i] >= hi)
+      return fail(BS_EINVAL, "bs_net_inject_spikes: step outside the window this handle ran last");
+    grp[(size_t)(step[i] - lo) * net->pops.size() + p].push_back((uint32_t)(g - net->pops[p].gid0));
+  }
+  HIPCHK(hipSetDevice(net->device));
+  for (long long s = lo; s < hi; ++s)
+    for (size_t p = 0; p < net->pops.size(); ++p) {
+      const std::vector<uint32_t>& v = grp[(size_t)(s - lo) * net->pops.size() + p];
+      if (v.empty()) continue;
+      if ((long long)v.size() > net->pops[p].n) return fail(BS_EINVAL, "bs_net_inject_spikes: repeated spikes");
+      const unsigned cnt = (unsigned)v.size();
+      HIPCHK(hipMemcpy(net->d_sb + (size_t)(s % net->Ls) * net->Ntot + net->pops[p].gid0, v.data(),
+                       sizeof(uint32_t) * v.size(), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(net->d_sb_count + (size_t)(s % net->Ls) * net->pops.size() + p, &cnt, sizeof(unsigned),
+                       hipMemcpyHostToDevice));
+    }
+  return BS_OK;
 }
 
 int bs_net_spike_count(bs_net* net, int64_t* n) {

@@ -665,9 +665,14 @@ class _Desc:
 
 
 class _NetBuilder:
-    def __init__(self, sim: Simulator, host_edges=False):
+    """``remote``: indices of populations whose neurons live on another shard
+    (no channels, taps or updates here; their spikes are injected), and then
+    every local population keeps its spikes (they are the exchange)."""
+
+    def __init__(self, sim: Simulator, host_edges=False, remote=()):
         self.sim = sim
         self.host_edges = host_edges
+        self.remote = frozenset(remote)
 
     def build(self) -> _Desc:
         sim = self.sim
@@ -686,7 +691,9 @@ class _NetBuilder:
         # channels per post population, in registration (ordinal) order
         per_pop = [[] for _ in range(npop)]
         for r in sim._records:
-            per_pop[D.pop_index[id(r.post_pop)]].append(r)
+            q = D.pop_index[id(r.post_pop)]
+            if q not in self.remote:          # a remote population's inputs are its own shard's
+                per_pop[q].append(r)
         chans = (L.Channel * max(1, sum(len(x) for x in per_pop)))()
         cpops = (L.Population * npop)()
         tables, table_off = [], []
@@ -697,6 +704,8 @@ class _NetBuilder:
         vm_lists = [[] for _ in range(npop)]
         tap_order = []
         for rid, (pop, idx, names) in sim._analog_taps.items():
+            if D.pop_index[id(pop)] in self.remote:
+                continue
             for name in names:
                 if pop.recordables[name] != 0:
                     raise NotImplementedError('only V_m is recorded by the engine')
@@ -710,6 +719,8 @@ class _NetBuilder:
             D.tap_cols[(rid, name)] = np.arange(pop_tap_base[p] + start, pop_tap_base[p] + start + cnt)
         D.n_taps = int(pop_tap_base[-1])
         rec_pops = {D.pop_index[id(pop)] for (pop, _) in sim._taps.values()}
+        if self.remote:
+            rec_pops = set(range(npop)) - set(self.remote)
         off = 0
         for p, pop in enumerate(pops):
             P = cpops[p]
@@ -724,6 +735,7 @@ class _NetBuilder:
             P.n_vm = len(vm_lists[p])
             P.vm_idx = D.arr(np.asarray(vm_lists[p] or [0], dtype=np.int32), C.c_int32)
             P.record_spikes = 1 if p in rec_pops else 0
+            P.remote = 1 if p in self.remote else 0
             for r in per_pop[p]:
                 ch = chans[c]
                 ch.receptor = r.receptor
@@ -890,6 +902,19 @@ class _Engine:
         L.check(self.lib.bs_net_copy_spikes(self.h, gid.ctypes.data_as(C.POINTER(C.c_int32)),
                                             st.ctypes.data_as(C.POINTER(C.c_int32)), len(gid)))
         return vm, (gid[:cnt.value], st[:cnt.value])
+
+    def window(self) -> int:
+        """Steps per window launch."""
+        w = C.c_int32()
+        L.check(self.lib.bs_net_window(self.h, C.byref(w)))
+        return int(w.value)
+
+    def inject(self, gid, step):
+        """A peer shard's spikes of the window just run (global gids)."""
+        g = np.ascontiguousarray(gid, dtype=np.int32)
+        st = np.ascontiguousarray(step, dtype=np.int32)
+        L.check(self.lib.bs_net_inject_spikes(self.h, len(g), g.ctypes.data_as(C.POINTER(C.c_int32)),
+                                              st.ctypes.data_as(C.POINTER(C.c_int32))))
 
     def timing(self):
         vals = [C.c_double() for _ in range(3)] + [C.c_int64() for _ in range(2)]

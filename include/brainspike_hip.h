@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define BS_ABI_VERSION 4
+#define BS_ABI_VERSION 5
 
 /* ---- return codes ------------------------------------------------------ */
 #define BS_OK 0
@@ -92,7 +92,10 @@ typedef struct bs_population {
   int32_t n_vm;             /* voltmeter taps: local indices vm_idx[0..n_vm)     */
   const int32_t* vm_idx;
   int32_t record_spikes;    /* 1: keep this population's spikes for the caller  */
-  int32_t _pad;
+  int32_t remote;           /* 1: the neurons live on another shard: no update,
+                               no channels, no taps; their spikes arrive through
+                               bs_net_inject_spikes and are delivered like local
+                               ones                                             */
 } bs_population;
 
 typedef struct bs_channel {
@@ -189,6 +192,23 @@ int bs_net_last_timing(bs_net* net, double* deliver_ms, double* update_ms,
 /* Algorithmic bytes moved by the last run (roofline numerators). */
 int bs_net_last_bytes(bs_net* net, double* deliver_bytes, double* update_bytes,
                       int64_t* n_events);
+/* Steps per window (one k_window launch) of this handle: a spike emitted in
+ * one window reaches no step of the same window. */
+int bs_net_window(bs_net* net, int32_t* W);
+
+/* ---- shards: the spike-list exchange between windows ----------------------
+ *
+ * Replaces the min-delay communication of a distributed NEST run, which the
+ * reference does not have (its Simulator is one process); the per-window
+ * spike list is the exchange format: after every window each shard hands the
+ * (gid, step) list of its own populations' spikes to the others, which inject
+ * the spikes of their remote populations before their next window. The
+ * shards' windows must have the same length (bs_net_window) and advance in
+ * lockstep. gid in the global numbering (the populations' order in the
+ * descriptor, remote ones included); every step in the window this handle
+ * ran last; all of a window's spikes in one call, sorted by (step, gid) as
+ * bs_net_copy_spikes returns them. */
+int bs_net_inject_spikes(bs_net* net, int64_t n, const int32_t* gid, const int32_t* step);
 
 /* ---- connectivity sampled on the device ------------------------------------
  *

@@ -281,6 +281,8 @@ extern "C" int orc_create(const bs_net_desc* d, orc_net** out) {
   if (d->abi_version != BS_ABI_VERSION) return fail(BS_EINVAL, "orc_create: ABI mismatch");
   for (int q = 0; q < d->n_projs; ++q)
     if (d->projs[q].on_device) return fail(BS_EINVAL, "orc_create: the oracle reads host edge arrays only");
+  for (int p = 0; p < d->n_pops; ++p)
+    if (d->pops[p].remote) return fail(BS_EINVAL, "orc_create: the oracle runs whole networks (no remote populations)");
   orc_net* net = new orc_net();
   net->dtype = d->dtype;
   net->dt = d->dt_ms;
