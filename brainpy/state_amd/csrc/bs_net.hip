@@ -762,12 +762,13 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
     // current inputs are summed apart from the delta inputs, starting from
     // I_e + x (x = 0) for iaf_psc_delta and from x = 0 for the latching models
     const T cur0 = gpop.model == BS_IAF_PSC_DELTA ? (T)P[34] + (T)0 : (T)0;
-    if (nit <= 256 * kIT) {
+    // chunks of 256 * kIT items, channel-outer inside a chunk
+    auto inputs_chunk = [&](const int base0) {
       T acc0[kIT], acc1[kIT], accC[kIT];
       bool ok[kIT];
 #pragma unroll
       for (int i = 0; i < kIT; ++i) {
-        const int idx = tid + 256 * i;
+        const int idx = base0 + tid + 256 * i;
         ok[i] = idx < nit && (idx & (TS - 1)) < tl.count;
         acc0[i] = (T)0;
         acc1[i] = (T)0;
@@ -778,7 +779,7 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
         T v[kIT];
 #pragma unroll
         for (int i = 0; i < kIT; ++i) {
-          const int idx = tid + 256 * i, w = idx >> lgTS, l = idx & (TS - 1);
+          const int idx = base0 + tid + 256 * i, w = idx >> lgTS, l = idx & (TS - 1);
           v[i] = ok[i] ? chan_value<T>(u, cc, w, a.n0 + w, l) : (T)0;
         }
         if (cc.kind == BS_CH_CURRENT) {
@@ -794,30 +795,15 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
       }
 #pragma unroll
       for (int i = 0; i < kIT; ++i) {
-        const int idx = tid + 256 * i, w = idx >> lgTS, l = idx & (TS - 1);
+        const int idx = base0 + tid + 256 * i, w = idx >> lgTS, l = idx & (TS - 1);
         if (ok[i]) {
           inp[w * TS + l] = acc0[i];
           if (nrec > 1) inp[(Wm + w) * TS + l] = acc1[i];
           if (tl.ncur) inp[(nrec * Wm + w) * TS + l] = accC[i];
         }
       }
-    } else {
-      for (int idx = tid; idx < nit; idx += 256) {
-        const int w = idx >> lgTS, l = idx & (TS - 1);
-        if (l >= tl.count) continue;
-        T acc0 = (T)0, acc1 = (T)0, accC = cur0;
-        for (int c = 0; c < nch; ++c) {
-          const ChReg<T> cc = load_ch<T>(chs[c], tab, tables);
-          const T v = chan_value<T>(u, cc, w, a.n0 + w, l);
-          if (cc.kind == BS_CH_CURRENT) accC = accC + v;
-          else if (cc.receptor == 0) acc0 = acc0 + v;
-          else acc1 = acc1 + v;
-        }
-        inp[w * TS + l] = acc0;
-        if (nrec > 1) inp[(Wm + w) * TS + l] = acc1;
-        if (tl.ncur) inp[(nrec * Wm + w) * TS + l] = accC;
-      }
-    }
+    };
+    for (int base0 = 0; base0 < nit; base0 += 256 * kIT) inputs_chunk(base0);
   }
   __syncthreads();
 
