@@ -484,7 +484,7 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   // (the register slots cover the common sizes; loops take the rest).
   const long long tile_gid0 = tl.gid0 + tl.start;
   const int PL = (a.base_ptr && a.widx == 0) ? *a.prevlen_ptr : a.prevlen;
-  const int lgTS = TS == 256 ? 8 : (TS == 128 ? 7 : 6);
+  const int lgTS = TS == 256 ? 8 : (TS == 128 ? 7 : (TS == 64 ? 6 : 5));
   const int lane_t = tid & (TS - 1), row0 = tid >> lgTS, rows_per = 256 >> lgTS;
   const bool lane_ok = lane_t < tl.count;
   const int L2 = 2 * Wm;
@@ -1478,9 +1478,15 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     for (int p = 0; p < d->n_pops; ++p) rkf = rkf || d->pops[p].model == BS_AEIF_COND_ALPHA;
     const long long min_tiles = rkf ? 128 : 1024;
     while (TS > 64 && ntiles(TS) < min_tiles) TS /= 2;
+    // fewer 64-neuron tiles than CUs: the window is one partial round bound by
+    // each block's chain of dependent phases, and 32-neuron tiles halve the
+    // gather and inputs work of that chain (Brunel-11250: 24.7 -> 23.5 us)
+    int n_cu = 0;
+    HCHK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, net->device));
+    if (!rkf && TS == 64 && ntiles(64) < n_cu) TS = 32;
     if (d->tile_size != 0) {
-      if (d->tile_size != 64 && d->tile_size != 128 && d->tile_size != 256)
-        BAD("bs_net_create: tile_size must be 0, 64, 128 or 256");
+      if (d->tile_size != 32 && d->tile_size != 64 && d->tile_size != 128 && d->tile_size != 256)
+        BAD("bs_net_create: tile_size must be 0, 32, 64, 128 or 256");
       TS = d->tile_size;
     }
   }
@@ -1864,14 +1870,18 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   net->d_min = dmin;
   net->d_max = dmax;
 
-  // ---- Poisson producers: when the tiles leave at least 8 CUs idle, blocks
-  // on those CUs draw each population's first Poisson channel one window
-  // ahead (counts up to 65535) and the tiles read the counts instead
+  // ---- Poisson producers: when a launch of one or two rounds of tiles leaves
+  // at least 8 block slots free, blocks in those slots draw each population's
+  // first Poisson channel one window ahead (counts up to 65535) and the tiles
+  // read the counts instead
   net->pop_pc.assign(net->pops.size(), 0);
   {
     int n_cu = 0;
     HCHK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, net->device));
-    const long long spare = (long long)n_cu - (long long)net->tiles.size();
+    // blocks beyond the tiles on the CUs of a launch's one or two rounds
+    const long long nt_ = (long long)net->tiles.size();
+    const long long slots = nt_ <= n_cu ? n_cu : (nt_ <= 2ll * n_cu ? 2ll * n_cu : nt_);
+    const long long spare = slots - nt_;
     if (spare >= 8) {
       for (size_t p = 0; p < net->pops.size(); ++p) {
         const DevPop& dp = net->pops[p];

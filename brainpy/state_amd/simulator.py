@@ -243,7 +243,7 @@ class Simulator:
     max_window : int
         Upper bound on the steps advanced per update launch (0 = automatic).
     tile_size : int
-        Post neurons per block of the window kernel: 64, 128 or 256 (0 =
+        Post neurons per block of the window kernel: 32, 64, 128 or 256 (0 =
         automatic). Results do not depend on it; ``Ensemble.autotune`` picks it.
     connectivity : {'host', 'device'}
         Where the random rules (fixed_indegree, pairwise_bernoulli) draw their
@@ -265,8 +265,8 @@ class Simulator:
         else:
             raise ValueError(f"dtype must be 'float64' or 'float32', got {dtype!r}")
         self.max_window = int(max_window)
-        if int(tile_size) not in (0, 64, 128, 256):
-            raise ValueError('tile_size must be 0, 64, 128 or 256')
+        if int(tile_size) not in (0, 32, 64, 128, 256):
+            raise ValueError('tile_size must be 0, 32, 64, 128 or 256')
         self.tile_size = int(tile_size)
         if connectivity not in ('host', 'device'):
             raise ValueError(f"connectivity must be 'host' or 'device', got {connectivity!r}")

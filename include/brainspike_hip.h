@@ -149,8 +149,9 @@ typedef struct bs_net_desc {
   const double* tables;      /* concatenated S tables of all COUNT channels      */
   const int64_t* table_off;  /* [n_channels] offset into tables, -1 if none      */
   int64_t n_tables;
-  int32_t tile_size;         /* post neurons per block: 64, 128 or 256; 0 = the
-                                largest giving >= 1024 tiles                      */
+  int32_t tile_size;         /* post neurons per block: 32, 64, 128 or 256; 0 =
+                                the largest giving >= 1024 tiles, and 32 when
+                                even 64 gives fewer tiles than the GPU has CUs   */
   int32_t _pad2;
 } bs_net_desc;
 
