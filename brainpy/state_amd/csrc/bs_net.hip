@@ -57,12 +57,13 @@
 using namespace bs;
 
 // Gather batch: spikes per thread per pass, and 16-byte edge chunks per spike
-// and round (the loads one thread has in flight at once).
+// and round (the loads one thread has in flight at once). Three chunks cover
+// most (pre, tile) segments of 32- to 128-neuron tiles in one round.
 #ifndef BS_GATHER_SPIKES
 #define BS_GATHER_SPIKES 2
 #endif
 #ifndef BS_GATHER_CHUNKS
-#define BS_GATHER_CHUNKS 5
+#define BS_GATHER_CHUNKS 3
 #endif
 
 // ------------------------------------------------------------------ errors
@@ -633,7 +634,7 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   __syncthreads();
 
   // ---- 1. gather: two spikes per thread per pass, loads in phases; a
-  // segment's edge words come in as aligned 16-byte chunks, five per round
+  // segment's edge words come in as aligned 16-byte chunks, three per round
   constexpr int kQ = BS_GATHER_SPIKES, kCH = BS_GATHER_CHUNKS;
   const int total = gtot[ngp];
   for (int b0 = 0; b0 < total; b0 += kQ * 256) {
