@@ -60,7 +60,7 @@ using namespace bs;
 // and round (the loads one thread has in flight at once). Three chunks cover
 // most (pre, tile) segments of 32- to 128-neuron tiles in one round.
 #ifndef BS_GATHER_SPIKES
-#define BS_GATHER_SPIKES 2
+#define BS_GATHER_SPIKES 1
 #endif
 #ifndef BS_GATHER_CHUNKS
 #define BS_GATHER_CHUNKS 3
@@ -633,7 +633,7 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   }
   __syncthreads();
 
-  // ---- 1. gather: two spikes per thread per pass, loads in phases; a
+  // ---- 1. gather: one spike per thread per pass, loads in phases; a
   // segment's edge words come in as aligned 16-byte chunks, three per round
   constexpr int kQ = BS_GATHER_SPIKES, kCH = BS_GATHER_CHUNKS;
   const int total = gtot[ngp];
