@@ -57,3 +57,22 @@ def test_three_shards_per_edge_delays_and_split_runs():
     assert len(g_o) > 500 and len(g) == len(g_o)
     assert np.array_equal(g, g_o) and np.array_equal(s, s_o)
     np.testing.assert_allclose(vm_c, vm_o, rtol=1e-12, atol=0)
+
+
+def test_inject_rejects_bad_spike_lists():
+    from brainpy.state_amd._lib import EngineError
+    from brainpy.state_amd.simulator import _Engine, _NetBuilder
+    sim, h = brunel_delta(order=10, n_vm=0)            # E: gids 0..39, I: 40..49
+    eng = _Engine(_NetBuilder(sim, remote=[1]).build())
+    W = eng.window()
+    eng.run(W)
+    eng.inject(np.array([40, 41], np.int32), np.array([0, W - 1], np.int32))   # fine
+    with pytest.raises(EngineError, match='local to this shard'):
+        eng.inject(np.array([3], np.int32), np.array([0], np.int32))
+    with pytest.raises(EngineError, match='outside the window'):
+        eng.inject(np.array([42], np.int32), np.array([W], np.int32))
+    with pytest.raises(EngineError, match='gid out of range'):
+        eng.inject(np.array([50], np.int32), np.array([0], np.int32))
+    with pytest.raises(EngineError, match='repeated'):
+        eng.inject(np.full(11, 45, np.int32), np.zeros(11, np.int32))
+    eng.close()
