@@ -1374,24 +1374,6 @@ __global__ void k_seg_edges(const uint32_t* __restrict__ order, long long E, con
   if (eq) eq[k] = (long long)llrint(ldexp(w[e], F));
 }
 
-// Inline records from the segments and edge words (layout of the host build).
-__global__ void k_seg_inline(const uint32_t* __restrict__ seg, const uint32_t* __restrict__ ew, int npre, int nt,
-                             uint32_t* __restrict__ inl) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;  // (pre, tile)
-  if (i >= (long long)npre * nt) return;
-  const long long a = i / nt, t = i - a * nt;
-  const uint32_t b0 = seg[a * (nt + 1) + t];
-  const uint32_t c = seg[a * (nt + 1) + t + 1] - b0;
-  uint32_t r[16];
-  r[0] = c;
-#pragma unroll
-  for (int k = 0; k < 15; ++k) r[1 + k] = (c <= (uint32_t)kInl && (uint32_t)k < c) ? ew[b0 + k] : 0u;
-  if (c > (uint32_t)kInl) r[1] = b0;
-  uint4* o = (uint4*)(inl + i * 16);
-#pragma unroll
-  for (int k = 0; k < 4; ++k) o[k] = make_uint4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
-}
-
 // Scratch of the device segment build, grown to the largest projection.
 struct SegScratch {
   uint32_t* kv[4] = {nullptr, nullptr, nullptr, nullptr};  // keys in/out, values in/out
@@ -1413,7 +1395,7 @@ struct SegScratch {
 // edges on net->stream; adds the out-degrees to net->outdeg and returns the
 // projection's delay range. Messages match the host build's.
 static int build_segments_device(bs_net* net, const bs_projection& pj, int kind, int F, int npre, int npost,
-                                 long long g0, SegScratch& S, GProj& G, int& pdmin, int& pdmax, bool inl_ok) {
+                                 long long g0, SegScratch& S, GProj& G, int& pdmin, int& pdmax) {
   const int TS = net->TS;
   const long long E = pj.n_edges;
   const int nt = (npost + TS - 1) / TS;
@@ -1481,20 +1463,6 @@ static int build_segments_device(bs_net* net, const bs_projection& pj, int kind,
   hipLaunchKernelGGL(k_seg_edges, dim3((unsigned)((E + 4 + 255) / 256)), dim3(256), 0, st, S.kv[3], E, pj.post,
                      pj.delay, pj.delay_steps, kind == BS_CH_FIXED ? pj.weight : nullptr, F, TS, dew, deq);
   HIPCHK(hipGetLastError());
-  // inline records, chosen as by the host build
-  const double cells = (double)npre * nt;
-  if (inl_ok && kind == BS_CH_COUNT && E > 0 && (double)E <= 8.0 * cells && cells * 64.0 <= (double)(1ll << 30)) {
-    uint32_t* dinl = nullptr;
-    rc = dmalloc(&dinl, (size_t)npre * nt * 16);
-    if (rc) return rc;
-    net->owned.push_back(dinl);
-    const long long ncell = (long long)npre * nt;
-    hipLaunchKernelGGL(k_seg_inline, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, st, dseg, dew, npre, nt,
-                       dinl);
-    HIPCHK(hipGetLastError());
-    G.inl = (const uint4*)dinl;
-    net->has_inl = true;
-  }
   int stats[4];
   std::vector<int> deg((size_t)npre);
   HIPCHK(hipMemcpyAsync(stats, S.stats, sizeof(stats), hipMemcpyDeviceToHost, st));
@@ -1933,8 +1901,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     int pdmin = 0, pdmax = 0;
     if (pj.on_device) {
       CHK(build_segments_device(net, pj, kind, d->channels[pj.channel].fixed_shift, d->pops[pj.pre_pop].n,
-                                d->pops[pj.post_pop].n, gid0_of[pj.pre_pop], seg_scratch, G, pdmin, pdmax,
-                                inl_ok));
+                                d->pops[pj.post_pop].n, gid0_of[pj.pre_pop], seg_scratch, G, pdmin, pdmax));
     } else {
       StaticSeg& S = sseg[q];
       if (!S.err.empty()) BAD(S.err);
