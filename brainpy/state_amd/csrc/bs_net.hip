@@ -138,15 +138,7 @@ struct GProj {
   const uint32_t* seg;
   const uint32_t* edges;
   const long long* q;  // FIXED channels: 2^F fixed-point weights per edge
-  const uint4* inl;    // or NULL: 64-byte (pre, tile) records with the edges inline
 };
-
-// Inline segment records (COUNT projections with small segments): one 64-byte
-// record per (pre, tile), word 0 the edge count c; for c <= kInl words 1..c are
-// the edge words themselves, else word 1 is the segment's offset in edges[].
-// The gather then reads one record (one cache line) instead of the segment
-// bounds and then the edge words.
-constexpr int kInl = 15;
 
 // Poisson channel whose counts producer blocks draw one window ahead: the
 // population's neurons gid0 .. gid0 + n - 1, channel chans[ch].
@@ -219,7 +211,6 @@ struct GInfo {
   const uint32_t* seg;
   const uint32_t* edges;
   const long long* q;
-  const uint4* inl;  // inline segment records, or NULL
   int pre_gid0, nt1, kind, cw, half, fx, s_lo, pad;
 };
 
@@ -437,9 +428,7 @@ __global__ __launch_bounds__(256) void k_pois_fill(WinArgs a) {
   produce_poisson(a, a.n0, (int)blockIdx.x, (int)gridDim.x);
 }
 
-// INL: some gather source of the network has inline segment records (a
-// compile-time variant, so networks without them run the plain gather).
-template <typename T, bool INL>
+template <typename T>
 __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   if (a.base_ptr) {
@@ -673,39 +662,13 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
         si[q] = a.sb[(long long)(s % a.Ls) * a.Ntot + gi[g].pre_gid0 + (r - pg[lo])];
       }
     }
-    // the spike's (pre, tile) segment: an inline record (one 64-byte load
-    // that holds small segments' edges), or the segment bounds
-    uint4 rec[kQ][4];
-    int ninl[kQ];
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
       e0[q] = e1[q] = 0;
-      ninl[q] = 0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) rec[q][k] = make_uint4(0u, 0u, 0u, 0u);
       if (sg[q] >= 0) {
-        const GInfo& G = gi[sg[q]];
-        if (INL && G.inl) {
-          const uint4* rp = G.inl + ((long long)si[q] * (G.nt1 - 1) + tile_in_pop) * 4;
-#pragma unroll
-          for (int k = 0; k < 4; ++k) rec[q][k] = rp[k];
-        } else {
-          const uint32_t* sp = G.seg + (long long)si[q] * G.nt1 + tile_in_pop;
-          e0[q] = sp[0];
-          e1[q] = sp[1];
-        }
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < kQ; ++q) {
-      if (INL && sg[q] >= 0 && gi[sg[q]].inl) {
-        const uint32_t c = rec[q][0].x;
-        if (c <= (uint32_t)kInl) {
-          ninl[q] = (int)c;
-        } else {  // a large segment: its edges from edges[], as for the bounds
-          e0[q] = rec[q][0].y;
-          e1[q] = e0[q] + c;
-        }
+        const uint32_t* sp = gi[sg[q]].seg + (long long)si[q] * gi[sg[q]].nt1 + tile_in_pop;
+        e0[q] = sp[0];
+        e1[q] = sp[1];
       }
     }
     const uint4* ep[kQ];
@@ -727,35 +690,6 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
       nchk[q] = e1[q] > e0[q] ? (e1[q] - a0[q] + 3u) >> 2 : 0u;
       rounds = max(rounds, (int)((nchk[q] + kCH - 1) / kCH));
     }
-    // one edge word of spike q (e: its index in edges[], read for FIXED only)
-    auto deliver = [&](int q, uint32_t word, uint32_t e) {
-      const int w = dn[q] + (int)(word >> 8);
-      const int ln = (int)(word & 255u);
-      if (w < H) {  // an LDS counter (this window's steps, and later ones when they fit)
-        if (kc[q])
-          atomicAdd(&cnt[(w * ncw + cwq[q]) * TS + ln], incq[q]);
-        else
-          atomicAdd((unsigned long long*)&fxa[(w * nfx + fxq[q]) * TS + ln], (unsigned long long)qp[q][e]);
-      } else {  // beyond the LDS horizon: this tile's slot of the arrival ring
-        const size_t slot = (size_t)((a.n0 + w) % a.Lr);
-        if (kc[q])
-          __hip_atomic_fetch_add(&a.ring_cnt[(slot * a.NCW + cwq[q]) * a.Ntot + tile_gid0 + ln], incq[q],
-                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        else
-          __hip_atomic_fetch_add((unsigned long long*)&a.ring_fx[(slot * a.NFX + fxq[q]) * a.Ntot + tile_gid0 + ln],
-                                 (unsigned long long)qp[q][e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      }
-    };
-    // inline edges (COUNT projections only)
-#pragma unroll
-    for (int q = 0; q < (INL ? kQ : 0); ++q) {
-      const uint32_t iw[kInl] = {rec[q][0].y, rec[q][0].z, rec[q][0].w, rec[q][1].x, rec[q][1].y,
-                                 rec[q][1].z, rec[q][1].w, rec[q][2].x, rec[q][2].y, rec[q][2].z,
-                                 rec[q][2].w, rec[q][3].x, rec[q][3].y, rec[q][3].z, rec[q][3].w};
-#pragma unroll
-      for (int m = 0; m < kInl; ++m)
-        if (m < ninl[q]) deliver(q, iw[m], 0u);
-    }
     for (int rd = 0; rd < rounds; ++rd) {
       uint4 v[kQ][kCH];
 #pragma unroll
@@ -773,7 +707,26 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
 #pragma unroll
           for (int m = 0; m < 4; ++m) {
             const uint32_t e = a0[q] + (uint32_t)((rd * kCH + c) * 4 + m);
-            if (e >= e0[q] && e < e1[q]) deliver(q, wv[m], e);
+            if (e >= e0[q] && e < e1[q]) {
+              const int w = dn[q] + (int)(wv[m] >> 8);
+              const int ln = (int)(wv[m] & 255u);
+              if (w < H) {  // an LDS counter (this window's steps, and later ones when they fit)
+                if (kc[q])
+                  atomicAdd(&cnt[(w * ncw + cwq[q]) * TS + ln], incq[q]);
+                else
+                  atomicAdd((unsigned long long*)&fxa[(w * nfx + fxq[q]) * TS + ln],
+                            (unsigned long long)qp[q][e]);
+              } else {  // beyond the LDS horizon: this tile's slot of the arrival ring
+                const size_t slot = (size_t)((a.n0 + w) % a.Lr);
+                if (kc[q])
+                  __hip_atomic_fetch_add(&a.ring_cnt[(slot * a.NCW + cwq[q]) * a.Ntot + tile_gid0 + ln],
+                                         incq[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                else
+                  __hip_atomic_fetch_add(
+                      (unsigned long long*)&a.ring_fx[(slot * a.NFX + fxq[q]) * a.Ntot + tile_gid0 + ln],
+                      (unsigned long long)qp[q][e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              }
+            }
           }
         }
     }
@@ -1124,8 +1077,6 @@ struct bs_net {
   int* d_prevlen = nullptr;        // graph replays: length of the window before the chunk
   int last_win = 0;                // length of the last window run (0 after reset)
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // one full chunk of window launches per record buffer
-  bool has_inl = false;            // some gather source has inline segment records
-  int n_cu = 0;
   // Poisson producer blocks (when the tiles leave CUs idle)
   std::vector<int> pop_pc;         // population has a drawn-ahead Poisson channel
   std::vector<PJob> pjobs;
@@ -1534,7 +1485,6 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     int n_cu = 0;
     HCHK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, net->device));
     if (!rkf && TS == 64 && ntiles(64) < n_cu) TS = 32;
-    net->n_cu = n_cu;
     if (d->tile_size != 0) {
       if (d->tile_size != 32 && d->tile_size != 64 && d->tile_size != 128 && d->tile_size != 256)
         BAD("bs_net_create: tile_size must be 0, 32, 64, 128 or 256");
@@ -1670,11 +1620,9 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     std::string err;
     int pdmin = 0, pdmax = 0, nt = 0;
     std::vector<uint32_t> seg, ew;
-    std::vector<uint32_t> inl;  // inline records (16 words per (pre, tile)), when chosen
     std::vector<long long> eq;
   };
   std::vector<StaticSeg> sseg(d->n_projs);
-  const bool inl_ok = (long long)net->tiles.size() <= 2ll * net->n_cu;
   std::vector<long long> gid0_of(d->n_pops);
   for (int p = 0; p < d->n_pops; ++p) gid0_of[p] = net->pops[p].gid0;
   {
@@ -1730,28 +1678,6 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
         const uint32_t k = pos[(size_t)a * (nt + 1) + b / TS]++;
         S.ew[k] = (uint32_t)(b % TS) | ((uint32_t)dd << 8);
         if (kind == BS_CH_FIXED) S.eq[k] = (long long)std::llrint(std::ldexp(pj.weight[e], F));
-      }
-      // inline records for COUNT projections whose segments are small (mean
-      // <= 8 edges) while the records stay within 1 GiB, in networks whose
-      // launch is one or two rounds of tiles (latency-bound: the record saves
-      // a dependent load; in many-round launches the plain gather is faster)
-      if (inl_ok && kind == BS_CH_COUNT && pj.n_edges > 0) {
-        const double cells = (double)npre * nt;
-        if ((double)pj.n_edges <= 8.0 * cells && cells * 64.0 <= (double)(1ll << 30)) {
-          S.inl.assign((size_t)npre * nt * 16, 0u);
-          for (int a = 0; a < npre; ++a)
-            for (int t = 0; t < nt; ++t) {
-              const uint32_t b0 = S.seg[(size_t)a * (nt + 1) + t];
-              const uint32_t c = S.seg[(size_t)a * (nt + 1) + t + 1] - b0;
-              uint32_t* r = &S.inl[((size_t)a * nt + t) * 16];
-              r[0] = c;
-              if (c <= (uint32_t)kInl) {
-                for (uint32_t k = 0; k < c; ++k) r[1 + k] = S.ew[b0 + k];
-              } else {
-                r[1] = b0;
-              }
-            }
-        }
       }
       S.pdmin = pdmin;
       S.pdmax = pdmax;
@@ -1913,16 +1839,9 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
       CHK(upload(net, &dseg, S.seg.data(), S.seg.size()));
       CHK(upload(net, &dew, S.ew.data(), S.ew.size()));
       if (kind == BS_CH_FIXED) CHK(upload(net, &deq, S.eq.data(), S.eq.size()));
-      if (!S.inl.empty()) {
-        uint32_t* dinl = nullptr;
-        CHK(upload(net, &dinl, S.inl.data(), S.inl.size()));
-        G.inl = (const uint4*)dinl;
-        net->has_inl = true;
-      }
       std::vector<uint32_t>().swap(S.seg);  // host copies no longer needed
       std::vector<uint32_t>().swap(S.ew);
       std::vector<long long>().swap(S.eq);
-      std::vector<uint32_t>().swap(S.inl);
       G.seg = dseg;
       G.edges = dew;
       G.q = deq;
@@ -2070,7 +1989,6 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
         I.seg = G.seg;
         I.edges = G.edges;
         I.q = G.q;
-        I.inl = G.inl;
         I.pre_gid0 = net->pops[G.pre_pop].gid0;
         I.nt1 = G.nt + 1;
         I.kind = ch.kind;
@@ -2289,13 +2207,9 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     }
   }
   if (gid) HCHK(hipMemcpy(net->d_vm_col, vm_col.data(), 4 * gid, hipMemcpyHostToDevice));
-  HCHK(hipFuncSetAttribute((const void*)k_window<double, false>,
+  HCHK(hipFuncSetAttribute((const void*)k_window<double>,
                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget));
-  HCHK(hipFuncSetAttribute((const void*)k_window<double, true>,
-                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget));
-  HCHK(hipFuncSetAttribute((const void*)k_window<float, false>,
-                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget));
-  HCHK(hipFuncSetAttribute((const void*)k_window<float, true>,
+  HCHK(hipFuncSetAttribute((const void*)k_window<float>,
                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget));
   {
     int rc = upload_state(net);
@@ -2398,12 +2312,8 @@ static void launch_window(bs_net* net, int n0, int W, long long vm_step0, int pr
   a.pjobs = net->d_pjobs;
   a.n_pjobs = (int)net->pjobs.size();
   a.pjob_n = net->pjob_n;
-  if (net->has_inl)
-    hipLaunchKernelGGL((k_window<T, true>), dim3((unsigned)(net->tiles.size() + net->n_prod)), dim3(256),
-                       net->lds_bytes, net->stream, a);
-  else
-    hipLaunchKernelGGL((k_window<T, false>), dim3((unsigned)(net->tiles.size() + net->n_prod)), dim3(256),
-                       net->lds_bytes, net->stream, a);
+  hipLaunchKernelGGL(k_window<T>, dim3((unsigned)(net->tiles.size() + net->n_prod)), dim3(256),
+                     net->lds_bytes, net->stream, a);
 }
 
 // Poisson counts of the window starting at n0 (before a run's first window).
