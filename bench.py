@@ -146,6 +146,14 @@ def main():
                 'us_per_launch': round(kms * 1e3, 3),
                 'launch_fraction_of_wall': round(tm['update_ms'] / max(1e-9, tm['total_ms']), 4)}
 
+    # ---- BPTT leg, before the CPU legs (their single-core load slows the host
+    # side of the next GPU measurement), with the network engine released
+    n_synapses = int(desc.c.projs[0].n_edges + desc.c.projs[1].n_edges +
+                     desc.c.projs[2].n_edges + desc.c.projs[3].n_edges)
+    eng.close()
+    sim._engine = None
+    bptt = run_bptt(args) if args.bptt_steps > 0 else None
+
     # ---- CPU baseline: the oracle on a bounded sample of the same network
     cpu = None
     if not args.no_cpu:
@@ -166,8 +174,6 @@ def main():
                'rtf': round(cwall / (args.cpu_steps * sim.dt / 1000.0), 3)}
         log(f'cpu oracle: {args.cpu_steps} steps in {cwall:.1f}s, {ev / cwall:.3e} ev/s')
 
-    bptt = run_bptt(args) if args.bptt_steps > 0 else None
-
     out = {
         'metric': 'synaptic events/s (Brunel-11250, 1 GPU)',
         'value': round(value, 1),
@@ -183,8 +189,7 @@ def main():
         'data': 'synthetic (seeded Brunel network, on-device Poisson drive)',
         'config': {'workload': 'brunel-11250 iaf_psc_delta, CE=900 CI=225, per-edge delays '
                                '1.0-3.0 ms, dt=0.1 ms', 'n_neurons': 11250,
-                   'n_synapses': int(desc.c.projs[0].n_edges + desc.c.projs[1].n_edges +
-                                     desc.c.projs[2].n_edges + desc.c.projs[3].n_edges),
+                   'n_synapses': n_synapses,
                    'sim_ms': args.steps * sim.dt, 'parallelism': 'single-gpu'},
         'rtf': round(wall / sim_s, 6),
         'firing_rate_hz': round(rate, 2),
