@@ -8,6 +8,10 @@
 * ``dc_generator`` (``_nest_device/dc_generator.py:31-262``): a constant
   current (pA) inside the half-open window ``[origin + start, origin + stop)``,
   injected into the neuron's current-input seam (``simulator.py:1612-1745``).
+* ``step_current_generator`` (``_nest_device/step_current_generator.py:231-335``):
+  a piecewise-constant current, the latest plateau whose change time is
+  ``<= t``, gated by the same half-open window; realised as one dc plateau
+  per schedule entry (at most one is non-zero at any step).
 * ``spike_recorder`` / ``voltmeter`` / ``multimeter``: tapped by the Simulator
   every step after the update (``simulator.py:1858-1867``).
 """
@@ -19,7 +23,8 @@ import numpy as np
 
 from . import units as u
 
-__all__ = ['poisson_generator', 'dc_generator', 'spike_recorder', 'voltmeter', 'multimeter']
+__all__ = ['poisson_generator', 'dc_generator', 'step_current_generator', 'spike_recorder', 'voltmeter',
+           'multimeter']
 
 
 class poisson_generator:
@@ -86,17 +91,82 @@ class dc_generator:
         """(n_start, n_stop): active at steps n_start <= n < n_stop, i.e. where
         ``n * dt >= origin + start`` and ``n * dt < origin + stop``, both
         compared in float64 as the reference compares ``t`` (dc_generator.py:252-262)."""
-        def first_at_or_after(t):
-            n = max(0, int(math.floor(t / dt_ms)) - 2)
-            while float(n) * dt_ms < t:
-                n += 1
-            return n
         t_start = self.origin + self.start
-        n_start = first_at_or_after(t_start)
+        n_start = _first_step_at_or_after(t_start, dt_ms)
         if self.stop is None:
             return n_start, np.iinfo(np.int64).max
-        n_stop = first_at_or_after(self.origin + self.stop)
+        n_stop = _first_step_at_or_after(self.origin + self.stop, dt_ms)
         return n_start, max(n_stop, n_start)     # stop <= start: never active
+
+    def plateaus(self, dt_ms):
+        """[(amplitude, n_start, n_stop)]: the current as dc plateaus."""
+        return [(self.amplitude,) + self.step_window(dt_ms)]
+
+
+def _first_step_at_or_after(t, dt_ms):
+    """The first step n with ``n * dt >= t`` (float64 product, as the step's time)."""
+    if t == math.inf:
+        return np.iinfo(np.int64).max
+    n = max(0, int(math.floor(t / dt_ms)) - 2)
+    while float(n) * dt_ms < t:
+        n += 1
+    return n
+
+
+def _schedule(x, dim):
+    if isinstance(x, u.Quantity) or isinstance(x, np.ndarray):
+        return [float(v) for v in np.ravel(u.magnitude(x, dim))]
+    return [u.scalar(v, dim) for v in x]
+
+
+class step_current_generator:
+    """Piecewise-constant current generator (NEST ``step_current_generator``):
+    ``amplitude_values[k]`` (pA) from ``amplitude_times[k]`` (ms) until the next
+    change time, 0 before the first, gated by ``origin + start <= t < origin +
+    stop`` (``step_current_generator.py:269-335``: the plateau is the last
+    ``k`` with ``amplitude_times[k] <= t``, compared in float64)."""
+    __module__ = 'brainpy.state_amd'
+    _is_generator = True
+    _injects_current = True
+
+    def __init__(self, in_size=1, amplitude_times=(), amplitude_values=(), start=0.0, stop=None,
+                 origin=0.0, name=None):
+        self.in_size = int(in_size)
+        times = _schedule(amplitude_times, 'time')
+        values = _schedule(amplitude_values, 'current')
+        if len(times) != len(values):
+            raise ValueError('amplitude_times and amplitude_values must have the same length. '
+                             f'Got {len(times)} and {len(values)}.')
+        if len(times) == 0:
+            raise AssertionError('At least one schedule entry is required. Got len(amplitude_times) = 0.')
+        for i in range(1, len(times)):
+            if times[i] <= times[i - 1]:
+                raise ValueError('amplitude_times must be strictly increasing. '
+                                 f'Got {times[i - 1]} >= {times[i]} at index {i}.')
+        self.amplitude_times = times
+        self.amplitude_values = values
+        self.start = u.scalar(start, 'time')
+        self.stop = None if stop is None else u.scalar(stop, 'time')
+        self.origin = u.scalar(origin, 'time')
+        self.name = name
+
+    def plateaus(self, dt_ms):
+        """[(amplitude, n_start, n_stop)]: plateau k is on at steps n_start <= n
+        < n_stop, the steps where ``amplitude_times[k] <= n * dt <
+        amplitude_times[k + 1]`` inside the activity window. The plateaus are
+        disjoint, so their sum is the scheduled current bit for bit; empty
+        ones are dropped."""
+        w0 = _first_step_at_or_after(self.origin + self.start, dt_ms)
+        w1 = (np.iinfo(np.int64).max if self.stop is None
+              else _first_step_at_or_after(self.origin + self.stop, dt_ms))
+        edges = [_first_step_at_or_after(t, dt_ms) for t in self.amplitude_times]
+        edges.append(np.iinfo(np.int64).max)
+        out = []
+        for k, amp in enumerate(self.amplitude_values):
+            n0, n1 = max(edges[k], w0), min(edges[k + 1], w1)
+            if n0 < n1:
+                out.append((amp, n0, n1))
+        return out
 
 
 class spike_recorder:

@@ -692,7 +692,11 @@ class _NetBuilder:
         per_pop = [[] for _ in range(npop)]
         for r in sim._records:
             q = D.pop_index[id(r.post_pop)]
-            if q not in self.remote:          # a remote population's inputs are its own shard's
+            if q in self.remote:              # a remote population's inputs are its own shard's
+                continue
+            if isinstance(r, _CurChannel):    # one current channel per plateau, in schedule order
+                per_pop[q].extend((r, piece) for piece in r.gen.plateaus(dt))
+            else:
                 per_pop[q].append(r)
         chans = (L.Channel * max(1, sum(len(x) for x in per_pop)))()
         cpops = (L.Population * npop)()
@@ -738,13 +742,15 @@ class _NetBuilder:
             P.remote = 1 if p in self.remote else 0
             for r in per_pop[p]:
                 ch = chans[c]
+                piece = None
+                if isinstance(r, tuple):
+                    r, piece = r
                 ch.receptor = r.receptor
-                if isinstance(r, _CurChannel):
-                    g = r.gen
+                if piece is not None:
+                    amp, n0, n1 = piece
                     ch.kind = L.BS_CH_CURRENT
                     # device.update() * weight (simulator.py:1738-1740), one product
-                    ch.weight = g.amplitude if r.weight is None else g.amplitude * r.weight
-                    n0, n1 = g.step_window(dt)
+                    ch.weight = amp if r.weight is None else amp * r.weight
                     ch.t_min_step = int(n0)
                     ch.t_max_step = int(min(n1, np.iinfo(np.int64).max))
                     if len(r.post_idx) == pop.n and np.array_equal(r.post_idx, np.arange(pop.n)):

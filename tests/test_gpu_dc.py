@@ -1,20 +1,21 @@
-"""GPU: dc_generator current inputs (full populations, subsets, start/stop/
-origin windows, scaled by connection weights) on all four models, mixed with
-Poisson drive and recurrent projections: spikes and V_m bit-exact against the
-oracle from the same descriptor."""
+"""GPU: dc_generator and step_current_generator current inputs (full
+populations, subsets, start/stop/origin windows, schedules, scaled by
+connection weights) on all four models, mixed with Poisson drive and recurrent
+projections: spikes and V_m bit-exact against the oracle from the same
+descriptor."""
 import numpy as np
 import pytest
 
 import oracle
 from brainpy.state_amd import (Simulator, aeif_cond_alpha, dc_generator, fixed_indegree,
                                iaf_psc_alpha, iaf_psc_delta, iaf_psc_exp, poisson_generator,
-                               spike_recorder, units as u, voltmeter)
+                               spike_recorder, step_current_generator, units as u, voltmeter)
 from brainpy.state_amd.init import Uniform
 
 pytestmark = pytest.mark.gpu
 
 
-def _net(model, dtype='float64'):
+def _net(model, dtype='float64', schedules=False):
     sim = Simulator(dt=0.1 * u.ms, dtype=dtype)
     cond = model is aeif_cond_alpha
     kw = {} if cond else dict(V_initializer=Uniform(-70., -60., seed=5))
@@ -31,6 +32,15 @@ def _net(model, dtype='float64'):
     sim.connect(sim.create(dc_generator, amplitude=450.0, start=20.0, stop=80.0), b[:40])
     sim.connect(sim.create(dc_generator, amplitude=100.0, start=10.0, stop=50.0, origin=40.0), a[100:],
                 weight=1.5)
+    if schedules:
+        # step drive: a schedule on a whole population (negative plateau too),
+        # a gated one on a subset, a scaled one with an origin
+        sim.connect(sim.create(step_current_generator, amplitude_times=[0.0, 30.0, 60.0, 95.0],
+                               amplitude_values=[150.0, -200.0, 400.0, 0.0]), a)
+        sim.connect(sim.create(step_current_generator, amplitude_times=[12.3, 47.0],
+                               amplitude_values=[500.0, 250.0], start=15.0, stop=120.0), b[10:70])
+        sim.connect(sim.create(step_current_generator, amplitude_times=[5.0, 25.0],
+                               amplitude_values=[300.0, 120.0], origin=20.0), a[:150], weight=0.8)
     sim.connect(a, a + b, weight=wE, delay=1.0, rule=fixed_indegree(20), seed=4, **rt)
     sim.connect(b, a + b, weight=wI, delay=2.0, rule=fixed_indegree(8), seed=5, **rti)
     for pop in (a, b):
@@ -64,3 +74,17 @@ def test_dc_current_float32_runs_against_float32_oracle():
     vm_g, (g_g, s_g) = eng.run(1000)
     vm_o, g_o, s_o = oracle.OracleNet(desc).run(1000)
     assert np.array_equal(g_g, g_o) and np.array_equal(s_g, s_o)
+
+
+@pytest.mark.parametrize('model', [iaf_psc_delta, iaf_psc_exp, iaf_psc_alpha, aeif_cond_alpha])
+def test_step_current_network_bit_exact(model):
+    sim = _net(model, schedules=True)
+    desc = sim.build_desc()
+    eng = sim._ensure_engine()
+    eng.reset()
+    n = 1500
+    vm_g, (g_g, s_g) = eng.run(n)
+    vm_o, g_o, s_o = oracle.OracleNet(desc).run(n)
+    assert len(g_o) > 20, 'network too quiet for a useful check'
+    assert np.array_equal(g_g, g_o) and np.array_equal(s_g, s_o)
+    np.testing.assert_allclose(vm_g, vm_o, rtol=1e-12, atol=0)
