@@ -160,4 +160,52 @@ __device__ __forceinline__ double pow_p(double x, double y) {
   return exp_p(y * log_p(x));
 }
 
+// ---------------------------------------------------------------- sin
+// pi/2 in three parts (Cody & Waite): PIO2_1 and PIO2_2 carry 33 significant
+// bits, so k * PIO2_1 and k * PIO2_2 are exact for |k| < 2^20; PIO2_3 rounds
+// the rest. BS_2_PI is 2/pi rounded.
+#define BS_PIO2_1 0x1.921fb54400000p+0
+#define BS_PIO2_2 0x1.0b4611a600000p-34
+#define BS_PIO2_3 0x1.3198a2e037073p-69
+#define BS_2_PI 0x1.45f306dc9c883p-1
+
+// sin(x): k = rint(x 2/pi), r = ((x - k PIO2_1) - k PIO2_2) - k PIO2_3,
+// |r| <= pi/4 (plus rounding); by the quadrant k mod 4 the Taylor series of
+// sin r (to r^21) or cos r (to r^20), Horner in z = r^2 (truncation below
+// 1e-23 relative). Each operation is one IEEE-rounded op in this order.
+__device__ __forceinline__ double sin_p(double x) {
+  if (!(x - x == 0.0)) return x - x;  // inf and NaN give NaN
+  const double kd = __builtin_rint(x * BS_2_PI);
+  const double r = ((x - kd * BS_PIO2_1) - kd * BS_PIO2_2) - kd * BS_PIO2_3;
+  const double z = r * r;
+  const int q = (int)((long long)kd & 3);
+  double v;
+  if (q & 1) {
+    double c = 1.0 / 2432902008176640000.0;   // 1/20!
+    c = -1.0 / 6402373705728000.0 + z * c;    // 1/18!
+    c = 1.0 / 20922789888000.0 + z * c;       // 1/16!
+    c = -1.0 / 87178291200.0 + z * c;         // 1/14!
+    c = 1.0 / 479001600.0 + z * c;            // 1/12!
+    c = -1.0 / 3628800.0 + z * c;             // 1/10!
+    c = 1.0 / 40320.0 + z * c;                // 1/8!
+    c = -1.0 / 720.0 + z * c;                 // 1/6!
+    c = 1.0 / 24.0 + z * c;                   // 1/4!
+    c = -0.5 + z * c;
+    v = 1.0 + z * c;
+  } else {
+    double s = 1.0 / 51090942171709440000.0;  // 1/21!
+    s = -1.0 / 121645100408832000.0 + z * s;  // 1/19!
+    s = 1.0 / 355687428096000.0 + z * s;      // 1/17!
+    s = -1.0 / 1307674368000.0 + z * s;       // 1/15!
+    s = 1.0 / 6227020800.0 + z * s;           // 1/13!
+    s = -1.0 / 39916800.0 + z * s;            // 1/11!
+    s = 1.0 / 362880.0 + z * s;               // 1/9!
+    s = -1.0 / 5040.0 + z * s;                // 1/7!
+    s = 1.0 / 120.0 + z * s;                  // 1/5!
+    s = -1.0 / 6.0 + z * s;                   // 1/3!
+    v = r + (r * z) * s;
+  }
+  return (q & 2) ? -v : v;
+}
+
 }  // namespace bs

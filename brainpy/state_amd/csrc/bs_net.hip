@@ -349,6 +349,14 @@ __device__ __forceinline__ T chan_value(const UpdCtx<T>& u, const ChReg<T>& c, i
     const int e = c.gen_index ? c.gen_index[j] : j;
     return (e >= 0 && (long long)step >= c.tmin && (long long)step < c.tmax) ? c.weight : (T)0;
   }
+  if (c.kind == BS_CH_AC) {  // ac_generator: (A sin(omega t + phi) + offset) * weight
+    const int j = u.start + lane;
+    const int e = c.gen_index ? c.gen_index[j] : j;
+    if (e < 0 || (long long)step < c.tmin || (long long)step >= c.tmax) return (T)0;
+    const double* p = c.cdf;  // {A, omega, phi, offset, weight, dt}
+    const double t = (double)step * p[5];
+    return (T)((p[0] * sin_p(p[1] * t + p[2]) + p[3]) * p[4]);
+  }
   long long q;  // BS_CH_FIXED
   if (c.pacc) {  // plastic: this step's deposits (windows of one step), consumed here
     const int j = u.start + lane;
@@ -783,7 +791,7 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
           const int idx = base0 + tid + 256 * i, w = idx >> lgTS, l = idx & (TS - 1);
           v[i] = ok[i] ? chan_value<T>(u, cc, w, a.n0 + w, l) : (T)0;
         }
-        if (cc.kind == BS_CH_CURRENT) {
+        if (cc.kind == BS_CH_CURRENT || cc.kind == BS_CH_AC) {
 #pragma unroll
           for (int i = 0; i < kIT; ++i) accC[i] = accC[i] + v[i];
         } else if (cc.receptor == 0) {
@@ -1576,6 +1584,23 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
       } else if (ch.kind == BS_CH_CURRENT) {
         if (!(ch.weight == ch.weight)) BAD("current channel value must not be NaN");
         if (ch.receptor != 0) BAD("current channels feed the current-input seam (receptor 0)");
+        dc.weight = ch.weight;
+        dc.tmin = ch.t_min_step;
+        dc.tmax = ch.t_max_step;
+        if (ch.gen_index) {
+          int* g = nullptr;
+          CHK(upload(net, &g, ch.gen_index, (size_t)sp.n));
+          dc.gen_index = g;
+        }
+        ++dp.n_cur;
+      } else if (ch.kind == BS_CH_AC) {
+        const double pr[6] = {ch.ac[0], ch.ac[1], ch.ac[2], ch.ac[3], ch.weight, d->dt_ms};
+        for (double x : pr)
+          if (!std::isfinite(x)) BAD("ac channel parameters must be finite");
+        if (ch.receptor != 0) BAD("current channels feed the current-input seam (receptor 0)");
+        double* dpr = nullptr;
+        CHK(upload(net, &dpr, pr, 6));
+        dc.cdf = dpr;
         dc.weight = ch.weight;
         dc.tmin = ch.t_min_step;
         dc.tmax = ch.t_max_step;

@@ -12,6 +12,9 @@
   a piecewise-constant current, the latest plateau whose change time is
   ``<= t``, gated by the same half-open window; realised as one dc plateau
   per schedule entry (at most one is non-zero at any step).
+* ``ac_generator`` (``_nest_device/ac_generator.py:236-325``): ``amplitude *
+  sin(2 pi frequency t + phase) + offset`` inside the same window, evaluated
+  on the device each step.
 * ``spike_recorder`` / ``voltmeter`` / ``multimeter``: tapped by the Simulator
   every step after the update (``simulator.py:1858-1867``).
 """
@@ -23,8 +26,8 @@ import numpy as np
 
 from . import units as u
 
-__all__ = ['poisson_generator', 'dc_generator', 'step_current_generator', 'spike_recorder', 'voltmeter',
-           'multimeter']
+__all__ = ['poisson_generator', 'dc_generator', 'step_current_generator', 'ac_generator', 'spike_recorder',
+           'voltmeter', 'multimeter']
 
 
 class poisson_generator:
@@ -167,6 +170,42 @@ class step_current_generator:
             if n0 < n1:
                 out.append((amp, n0, n1))
         return out
+
+
+class ac_generator:
+    """Sinusoidal current generator (NEST ``ac_generator``): ``amplitude *
+    sin(omega t + phi) + offset`` (pA) while ``origin + start <= t < origin +
+    stop``, else 0, with ``omega = 2 pi frequency / 1000`` rad/ms, ``phi`` the
+    phase in degrees converted to radians and ``t = n * dt``
+    (``ac_generator.py:264-325``)."""
+    __module__ = 'brainpy.state_amd'
+    _is_generator = True
+    _injects_current = True
+
+    def __init__(self, in_size=1, amplitude=0.0, offset=0.0, frequency=0.0, phase=0.0, start=0.0,
+                 stop=None, origin=0.0, name=None):
+        self.in_size = int(in_size)
+        self.amplitude = u.scalar(amplitude, 'current')
+        self.offset = u.scalar(offset, 'current')
+        self.frequency = u.scalar(frequency, 'frequency')
+        self.phase = float(phase)
+        self.start = u.scalar(start, 'time')
+        self.stop = None if stop is None else u.scalar(stop, 'time')
+        self.origin = u.scalar(origin, 'time')
+        self.name = name
+
+    def step_window(self, dt_ms):
+        """(n_start, n_stop), as for dc_generator."""
+        return dc_generator.step_window(self, dt_ms)
+
+    def ac_params(self):
+        """(amplitude, omega in rad/ms, phase in rad, offset)."""
+        return (self.amplitude, 2.0 * math.pi * self.frequency / 1000.0,
+                self.phase * 2.0 * math.pi / 360.0, self.offset)
+
+    def plateaus(self, dt_ms):
+        """One channel: the window, with the sinusoid's parameters."""
+        return [('ac',) + self.step_window(dt_ms)]
 
 
 class spike_recorder:

@@ -748,9 +748,15 @@ class _NetBuilder:
                 ch.receptor = r.receptor
                 if piece is not None:
                     amp, n0, n1 = piece
-                    ch.kind = L.BS_CH_CURRENT
-                    # device.update() * weight (simulator.py:1738-1740), one product
-                    ch.weight = amp if r.weight is None else amp * r.weight
+                    if amp == 'ac':   # the sinusoid, evaluated on the device
+                        ch.kind = L.BS_CH_AC
+                        ch.weight = 1.0 if r.weight is None else r.weight
+                        for k, v in enumerate(r.gen.ac_params()):
+                            ch.ac[k] = v
+                    else:
+                        ch.kind = L.BS_CH_CURRENT
+                        # device.update() * weight (simulator.py:1738-1740), one product
+                        ch.weight = amp if r.weight is None else amp * r.weight
                     ch.t_min_step = int(n0)
                     ch.t_max_step = int(min(n1, np.iinfo(np.int64).max))
                     if len(r.post_idx) == pop.n and np.array_equal(r.post_idx, np.arange(pop.n)):

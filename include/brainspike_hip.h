@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define BS_ABI_VERSION 5
+#define BS_ABI_VERSION 6
 
 /* ---- return codes ------------------------------------------------------ */
 #define BS_OK 0
@@ -63,7 +63,7 @@ enum bs_channel_kind {
   BS_CH_FIXED = 2,   /* heterogeneous weights: exact 2^-F fixed-point sum       */
   BS_CH_POISSON = 3, /* one_to_one poisson_generator: value = count * weight
                         (event_proj.py:311-312, poisson_generator.py:501-576)    */
-  BS_CH_CURRENT = 4  /* dc_generator current (pA) into the neuron's current-input
+  BS_CH_CURRENT = 4, /* dc_generator current (pA) into the neuron's current-input
                         seam (simulator.py:1612-1745): value = weight (amplitude
                         times the connection weight) at steps t_min_step <= step
                         < t_max_step, else 0; summed in list order with the
@@ -71,6 +71,15 @@ enum bs_channel_kind {
                         inputs. iaf_psc_delta adds the sum to I_e in the same
                         step; iaf_psc_exp / iaf_psc_alpha / aeif_cond_alpha
                         latch it for the next step (i_0 / y0 / I_stim)          */
+  BS_CH_AC = 5       /* ac_generator current (ac_generator.py:264-325), a
+                        current channel like BS_CH_CURRENT: value =
+                        (ac[0] * sin(ac[1] * t + ac[2]) + ac[3]) * weight with
+                        t = step * dt (ms), ac = {amplitude (pA), omega (rad/ms)
+                        = 2 pi frequency / 1000, phase (rad), offset (pA)},
+                        inside t_min_step <= step < t_max_step, else 0. The
+                        sine is the engine's own (Cody-Waite reduction by
+                        pi/2 and Taylor series), within a few ulp of the
+                        exact sine for |argument| < 1.6e6 rad                   */
 };
 
 /* Parameter block lengths (doubles) per model; layouts are documented in
@@ -101,7 +110,7 @@ typedef struct bs_population {
 typedef struct bs_channel {
   int32_t kind;            /* enum bs_channel_kind                               */
   int32_t receptor;        /* aeif_cond_alpha: 0 -> g_ex, 1 -> g_in; else 0       */
-  double weight;           /* COUNT, POISSON                                     */
+  double weight;           /* COUNT, POISSON, CURRENT, AC (1 = unscaled)         */
   int32_t max_count;       /* COUNT: largest event count per (post, step)        */
   int32_t fixed_shift;     /* FIXED: F, value = sum(q_e) * 2^-F                  */
   /* POISSON only: */
@@ -112,7 +121,8 @@ typedef struct bs_channel {
   uint32_t seed;           /* Simulator._derive_seed(rng_seed, ordinal)          */
   int32_t delay_steps;     /* delay / dt                                         */
   const int32_t* gen_index;/* [n_post_pop] generator element feeding each post,
-                              -1 if none; NULL = identity (POISSON, CURRENT)     */
+                              -1 if none; NULL = identity (POISSON, CURRENT, AC) */
+  double ac[4];            /* AC: amplitude, omega (rad/ms), phase (rad), offset  */
 } bs_channel;
 
 /* STDP parameters (stdp_synapse.py:115-179): additive/multiplicative pair rule. */

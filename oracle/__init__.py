@@ -85,6 +85,7 @@ def lib():
             'orc_poisson': (C.c_uint32, [C.c_uint32] * 3 + [C.c_double]),
             'orc_exp': (C.c_double, [C.c_double]),
             'orc_log': (C.c_double, [C.c_double]),
+            'orc_sin': (C.c_double, [C.c_double]),
             'orc_pow': (C.c_double, [C.c_double, C.c_double]),
             'orc_stdp_edge': (C.c_double, [C.c_double] * 5 + [C.c_void_p]),
             'orc_conn_fixed_indegree': (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
@@ -121,6 +122,10 @@ def exp(x):
 
 def log(x):
     return lib().orc_log(float(x))
+
+
+def sin(x):
+    return lib().orc_sin(float(x))
 
 
 def pow(x, y):  # noqa: A001

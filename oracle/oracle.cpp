@@ -132,6 +132,41 @@ extern "C" double orc_exp(double x) {
   return ldexp(p, (int)k);
 }
 
+// sin(x) of the ac_generator current (the reference calls jnp.sin,
+// ac_generator.py:312-313). The same function the device evaluates: reduction
+// by pi/2 in three parts (Cody & Waite; 33 + 33 + 53 bits, exact products for
+// |k| < 2^20), then by quadrant the Taylor series of sin r to r^21 or of
+// cos r to r^20, Horner in z = r^2 from the highest term, op for op.
+static const double kPio2[3] = {0x1.921fb54400000p+0, 0x1.0b4611a600000p-34,
+                                0x1.3198a2e037073p-69};
+static const double kTwoOverPi = 0x1.45f306dc9c883p-1;
+
+extern "C" double orc_sin(double x) {
+  if (!isfinite(x)) return x - x;
+  const double k = rint(x * kTwoOverPi);
+  const double r = ((x - k * kPio2[0]) - k * kPio2[1]) - k * kPio2[2];
+  const double z = r * r;
+  // fact[n] = n! as a double literal's value: the series divide 1 by it
+  static const double fact[22] = {1.0, 1.0, 2.0, 6.0, 24.0, 120.0, 720.0, 5040.0, 40320.0,
+                                  362880.0, 3628800.0, 39916800.0, 479001600.0,
+                                  6227020800.0, 87178291200.0, 1307674368000.0,
+                                  20922789888000.0, 355687428096000.0, 6402373705728000.0,
+                                  121645100408832000.0, 2432902008176640000.0,
+                                  51090942171709440000.0};
+  const long long q = (long long)k & 3;
+  double v;
+  if (q & 1) {  // cos r = 1 + z (-1/2! + z (1/4! - ...)), top term 1/20!
+    double c = 1.0 / fact[20];
+    for (int n = 18; n >= 2; n -= 2) c = ((n / 2) % 2 ? -1.0 : 1.0) / fact[n] + z * c;
+    v = 1.0 + z * c;
+  } else {      // sin r = r + r z (-1/3! + z (1/5! - ...)), top term 1/21!
+    double c = 1.0 / fact[21];
+    for (int n = 19; n >= 3; n -= 2) c = ((n / 2) % 2 ? -1.0 : 1.0) / fact[n] + z * c;
+    v = r + (r * z) * c;
+  }
+  return (q & 2) ? -v : v;
+}
+
 extern "C" double orc_log(double x) {
   if (isnan(x) || x < 0.0) return NAN;
   if (x == 0.0) return -INFINITY;
@@ -304,7 +339,7 @@ extern "C" int orc_create(const bs_net_desc* d, orc_net** out) {
     for (int k = 0; k < sp.n_vm; ++k) net->taps.emplace_back(p, sp.vm_idx[k]);
     for (int c = sp.ch_begin; c < sp.ch_begin + sp.ch_count; ++c) {
       const bs_channel& ch = d->channels[c];
-      if (ch.kind == BS_CH_POISSON || ch.kind == BS_CH_CURRENT) {
+      if (ch.kind == BS_CH_POISSON || ch.kind == BS_CH_CURRENT || ch.kind == BS_CH_AC) {
         std::vector<int> gi(sp.n);
         for (int i = 0; i < sp.n; ++i) gi[i] = ch.gen_index ? ch.gen_index[i] : i;
         net->gen_index[c] = gi;
@@ -373,7 +408,7 @@ extern "C" int orc_create(const bs_net_desc* d, orc_net** out) {
       net->chan_pop[c] = p;
       if (d->channels[c].kind == BS_CH_POISSON)
         net->ghist[c].assign(net->Lh, std::vector<double>(sp.n, 0.0));
-      else if (d->channels[c].kind == BS_CH_CURRENT)
+      else if (d->channels[c].kind == BS_CH_CURRENT || d->channels[c].kind == BS_CH_AC)
         continue;  // no arrival ring: the current enters in its own step
       else if (d->channels[c].kind == BS_CH_COUNT)
         net->cring[c].assign(net->Lh, std::vector<double>(sp.n, 0.0));
@@ -700,6 +735,15 @@ static int run_t(orc_net* net, int64_t n_steps, double* vm_out) {
         const bool active = ch.t_min_step <= n && n < ch.t_max_step;
         const std::vector<int>& gi = net->gen_index[c];
         for (int j = 0; j < npost; ++j) yin[c][j] = (active && gi[j] >= 0) ? (T)ch.weight : (T)0;
+      } else if (ch.kind == BS_CH_AC) {
+        // phase 1b: ac_generator current amplitude * sin(omega t + phi) + offset
+        // at t = n dt, times the connection weight, inside [t_min_step,
+        // t_max_step) (ac_generator.py:300-325, simulator.py:1737-1745)
+        const bool active = ch.t_min_step <= n && n < ch.t_max_step;
+        const double t = (double)n * net->dt;
+        const double I = (ch.ac[0] * orc_sin(ch.ac[1] * t + ch.ac[2]) + ch.ac[3]) * ch.weight;
+        const std::vector<int>& gi = net->gen_index[c];
+        for (int j = 0; j < npost; ++j) yin[c][j] = (active && gi[j] >= 0) ? (T)I : (T)0;
       }
     }
     // Static projections, in the kernel's DelayScatter form (delay_scatter.py:
@@ -767,7 +811,7 @@ static int run_t(orc_net* net, int64_t n_steps, double* vm_out) {
         // sum_current_inputs(init): I_e + x for iaf_psc_delta, x for the others (x = 0)
         T cur = sp.model == BS_IAF_PSC_DELTA ? (T)P[34] + (T)0 : (T)0;
         for (int c = sp.ch_begin; c < sp.ch_begin + sp.ch_count; ++c) {
-          if (net->chans[c].kind == BS_CH_CURRENT)
+          if (net->chans[c].kind == BS_CH_CURRENT || net->chans[c].kind == BS_CH_AC)
             cur = cur + yin[c][j];
           else
             acc[net->chans[c].receptor] = acc[net->chans[c].receptor] + yin[c][j];

@@ -55,6 +55,7 @@ void orc_threefry2x32(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t c1, uint32
 uint32_t orc_poisson(uint32_t seed, uint32_t step, uint32_t elem, double lam);
 double orc_exp(double x);
 double orc_log(double x);
+double orc_sin(double x);
 double orc_pow(double x, double y);
 /* connectivity samplers of the device path (definition in bs_conn.hip) */
 int orc_conn_fixed_indegree(int64_t seed, int32_t n_pre, int32_t n_post, int32_t K, int32_t exclude_self,

@@ -1,13 +1,13 @@
-"""GPU: dc_generator and step_current_generator current inputs (full
-populations, subsets, start/stop/origin windows, schedules, scaled by
-connection weights) on all four models, mixed with Poisson drive and recurrent
+"""GPU: dc_generator, step_current_generator and ac_generator current inputs
+(full populations, subsets, start/stop/origin windows, schedules, sinusoids,
+scaled by connection weights) on all four models, mixed with Poisson drive and recurrent
 projections: spikes and V_m bit-exact against the oracle from the same
 descriptor."""
 import numpy as np
 import pytest
 
 import oracle
-from brainpy.state_amd import (Simulator, aeif_cond_alpha, dc_generator, fixed_indegree,
+from brainpy.state_amd import (Simulator, ac_generator, aeif_cond_alpha, dc_generator, fixed_indegree,
                                iaf_psc_alpha, iaf_psc_delta, iaf_psc_exp, poisson_generator,
                                spike_recorder, step_current_generator, units as u, voltmeter)
 from brainpy.state_amd.init import Uniform
@@ -41,6 +41,10 @@ def _net(model, dtype='float64', schedules=False):
                                amplitude_values=[500.0, 250.0], start=15.0, stop=120.0), b[10:70])
         sim.connect(sim.create(step_current_generator, amplitude_times=[5.0, 25.0],
                                amplitude_values=[300.0, 120.0], origin=20.0), a[:150], weight=0.8)
+        # sinusoids: a whole population, and a gated, phase-shifted, scaled subset
+        sim.connect(sim.create(ac_generator, amplitude=180.0, offset=60.0, frequency=35.0, phase=20.0), b)
+        sim.connect(sim.create(ac_generator, amplitude=250.0, frequency=110.0, phase=-45.0, start=8.0,
+                               stop=130.0), a[50:250], weight=1.3)
     sim.connect(a, a + b, weight=wE, delay=1.0, rule=fixed_indegree(20), seed=4, **rt)
     sim.connect(b, a + b, weight=wI, delay=2.0, rule=fixed_indegree(8), seed=5, **rti)
     for pop in (a, b):
@@ -88,3 +92,15 @@ def test_step_current_network_bit_exact(model):
     assert len(g_o) > 20, 'network too quiet for a useful check'
     assert np.array_equal(g_g, g_o) and np.array_equal(s_g, s_o)
     np.testing.assert_allclose(vm_g, vm_o, rtol=1e-12, atol=0)
+
+
+def test_step_and_ac_currents_float32_against_float32_oracle():
+    sim = _net(iaf_psc_alpha, dtype='float32', schedules=True)
+    desc = sim.build_desc()
+    eng = sim._ensure_engine()
+    eng.reset()
+    vm_g, (g_g, s_g) = eng.run(1000)
+    vm_o, g_o, s_o = oracle.OracleNet(desc).run(1000)
+    assert len(g_o) > 20
+    assert np.array_equal(g_g, g_o) and np.array_equal(s_g, s_o)
+    np.testing.assert_allclose(vm_g, vm_o, rtol=1e-6, atol=1e-4)
