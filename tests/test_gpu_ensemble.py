@@ -1,6 +1,7 @@
 """GPU checks of the batched ensemble: every network advanced inside one engine
 gives exactly the raster of its own plain run; the launch shape (tile size,
-window bound) changes nothing; the autotuner returns a measured candidate."""
+window bound) changes nothing; the autotuner returns a measured candidate;
+current generators (dc, step, ac) survive the merge."""
 import numpy as np
 import pytest
 
@@ -66,3 +67,36 @@ def test_device_connectivity_ensemble_equals_plain_runs():
         assert np.array_equal(g, g0) and np.array_equal(s, s0)
         total += len(g)
     assert total > 1000
+
+
+def _current_driven(k):
+    from brainpy.state_amd import (Simulator, ac_generator, dc_generator, fixed_indegree, iaf_psc_exp,
+                                   spike_recorder, step_current_generator)
+    sim = Simulator(dt=0.1)
+    a = sim.create(iaf_psc_exp, 200 + 60 * k)
+    b = sim.create(iaf_psc_exp, 90)
+    sim.connect(sim.create(dc_generator, amplitude=300.0 + 20 * k), a)
+    sim.connect(sim.create(step_current_generator, amplitude_times=[0.0, 40.0 + k],
+                           amplitude_values=[350.0, 150.0]), b)
+    sim.connect(sim.create(ac_generator, amplitude=200.0, offset=50.0, frequency=20.0 + 7 * k,
+                           phase=15.0 * k, start=5.0), a[: 100], weight=1.1)
+    sim.connect(a, b, weight=60.0, delay=1.0, rule=fixed_indegree(20), seed=k + 1)
+    sim.connect(b, a, weight=-120.0, delay=1.5, rule=fixed_indegree(10), seed=k + 11)
+    for pop in (a, b):
+        sim.connect(pop, sim.create(spike_recorder))
+    return sim
+
+
+def test_current_generators_in_an_ensemble_equal_plain_runs():
+    """dc, step and ac current channels keep their meaning when the networks'
+    descriptors are merged into one engine."""
+    sims = [_current_driven(k) for k in range(3)]
+    ens = Ensemble(sims)
+    parts = ens.run(1500)
+    total = 0
+    for sim, (vm, g, s) in zip(sims, parts):
+        g0, s0 = _plain(sim, 1500)
+        assert np.array_equal(g, g0) and np.array_equal(s, s0)
+        total += len(g)
+    assert total > 500
+    ens.close()
