@@ -4,28 +4,43 @@ Workload (one GPU): Brunel balanced network, NE = 9000 + NI = 2250
 iaf_psc_delta neurons, fixed in-degree CE = 900 / CI = 225 (12,656,250
 synapses), per-edge delays uniform over the integer multiples of dt in
 1.0-3.0 ms, Poisson drive at 2 nu_th (lambda = 2 per step), dt = 0.1 ms, spikes
-recorded from every neuron and V_m from 10. A bench "step" is one network time
-step (dt); ``--steps`` steps are timed after ``--warmup`` untimed steps, with
-the network already resident in HBM.
+recorded from every neuron and V_m from 10, float64.
+
+A bench "step" is one simulated window of ``--window-ms`` (default 1000 ms =
+10,000 network steps), the reference benchmark's timed unit
+(benchmarks/brunel_simulator_baseline.py:186-343). The network first settles
+for ``--settle-ms`` (200 ms) from rest, then runs ``--warmup`` untimed windows,
+then ``--steps`` timed windows, with the network resident in HBM.
 
 The JSON line carries:
-  value        synaptic events/s (sum over spikes of the spiking neuron's
-               out-degree, divided by the timed wall time)
+  value        synaptic events/s: sum over the timed windows' spikes of the
+               spiking neuron's out-degree, over the timed wall time (which
+               includes sorting and copying the spike records to the host)
   rtf          wall time / simulated time
-  roofline     the dominant kernel's algorithmic bytes per launch over its mean
-               HIP-event duration, against 8 TB/s
-  cpu_baseline the CPU oracle (one thread) on a bounded sample of the same
-               network, events/s
+  roofline     the window kernel's algorithmic bytes per launch (delivery:
+               8 B per gathered spike + 4 B per edge; update: state and
+               refractory counter read and written once per window; 8 B per
+               spike record and V_m sample) over its mean HIP-event duration,
+               on a steady-state window after the timed ones; ``traffic`` is
+               HBM bytes per launch of the same kernel over a steady-state
+               window, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by
+               a plain device copy
+  cpu_baseline the CPU oracle (C++ restatement, one thread) on one simulated
+               second of the same network after the same settle, pinned to
+               one core and unpinned, with the host's CPU model and affinity
   bptt         the differentiation workload (recurrent iaf_psc_exp, 4096
                neurons, 10 % trainable recurrent and input weights, 200 Poisson
                inputs, batch 64, 1000 steps, checkpoint 100): gradient steps/s
                of the device adjoint, and the CPU reference (torch float64
                autograd) on a bounded sample
+
+The run exits with status 1 when the timed windows deliver no events.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -41,13 +56,17 @@ def log(*a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=100000)
-    ap.add_argument('--warmup', type=int, default=2000)
+    ap.add_argument('--steps', type=int, default=20, help='timed windows of --window-ms each')
+    ap.add_argument('--warmup', type=int, default=5, help='untimed windows after the settle')
+    ap.add_argument('--window-ms', type=float, default=1000.0)
+    ap.add_argument('--settle-ms', type=float, default=200.0)
     ap.add_argument('--dtype', default='float64', choices=['float64', 'float32'])
-    ap.add_argument('--cpu-steps', type=int, default=20000)
+    ap.add_argument('--cpu-ms', type=float, default=1000.0, help='simulated ms timed on the CPU oracle')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--cpu-leg', choices=['pinned', 'unpinned'], default=None,
+                    help=argparse.SUPPRESS)
     ap.add_argument('--profile-only', action='store_true',
-                    help='run warmup + timed steps only (for rocprofv3)')
+                    help='run the settle, warm-up and timed windows only (for rocprofv3)')
     ap.add_argument('--bptt-steps', type=int, default=5, help='timed gradient steps (0: skip)')
     ap.add_argument('--bptt-warmup', type=int, default=1)
     ap.add_argument('--no-pmc', action='store_true',
@@ -74,12 +93,16 @@ def main():
     ap.add_argument('--direct', action='store_true',
                     help='launch every window directly instead of replaying the captured graph')
     args = ap.parse_args()
+    if args.cpu_leg:
+        print(json.dumps(run_cpu_leg(args)), flush=True)
+        return
     if args.gpus != 1:
         print(f'bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported '
               '(multi-GPU sharding is design-only, see DESIGN.md)', file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
+    import torch
     from brainpy.state_amd.networks import brunel_delta
 
     if args.build_bench:
@@ -103,77 +126,95 @@ def main():
     sim, h = brunel_delta(order=2250, delay=('uniform', 1.0, 3.0), n_vm=10, dtype=args.dtype)
     desc = sim.build_desc()
     eng = sim._ensure_engine()
-    log(f'built Brunel-11250 in {time.time() - t0:.1f}s')
+    log(f'built Brunel-11250 in {time.time() - t0:.1f}s (window {eng.window()} steps per launch)')
+    win = int(round(args.window_ms / sim.dt))
+    settle = int(round(args.settle_ms / sim.dt))
     eng.reset()
     if args.direct:
         eng.set_timing(True)
-    eng.run(args.warmup)
-    log(f'warmup {args.warmup} steps done')
+    eng.run(settle)
+    for _ in range(args.warmup):
+        eng.run(win)
+    log(f'settle {settle} + warm-up {args.warmup} x {win} steps done')
 
-    # ---- timed run (bs_net_run returns after the stream has drained)
+    # ---- timed windows (bs_net_run returns after its stream has drained)
+    torch.cuda.synchronize()
+    walls, n_events, n_spikes = [], 0, 0
     t1 = time.perf_counter()
-    _, (gids, steps) = eng.run(args.steps)
+    for _ in range(args.steps):
+        tw = time.perf_counter()
+        _, (gids, steps) = eng.run(win)
+        walls.append(time.perf_counter() - tw)
+        n_events += eng.bytes()['n_events']
+        n_spikes += len(gids)
+    torch.cuda.synchronize()
     wall = time.perf_counter() - t1
-    by = eng.bytes()
-    n_events = by['n_events']
-    sim_s = args.steps * sim.dt / 1000.0
-    value = n_events / wall
-    rate = len(gids) / 11250 / sim_s
-    log(f'timed {args.steps} steps: {wall:.3f}s, {len(gids)} spikes ({rate:.1f} Hz), '
-        f'{n_events} events, {value:.3e} ev/s, RTF {wall / sim_s:.4f}')
+    sim_s = args.steps * win * sim.dt / 1000.0
+    value = n_events / wall if wall > 0 else 0.0
+    rate = n_spikes / 11250 / sim_s if sim_s > 0 else 0.0
+    log(f'timed {args.steps} x {win} steps: {wall:.3f}s (median window {np.median(walls) * 1e3:.2f} ms), '
+        f'{n_spikes} spikes ({rate:.1f} Hz), {n_events} events, {value:.3e} ev/s, RTF {wall / max(sim_s, 1e-12):.4f}')
     if args.profile_only:
         return
 
-    # ---- kernel timing (HIP events around every launch, separate run)
+    # ---- window kernel: HIP events around every launch on the engine's
+    # stream, over one more steady-state window
     eng.set_timing(True)
-    nprof = min(args.steps, 20000)
-    eng.run(nprof)
+    eng.run(win)
     tm = eng.timing()
-    by2 = eng.bytes()
+    by = eng.bytes()
     eng.set_timing(False)
     n_launch = max(1, tm['n_update'])
     kms = tm['update_ms'] / n_launch
-    kbytes = (by2['deliver_bytes'] + by2['update_bytes']) / n_launch
+    del_b, upd_b = by['deliver_bytes'] / n_launch, by['update_bytes'] / n_launch
+    kbytes = del_b + upd_b
     achieved = kbytes / (kms * 1e-3) / 1e9
-    log(f'k_window: {kms * 1e3:.2f} us/launch, {kbytes / 1e6:.3f} MB/launch '
-        f'(gather {by2["deliver_bytes"] / n_launch / 1e6:.3f} MB, update '
-        f'{by2["update_bytes"] / n_launch / 1e6:.3f} MB), {n_launch} launches')
-    traffic, tnote = (None, 'skipped (--no-pmc)') if args.no_pmc else pmc_traffic('k_window')
+    log(f'k_window: {kms * 1e3:.2f} us/launch over {n_launch} launches, {kbytes / 1e6:.3f} MB/launch '
+        f'(delivery {del_b / 1e6:.3f} MB = {del_b / (kms * 1e-3) / 1e9:.1f} GB/s, update '
+        f'{upd_b / 1e6:.3f} MB = {upd_b / (kms * 1e-3) / 1e9:.1f} GB/s)')
+    traffic, tnote = (None, 'skipped (--no-pmc)') if args.no_pmc else pmc_traffic('k_window', n_last=n_launch)
     log(f'k_window HBM traffic per launch: {traffic} ({tnote})')
     roofline = {'bound': 'hbm', 'kernel': 'k_window', 'achieved': round(achieved, 2),
                 'peak': 8000.0, 'unit': 'GB/s', 'frac': round(achieved / 8000.0, 5),
                 'traffic': traffic, 'traffic_note': tnote, 'bytes_per_launch': round(kbytes),
-                'us_per_launch': round(kms * 1e3, 3),
+                'delivery_bytes_per_launch': round(del_b), 'update_bytes_per_launch': round(upd_b),
+                'delivery_gbs': round(del_b / (kms * 1e-3) / 1e9, 2),
+                'update_gbs': round(upd_b / (kms * 1e-3) / 1e9, 2),
+                'us_per_launch': round(kms * 1e3, 3), 'launches': n_launch,
                 'launch_fraction_of_wall': round(tm['update_ms'] / max(1e-9, tm['total_ms']), 4)}
 
-    # ---- BPTT leg, before the CPU legs (their single-core load slows the host
-    # side of the next GPU measurement), with the network engine released
-    n_synapses = int(desc.c.projs[0].n_edges + desc.c.projs[1].n_edges +
-                     desc.c.projs[2].n_edges + desc.c.projs[3].n_edges)
+    # ---- BPTT leg, before the CPU legs, with the network engine released
+    n_synapses = int(sum(desc.c.projs[k].n_edges for k in range(4)))
     eng.close()
     sim._engine = None
     bptt = run_bptt(args) if args.bptt_steps > 0 else None
 
-    # ---- CPU baseline: the oracle on a bounded sample of the same network
+    # ---- CPU baseline: the oracle on one simulated second of the same
+    # network, in child processes (pinned to one core, then unpinned)
     cpu = None
     if not args.no_cpu:
-        import oracle
-        orc = oracle.OracleNet(desc)
-        t2 = time.perf_counter()
-        _, g_o, s_o = orc.run(args.cpu_steps)
-        cwall = time.perf_counter() - t2
-        rowlen = np.zeros(desc.ntot, dtype=np.int64)
-        for pj in desc.c.projs[:desc.c.n_projs]:
-            gid0 = sum(desc.c.pops[k].n for k in range(pj.pre_pop))
-            pre = np.ctypeslib.as_array(pj.pre, shape=(pj.n_edges,))
-            rowlen += np.bincount(pre + gid0, minlength=desc.ntot)
-        ev = int(rowlen[g_o].sum())
-        cpu = {'value': round(ev / cwall, 1), 'unit': 'events/s', 'cores': 1, 'kind': 'port',
-               'sample': f'first {args.cpu_steps} steps ({args.cpu_steps * sim.dt:.0f} ms) of the '
-                         f'same network, {len(g_o)} spikes, {cwall:.1f}s wall',
-               'rtf': round(cwall / (args.cpu_steps * sim.dt / 1000.0), 3)}
-        log(f'cpu oracle: {args.cpu_steps} steps in {cwall:.1f}s, {ev / cwall:.3e} ev/s')
+        import subprocess
+        legs = {}
+        for leg in ('pinned', 'unpinned'):
+            cmd = [sys.executable, os.path.join(ROOT, 'bench.py'), '--cpu-leg', leg,
+                   '--cpu-ms', str(args.cpu_ms), '--settle-ms', str(args.settle_ms), '--dtype', args.dtype]
+            res = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+            if res.returncode != 0:
+                log(f'cpu leg {leg} failed:\n{res.stderr[-2000:]}')
+                continue
+            legs[leg] = json.loads(res.stdout.strip().splitlines()[-1])
+            log(f'cpu oracle ({leg}): {legs[leg]["events_per_s"]:.3e} ev/s, RTF {legs[leg]["rtf"]:.3f}')
+        if 'pinned' in legs:
+            p = legs['pinned']
+            cpu = {'value': p['events_per_s'], 'unit': 'events/s', 'cores': 1, 'kind': 'port',
+                   'sample': f'simulated ms {args.settle_ms:.0f}-{args.settle_ms + args.cpu_ms:.0f} of the same '
+                             f'network (after the same settle from rest), {p["spikes"]} spikes, '
+                             f'{p["wall_s"]:.1f}s wall; C++ oracle, one thread, pinned to core {p["core"]}',
+                   'rtf': p['rtf'], 'cpu_model': p['cpu_model'], 'affinity_cpus': p['affinity'],
+                   'os_cpu_count': p['cpu_count'], 'loadavg': p['loadavg'],
+                   'unpinned': legs.get('unpinned')}
 
+    ms_step = wall / max(1, args.steps) * 1e3
     out = {
         'metric': 'synaptic events/s (Brunel-11250, 1 GPU)',
         'value': round(value, 1),
@@ -181,7 +222,10 @@ def main():
         'n_gpus': 1,
         'steps': args.steps,
         'warmup': args.warmup,
-        'ms_per_step': round(wall / args.steps * 1e3, 6),
+        'ms_per_step': round(ms_step, 4),
+        'step_unit': f'one simulated window of {args.window_ms:g} ms ({win} network steps of dt = {sim.dt} ms)',
+        'ms_per_step_median': round(float(np.median(walls)) * 1e3, 4),
+        'ms_per_network_step': round(ms_step / win, 6),
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
@@ -189,23 +233,70 @@ def main():
         'data': 'synthetic (seeded Brunel network, on-device Poisson drive)',
         'config': {'workload': 'brunel-11250 iaf_psc_delta, CE=900 CI=225, per-edge delays '
                                '1.0-3.0 ms, dt=0.1 ms', 'n_neurons': 11250,
-                   'n_synapses': n_synapses,
-                   'sim_ms': args.steps * sim.dt, 'parallelism': 'single-gpu'},
+                   'n_synapses': n_synapses, 'settle_ms': args.settle_ms,
+                   'sim_ms_timed': args.steps * win * sim.dt, 'parallelism': 'single-gpu'},
         'rtf': round(wall / sim_s, 6),
         'firing_rate_hz': round(rate, 2),
+        'n_events': int(n_events),
         'roofline': roofline,
         'cpu_baseline': cpu,
         'bptt': bptt,
     }
     print(json.dumps(out), flush=True)
+    if n_events == 0:
+        log('bench: the timed windows delivered no synaptic events; the measurement is invalid')
+        sys.exit(1)
 
 
-def pmc_traffic(kernel, skip=50):
+def run_cpu_leg(args):
+    """One CPU-oracle leg (child process): pinned to the first core of the
+    affinity set, or unpinned; settle from rest, then time ``--cpu-ms``."""
+    allowed = sorted(os.sched_getaffinity(0))
+    core = None
+    if args.cpu_leg == 'pinned':
+        core = allowed[0]
+        os.sched_setaffinity(0, {core})
+    model = ''
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    import oracle
+    from brainpy.state_amd.networks import brunel_delta
+    sim, h = brunel_delta(order=2250, delay=('uniform', 1.0, 3.0), n_vm=10, dtype=args.dtype)
+    desc = sim.build_desc()
+    import numpy as np
+    rowlen = np.zeros(desc.ntot, dtype=np.int64)
+    for pj in desc.c.projs[:desc.c.n_projs]:
+        gid0 = sum(desc.c.pops[k].n for k in range(pj.pre_pop))
+        pre = np.ctypeslib.as_array(pj.pre, shape=(pj.n_edges,))
+        rowlen += np.bincount(pre + gid0, minlength=desc.ntot)
+    orc = oracle.OracleNet(desc)
+    orc.run(int(round(args.settle_ms / sim.dt)))
+    n = int(round(args.cpu_ms / sim.dt))
+    la0 = os.getloadavg()
+    t0 = time.perf_counter()
+    _, g, _ = orc.run(n)
+    wall = time.perf_counter() - t0
+    la1 = os.getloadavg()
+    ev = int(rowlen[g].sum())
+    return {'leg': args.cpu_leg, 'core': core, 'events_per_s': round(ev / wall, 1), 'events': ev,
+            'spikes': int(len(g)), 'wall_s': round(wall, 3), 'rtf': round(wall / (n * sim.dt * 1e-3), 3),
+            'cores_used': 1, 'affinity': len(allowed), 'cpu_count': os.cpu_count(), 'cpu_model': model,
+            'loadavg': [round(la0[0], 2), round(la1[0], 2)]}
+
+
+def pmc_traffic(kernel, n_last):
     """HBM bytes per launch of ``kernel`` from two rocprofv3 counter passes
-    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) over a short profile-only
-    run, each scaled by what the counter reads for a 256 MiB device copy.
-    The first ``skip`` dispatches (the network's start-up transient) are left
-    out. Returns (bytes or None, note)."""
+    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) over a profile-only run of
+    the same network (settle, one warm-up window, one timed window), each
+    scaled by what the counter reads for a 256 MiB device copy. Only the last
+    ``n_last`` dispatches (the timed steady-state window) are averaged.
+    Returns (bytes or None, note)."""
     import glob
     import shutil
     import sqlite3
@@ -216,16 +307,17 @@ def pmc_traffic(kernel, skip=50):
         return None, 'rocprofv3 not found'
     tmp = tempfile.mkdtemp(prefix='bench_pmc_', dir='/tmp')
     env = dict(os.environ, TMPDIR='/tmp')
-    prof = [os.path.join(ROOT, 'bench.py'), '--steps', '2000', '--warmup', '200', '--profile-only',
+    prof = [os.path.join(ROOT, 'bench.py'), '--steps', '1', '--warmup', '1', '--profile-only',
             '--no-cpu', '--bptt-steps', '0', '--no-pmc']
     copy = ['-c', "import torch; x = torch.ones(1 << 28, dtype=torch.uint8, device='cuda'); "
                   "y = torch.empty_like(x); [y.copy_(x) for _ in range(4)]; torch.cuda.synchronize()"]
 
     def values(counter, argv, tag, name):
+        log(f'rocprofv3 --pmc {counter} pass ({tag})')
         out = os.path.join(tmp, tag)
         cmd = [exe, '--pmc', counter, '-d', out, '-o', tag, '--', sys.executable, '-u'] + argv
         try:
-            subprocess.run(cmd, env=env, cwd='/tmp', capture_output=True, timeout=90, check=False)
+            subprocess.run(cmd, env=env, cwd='/tmp', capture_output=True, timeout=150, check=False)
         except subprocess.TimeoutExpired:
             return None
         dbs = glob.glob(os.path.join(out, '**', '*.db'), recursive=True)
@@ -246,11 +338,11 @@ def pmc_traffic(kernel, skip=50):
         return None, 'rocprofv3 counter passes gave no rows'
     copy_kib = (1 << 28) / 1024.0
     ff, fw = copy_kib / (sum(cf) / len(cf)), copy_kib / (sum(cw) / len(cw))
-    f = fetch[skip:] or fetch
-    w = write[skip:] or write
+    f = fetch[-n_last:]
+    w = write[-n_last:]
     kib = sum(f) / len(f) * ff + sum(w) / len(w) * fw
     return round(kib * 1024.0), (f'FETCH_SIZE x {ff:.3f} + WRITE_SIZE x {fw:.3f} (copy-kernel '
-                                 f'scale), mean of {len(f)} dispatches after the first {skip}')
+                                 f'scale), mean of the last {len(f)} dispatches (one steady-state window)')
 
 
 def run_build_bench(args):

@@ -91,6 +91,7 @@ BS_LOSS_RATE_MSE, BS_LOSS_MEMBRANE_MSE = 0, 1
 _P = C.c_void_p
 _SIGS = {
     'bs_abi_version': (C.c_int, []),
+    'bs_source_hash': (C.c_char_p, []),
     'bs_last_error': (C.c_char_p, []),
     'bs_device_count': (C.c_int, []),
     'bs_net_create': (C.c_int, [C.POINTER(NetDesc), C.POINTER(_P)]),
@@ -151,6 +152,11 @@ def lib():
         fn.argtypes = args
     if handle.bs_abi_version() != BS_ABI_VERSION:
         raise EngineUnavailable('libbrainspike_hip.so ABI version mismatch; rebuild it')
+    from .build import source_hash
+    built, here = handle.bs_source_hash().decode(), source_hash()
+    if built != here:
+        raise EngineUnavailable(f'libbrainspike_hip.so was built from other sources (hash {built[:12]}, '
+                                f'sources here {here[:12]}); rebuild it with __graft_entry__.build()')
     _lib = handle
     return _lib
 

@@ -6,7 +6,22 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = [os.path.join(HERE, 'csrc', f) for f in ('bs_net.hip', 'bs_bptt.hip', 'bs_conn.hip')]
+# every file the library is compiled from (sources and the headers they include)
+DEPS = SRC + [os.path.join(HERE, 'csrc', f) for f in ('bs_math.h', 'bs_models.h')] + [
+    os.path.join(HERE, '..', '..', 'include', 'brainspike_hip.h')]
 OUT = os.path.join(HERE, 'libbrainspike_hip.so')
+
+
+def source_hash() -> str:
+    """SHA-256 over the library's source files (name and bytes, in DEPS
+    order); compiled into the library and checked by the loader."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in DEPS:
+        h.update(os.path.basename(f).encode() + b'\0')
+        with open(f, 'rb') as fh:
+            h.update(fh.read())
+    return h.hexdigest()
 
 # -ffp-contract=off: every arithmetic op is one IEEE-rounded op (no FMA
 # contraction), which the FP64 bit-exact parity with the CPU oracle relies on.
@@ -19,8 +34,8 @@ LIBS = ['-L/opt/rocm/lib', '-lrocblas', '-Wl,-rpath,/opt/rocm/lib']
 
 def build(verbose: bool = False) -> str:
     hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
-    tmp = OUT + '.tmp'
-    cmd = [hipcc] + FLAGS + ['-o', tmp] + SRC + LIBS
+    tmp = OUT + f'.tmp{os.getpid()}'
+    cmd = [hipcc] + FLAGS + [f'-DBS_SOURCE_HASH="{source_hash()}"', '-o', tmp] + SRC + LIBS
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError('hipcc failed:\n' + res.stdout + res.stderr)

@@ -1119,6 +1119,10 @@ struct bs_net {
 
 // Public entry points keep the C linkage of their declarations in the header.
 int bs_abi_version(void) { return BS_ABI_VERSION; }
+#ifndef BS_SOURCE_HASH
+#define BS_SOURCE_HASH "unknown"
+#endif
+const char* bs_source_hash(void) { return BS_SOURCE_HASH; }
 const char* bs_last_error(void) { return g_err.c_str(); }
 int bs_device_count(void) {
   int n = 0;
@@ -2393,7 +2397,7 @@ struct ChunkJob {
 // pin_vm, its V_m rows into pinned memory, and marks its end with
 // ev_chunk[j.par]. Returns after queueing; nothing here waits for the device.
 static int enqueue_chunk(bs_net* net, const ChunkJob& j, bool pin_vm, std::vector<hipEvent_t>& evs,
-                         double state_bytes) {
+                         double win_bytes, double step_bytes) {
   net->rec_par = j.par;
   HIPCHK(hipMemsetAsync(net->d_rec_count + j.par, 0, sizeof(unsigned long long), net->stream));
   if (net->timing) {
@@ -2433,7 +2437,7 @@ static int enqueue_chunk(bs_net* net, const ChunkJob& j, bool pin_vm, std::vecto
     HIPCHK(hipGraphLaunch(G, net->stream));
     net->last_win = net->W;
     net->n_update += j.nwin;
-    net->b_update += state_bytes * j.chunk;
+    net->b_update += win_bytes * j.nwin + step_bytes * j.chunk;
   } else {
     for (int w = 0; w < j.nwin; ++w) {
       const long long n0 = j.c0 + (long long)w * net->W;
@@ -2448,7 +2452,7 @@ static int enqueue_chunk(bs_net* net, const ChunkJob& j, bool pin_vm, std::vecto
       if (net->timing) HIPCHK(hipEventRecord(evs[2 * w + 1], net->stream));
       HIPCHK(hipGetLastError());
       net->n_update++;
-      net->b_update += state_bytes * Wc;
+      net->b_update += win_bytes + step_bytes * Wc;
     }
   }
   HIPCHK(hipMemcpyAsync(net->h_cnt + j.par, net->d_rec_count + j.par, sizeof(unsigned long long),
@@ -2610,11 +2614,12 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
     launch_pois_fill(net, (int)net->step);
     HIPCHK(hipGetLastError());
   }
-  // Algorithmic update bytes per network step: every neuron reads and writes
-  // its state variables and its int32 refractory counter and reads one
-  // arrival slot (4 B count), plus 8 B per recorded V_m sample.
-  double state_bytes = 8.0 * net->n_vm_dev;
-  for (const DevPop& p : net->pops) state_bytes += (double)p.n * (2.0 * p.n_state * net->tsize + 8.0 + 4.0);
+  // Algorithmic update bytes: once per window every neuron reads and writes
+  // its state variables and its int32 refractory counter (the window's steps
+  // run with the state in registers); every step writes 8 B per V_m sample.
+  double win_bytes = 0.0;
+  for (const DevPop& p : net->pops) win_bytes += (double)p.n * (2.0 * p.n_state * net->tsize + 8.0);
+  const double step_bytes = 8.0 * net->n_vm_dev;
   long long done = 0;
   int rc = BS_OK;
   int par = 0;
@@ -2629,7 +2634,7 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
       cur.done = done;
       cur.par = par;
       cur.nwin = (int)((cur.chunk + net->W - 1) / net->W);
-      rc = enqueue_chunk(net, cur, pin_vm, evs, state_bytes);
+      rc = enqueue_chunk(net, cur, pin_vm, evs, win_bytes, step_bytes);
       if (rc != BS_OK) break;
       queued = true;
       net->step += cur.chunk;

@@ -13,10 +13,6 @@ from . import units as u
 __all__ = ['stdp_synapse']
 
 
-def _nest_sign(v: float) -> int:
-    return int(v >= 0.0) - int(v < 0.0)
-
-
 class stdp_synapse:
     """Pair-based STDP synapse spec (NEST ``stdp_synapse``)."""
     __module__ = 'brainpy.state_amd'
@@ -46,10 +42,13 @@ class stdp_synapse:
         if self.tau_minus <= 0.0:
             raise ValueError("'tau_minus' must be > 0.")
         w0 = float(getattr(weight, 'value', weight)) if not hasattr(weight, 'sample') else 1.0
-        if _nest_sign(w0) != _nest_sign(self.Wmax):
-            raise ValueError('Weight and Wmax must have same sign.')
+        # NEST's sign convention counts zero as positive: w and Wmax must lie
+        # on the same side of the split [0, inf) | (-inf, 0)
+        if (w0 < 0.0) != (self.Wmax < 0.0):
+            raise ValueError(f'stdp_synapse: the weight ({w0}) and Wmax ({self.Wmax}) must have the same '
+                             'sign (zero counts as positive)')
         if self.Kplus < 0.0:
-            raise ValueError('Kplus must be non-negative.')
+            raise ValueError(f'stdp_synapse: Kplus must be >= 0, got {self.Kplus}')
         if self.Kplus != 0.0:
             raise NotImplementedError('Kplus != 0 (traces start at 0 in the engine)')
 

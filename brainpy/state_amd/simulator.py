@@ -21,7 +21,7 @@ from . import _lib as L
 from .devices import dc_generator, multimeter, poisson_generator, spike_recorder
 from .dist import Distribution
 from .models import Neuron, aeif_cond_alpha, iaf_psc_delta
-from .nodeview import NodeView, _Segment, _flat_size
+from .nodeview import NodeView, Selection, population_size
 from .plasticity import stdp_synapse
 from .rules import ConnRule, _OneToOne, all_to_all
 
@@ -35,11 +35,12 @@ class _GeneratorSpec:
         self.params = params
 
 
-class _GenSegment(_Segment):
+class _GenSegment(Selection):
+    """A generator not yet realised: its spec, no population or indices."""
+
     def __init__(self, spec):
-        object.__setattr__(self, 'spec', spec)
-        object.__setattr__(self, 'population', None)
-        object.__setattr__(self, 'indices', np.arange(0))
+        super().__init__(None, ())
+        self.spec = spec
 
 
 def _is_generator(model_cls) -> bool:
@@ -297,7 +298,7 @@ class Simulator:
             return NodeView([_GenSegment(_GeneratorSpec(model_cls, p))])
         mod = model_cls(size, **p)
         if isinstance(mod, (spike_recorder, multimeter)):
-            return NodeView([_Segment(mod, np.arange(1))])
+            return NodeView([Selection(mod, np.arange(1))])
         if not isinstance(mod, Neuron):
             raise TypeError(f'{model_cls!r} is not a supported neuron model or device')
         self._pops.append(mod)
@@ -470,7 +471,7 @@ class Simulator:
         else:
             d = delay_to_steps(delay, self.dt)
             if np.ndim(d) == 1:
-                if d.shape[0] == _flat_size(pre_pop) and d.shape[0] != E:
+                if d.shape[0] == population_size(pre_pop) and d.shape[0] != E:
                     if pre_l is None:
                         pre_l, post_l = dev.host()
                     d = d[pre_l]             # axonal: one delay per pre neuron

@@ -169,6 +169,11 @@ typedef struct bs_net bs_net;
 
 /* ---- library ------------------------------------------------------------ */
 int bs_abi_version(void);
+/* SHA-256 (hex) of the sources this library was compiled from, as the build
+ * computed it (brainpy/state_amd/build.py source_hash); the Python loader
+ * refuses a library whose hash differs from the sources beside it. No
+ * counterpart in the reference, whose Python is its own source. */
+const char* bs_source_hash(void);
 const char* bs_last_error(void);
 /* Number of visible HIP devices (0 when none); never fails. */
 int bs_device_count(void);

@@ -52,10 +52,26 @@ LIB = os.path.join(HERE, 'liboracle.so')
 _lib = None
 
 
+DEPS = [os.path.join(HERE, 'oracle.cpp'), os.path.join(HERE, 'oracle.h'),
+        os.path.join(HERE, '..', 'include', 'brainspike_hip.h')]
+
+
+def source_hash():
+    """SHA-256 over the oracle's source files; compiled into liboracle.so so
+    that a library built from older sources is rebuilt before use."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in DEPS:
+        h.update(os.path.basename(f).encode() + b'\0')
+        with open(f, 'rb') as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
 def build():
-    tmp = LIB + '.tmp'
+    tmp = LIB + f'.tmp{os.getpid()}'
     cmd = ['g++', '-O2', '-std=c++17', '-fPIC', '-shared', '-ffp-contract=off',
-           '-o', tmp, os.path.join(HERE, 'oracle.cpp')]
+           f'-DORC_SOURCE_HASH="{source_hash()}"', '-o', tmp, os.path.join(HERE, 'oracle.cpp')]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError('oracle build failed:\n' + res.stdout + res.stderr)
@@ -66,7 +82,8 @@ def build():
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB):
+        want = source_hash()
+        if not os.path.exists(LIB) or _built_hash(LIB) != want:
             build()
         h = C.CDLL(LIB)
         P = C.c_void_p
@@ -99,6 +116,16 @@ def lib():
             f.argtypes = args
         _lib = h
     return _lib
+
+
+def _built_hash(path):
+    """The source hash compiled into a built liboracle.so, read in a child
+    process so that a stale library is never loaded into this one."""
+    import sys
+    code = ('import ctypes, sys; f = ctypes.CDLL(sys.argv[1]).orc_source_hash; '
+            'f.restype = ctypes.c_char_p; print(f().decode())')
+    res = subprocess.run([sys.executable, '-c', code, path], capture_output=True, text=True)
+    return res.stdout.strip() if res.returncode == 0 else None
 
 
 def _check(rc):

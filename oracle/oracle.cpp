@@ -14,6 +14,11 @@
 #include <string>
 #include <vector>
 
+#ifndef ORC_SOURCE_HASH
+#define ORC_SOURCE_HASH "unknown"
+#endif
+extern "C" const char* orc_source_hash(void) { return ORC_SOURCE_HASH; }
+
 static thread_local std::string g_err;
 static int fail(int code, const std::string& m) {
   g_err = m;

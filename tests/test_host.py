@@ -188,3 +188,39 @@ def test_stats_rate_cv_and_divergence():
     b_s = a_s.copy()
     b_s[-1] += 1
     assert stats.first_divergence(a_g, a_s, a_g, b_s) == 15
+
+
+def test_nodeview_algebra_and_indexing():
+    """NodeView: concatenation, sizes, and indexing over the concatenated
+    positions (single and multi-population views, slices, masks, arrays)."""
+    from brainpy.state_amd import NodeView
+    from brainpy.state_amd.models import iaf_psc_delta
+    a, b = iaf_psc_delta(5), iaf_psc_delta(3)
+    va, vb = NodeView.of(a), NodeView.of(b)
+    v = va + vb
+    assert len(v) == v.size == 8 and len(v.segments) == 2
+    assert [s.population for s in v.segments] == [a, b]
+    # single population: the reference's slicing
+    s = va[1:4]
+    assert len(s.segments) == 1 and s.segments[0].population is a
+    np.testing.assert_array_equal(s.segments[0].indices, [1, 2, 3])
+    np.testing.assert_array_equal(va[-1].segments[0].indices, [4])
+    # across populations: positions 3..6 = a[3], a[4], b[0], b[1]
+    x = v[3:7]
+    assert [sg.population for sg in x.segments] == [a, b]
+    np.testing.assert_array_equal(x.segments[0].indices, [3, 4])
+    np.testing.assert_array_equal(x.segments[1].indices, [0, 1])
+    # a mask and an index array with an order change split into runs
+    m = np.zeros(8, bool)
+    m[[0, 6, 7]] = True
+    y = v[m]
+    np.testing.assert_array_equal(y.segments[0].indices, [0])
+    np.testing.assert_array_equal(y.segments[1].indices, [1, 2])
+    z = v[np.array([6, 0, 1])]
+    assert [sg.population for sg in z.segments] == [b, a]
+    np.testing.assert_array_equal(z.segments[1].indices, [0, 1])
+    assert len(v[8:]) == 0
+    with pytest.raises(IndexError):
+        v[8]
+    with pytest.raises(TypeError):
+        va + 1
