@@ -66,24 +66,48 @@ def _map_local(sel, seg_idx):
     return sel[np.asarray(seg_idx, dtype=np.int64)].astype(np.int32, copy=False)
 
 
-def delay_to_steps(delay, dt):
-    """Delay (ms, scalar or array) -> integer steps; delay/dt must be integral.
-
-    The reference's InputDelay reads ``floor`` and ``ceil`` frames and lerps
-    between them (delay.py:114-128); for integral ``delay/dt`` that is an exact
-    single-frame read, the only case the engine implements.
-    """
-    if delay is None:
-        return 0
+def nest_delay_steps(delay, dt):
+    """Per-edge delays (ms) -> whole steps with NEST's rounding, as the
+    reference's per-edge delay path converts them (_kernel/delays.py:50-108):
+    ``floor(d * (1/dt) + 0.5)`` (round half up) in float64, with 1/dt snapped
+    to the nearest integer when it is within 1e-6 of one."""
     d = np.asarray(u.magnitude(delay, 'time'), dtype=np.float64)
-    steps = d / dt
-    r = np.rint(steps)
-    if np.any(np.abs(steps - r) > 1e-9 * np.maximum(1.0, np.abs(r))):
-        raise ValueError('delays must be integer multiples of dt (fractional delays are '
-                         'not supported by the engine)')
-    if np.any(r < 0):
+    if d.size and (not np.all(np.isfinite(d)) or np.min(d) < 0):
+        raise ValueError(f'delays must be finite and non-negative, got min {np.min(d)} ms')
+    spm = 1.0 / float(dt)
+    k = np.rint(spm)
+    if k >= 1 and abs(spm - k) <= 1e-6 * k:
+        spm = k
+    out = np.floor(d * spm + 0.5)
+    if out.size and out.max() >= (1 << 24):
+        raise ValueError('delay must be below 2^24 steps')
+    return int(out) if out.ndim == 0 else out.astype(np.int32)
+
+
+def input_delay_steps(delay, dt, dtype=np.float64):
+    """A homogeneous or per-pre-neuron (axonal) delay as the reference's
+    InputDelay reads it (delay.py:114-128): ``steps = delay / dt`` in the
+    network dtype, frames ``lo = floor(steps)`` and ``lo + 1``, output
+    ``x[lo] * (1 - frac) + x[lo + 1] * frac`` with ``frac = steps - lo``.
+
+    Returns ``(lo, frac)``: ``lo`` int (scalar) or int32 array, ``frac`` in
+    ``dtype`` (scalar or array), 0 where ``delay / dt`` is integral. Ratios
+    that are integral in exact arithmetic need not be in floating point (in
+    float64, 1.2 / 0.1 = 11.999999999999998): those lerp too, as the
+    reference does."""
+    if delay is None:
+        return 0, dtype(0.0)
+    d = np.asarray(u.magnitude(delay, 'time'), dtype=np.float64)
+    if d.size and (not np.all(np.isfinite(d)) or np.min(d) < 0):
         raise ValueError('delays must be >= 0')
-    return int(r) if r.ndim == 0 else r.astype(np.int32)
+    steps = d.astype(dtype) / dtype(dt)
+    lo = np.floor(steps)
+    frac = (steps - lo).astype(dtype)
+    if lo.size and lo.max() >= (1 << 24) - 1:
+        raise ValueError('delay must be below 2^24 steps')
+    if lo.ndim == 0:
+        return int(lo), dtype(frac)
+    return lo.astype(np.int32), frac
 
 
 def seq_sum_table(w, kmax, dtype):

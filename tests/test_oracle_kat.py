@@ -316,3 +316,40 @@ def test_event_conservation_delivered_equals_spikes_times_outdegree():
     o = oracle.OracleNet(sim.build_desc())
     v, g, s = o.run(10)
     assert np.sum(v[-1]) == pytest.approx(50 * 7 * 40 / 50 * 1.0 * 1.0, rel=1e-12)
+
+
+def test_fixed_point_delivery_is_the_exact_sum_within_quantisation():
+    """Per-edge weights are summed as 2^-F fixed-point quanta and converted
+    once. Pinned against math.fsum of the arriving weights: every post neuron
+    of a passive population (decay exactly 1, E_L = 0, threshold out of
+    reach) holds, after the single arrival step, exactly the delivered sum y;
+    |y - fsum| must stay within k 2^-(F+1) (quantising k weights) plus half
+    an ulp (the one conversion), for weights spread over 12 decades and both
+    signs. A dropped, doubled or sign-flipped term breaks the bound."""
+    from brainpy.state_amd import Simulator, all_to_all, spike_recorder
+    from brainpy.state_amd import units as u
+    sim = Simulator(dt=0.1 * u.ms, dtype='float64')
+    n_pre, n_post = 60, 40
+    pre = sim.create(iaf_psc_delta, n_pre, E_L=0., V_th=20., V_reset=0., tau_m=10., t_ref=1e6,
+                     V_initializer=Constant(21.))
+    post = sim.create(iaf_psc_delta, n_post, E_L=0., V_th=1e300, V_reset=0., tau_m=1e30, t_ref=2.,
+                      V_initializer=Constant(0.))
+    rng = np.random.default_rng(5)
+    w = rng.standard_normal(n_pre * n_post) * 10.0 ** rng.uniform(-6, 6, n_pre * n_post)
+    proj = sim.connect(pre, post, rule=all_to_all, weight=w * u.mV, delay=1.0 * u.ms)
+    sim.connect(pre, sim.create(spike_recorder))
+    desc = sim.build_desc()
+    orc = oracle.OracleNet(desc)
+    _, g, s = orc.run(14)
+    assert len(g) == n_pre and np.all(s == 0)  # every pre neuron fires once, at step 0
+    y = orc.state(1, 0, n_post)                # arrival at 0 + 1 + 10 = step 11
+    F = desc.c.channels[desc.c.projs[desc.proj_index[id(proj)]].channel].fixed_shift
+    pre_i, post_j = np.asarray(proj.pre_idx), np.asarray(proj.post_idx)
+    wt = np.asarray(proj.weight, np.float64)
+    for j in range(n_post):
+        ws = wt[post_j == j]
+        exact = math.fsum(ws.tolist())
+        bound = len(ws) * 2.0 ** -(F + 1) + 0.5 * math.ulp(exact)
+        assert abs(y[j] - exact) <= bound, (j, y[j], exact, bound)
+        # and the quantisation bound is far below the magnitude of the sum
+        assert bound <= 1e-12 * np.abs(ws).sum() + math.ulp(exact)
