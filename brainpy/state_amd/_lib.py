@@ -15,7 +15,7 @@ __all__ = ['lib', 'EngineUnavailable', 'EngineError', 'check', 'BS_NPARAM', 'BS_
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libbrainspike_hip.so')
 
-BS_ABI_VERSION = 6
+BS_ABI_VERSION = 7
 BS_NPARAM = 40
 BS_NSTATE = 9
 BS_F64, BS_F32 = 0, 1
@@ -51,7 +51,7 @@ class Channel(C.Structure):
                 ('max_count', C.c_int32), ('fixed_shift', C.c_int32), ('lam', C.c_double),
                 ('t_min_step', C.c_int64), ('t_max_step', C.c_int64), ('seed', C.c_uint32),
                 ('delay_steps', C.c_int32), ('gen_index', C.POINTER(C.c_int32)),
-                ('ac', C.c_double * 4)]
+                ('ac', C.c_double * 4), ('delay_frac', C.c_double)]
 
 
 class Stdp(C.Structure):

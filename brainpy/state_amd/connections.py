@@ -154,9 +154,12 @@ class SynapseCollection:
         return np.asarray(p.weight, dtype=np.float64)[sp.order][sp.kept]
 
     def _delays(self, sp):
-        d = sp.proj.delay
-        steps = np.full(sp.kept.shape[0], int(d)) if np.ndim(d) == 0 else np.asarray(d)[sp.order][sp.kept]
-        return steps.astype(np.float64) * self._sim.dt
+        d = np.asarray(sp.proj.delay, dtype=np.float64)
+        fr = getattr(sp.proj, 'delay_frac', None)
+        if fr is not None:                      # an InputDelay read between two frames
+            d = d + np.asarray(fr, dtype=np.float64)
+        steps = np.full(sp.kept.shape[0], float(d)) if np.ndim(d) == 0 else d[sp.order][sp.kept]
+        return steps * self._sim.dt
 
     # -- write -----------------------------------------------------------
     def set(self, key, value):
@@ -213,12 +216,28 @@ class SynapseCollection:
             if sp.proj.plastic:
                 raise ValueError(f"delay of the '{_model_name(sp.proj)}' projection is not settable "
                                  "via SynapseCollection (a plastic delay is fixed at connect() time).")
+        from .simulator import input_delay_steps
         q_ms = float(u.magnitude(value, 'time'))
-        steps = int(round(q_ms / self._sim.dt))     # rounded to the dt grid (NEST semantics)
+        # rounded to the dt grid, then the projection's InputDelay is rebuilt
+        # with that delay (connection_introspection.py:132-151), which reads
+        # it as delay.py:114-128 does: steps * dt / dt can carry a fraction in
+        # the network dtype (7 * 0.1 / 0.1 = 7.000000000000001)
+        steps = max(int(round(q_ms / self._sim.dt)), 0)
+        rounded = steps * self._sim.dt
+        lo, frac = input_delay_steps(rounded, self._sim.dt, self._sim.dtype)
         for sp in self._spanned:
-            sp.proj.delay = max(steps, 0)
+            if frac != 0 and not sp.proj.graded and sp.proj.pre_pop.ref_count(self._sim.dt) < 1:
+                raise NotImplementedError(
+                    "comm='sparse' with a fractional delay needs a presynaptic refractory period of "
+                    'at least one step')
+        for sp in self._spanned:
+            sp.proj.delay = lo
+            sp.proj.delay_frac = frac if frac != 0 else None
+            if frac != 0 and sp.proj.device_edges is not None:
+                sp.proj.pre_idx                 # the two-frame edges are built from host arrays
+                sp.proj.device_edges = None
         self._sim._connections_changed()
-        return steps * self._sim.dt
+        return rounded
 
 
 def _weight_unit(pop):

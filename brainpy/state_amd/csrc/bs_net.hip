@@ -114,6 +114,7 @@ struct DevChan {
   const double* cdf;             // POISSON: CDF table of Poisson(lam / parts)
   const unsigned short* guide;   // POISSON: 64-entry guide table
   long long* pacc;               // FIXED fed by a plastic projection: deposits per post
+  double frac;                   // POISSON: InputDelay lerp weight of frame delay + 1 (0: whole steps)
 };
 
 // One block's tile: everything its prologue needs before the population's
@@ -270,6 +271,7 @@ struct ChReg {
   const unsigned short* guide;  // Poisson guide table
   const int* gen_index;
   long long* pacc;              // plastic deposits (FIXED channel of a plastic projection)
+  double frac;                  // POISSON: lerp weight of the frame one step further back
 };
 
 template <typename T>
@@ -293,6 +295,7 @@ __device__ __forceinline__ ChReg<T> load_ch(const DevChan& ch, const unsigned ch
   c.weight = (T)ch.weight;
   c.gen_index = ch.gen_index;
   c.pacc = ch.pacc;
+  c.frac = ch.frac;
   const bool staged = ch.lds_off >= 0;
   c.tab = staged ? (const T*)(tab + ch.lds_off) : tables + ch.table_off;
   c.tabg = tables + ch.table_off;
@@ -342,7 +345,15 @@ __device__ __forceinline__ T chan_value(const UpdCtx<T>& u, const ChReg<T>& c, i
     uint32_t cn = 0;
     if (e >= 0 && sgs >= 0 && sgs > c.tmin && sgs <= c.tmax)
       cn = poisson_inv(c.seed, (uint32_t)sgs, (uint32_t)e, c.cdf, c.guide, c.cdf_len, c.parts);
-    return (T)cn * c.weight;
+    if (c.frac == 0.0) return (T)cn * c.weight;
+    // InputDelay lerp (delay.py:114-128): frames d and d + 1 of the
+    // generator's output, x_lo * (1 - frac) + x_hi * frac, then * weight
+    const long long sgh = sgs - 1;
+    uint32_t ch = 0;
+    if (e >= 0 && sgh >= 0 && sgh > c.tmin && sgh <= c.tmax)
+      ch = poisson_inv(c.seed, (uint32_t)sgh, (uint32_t)e, c.cdf, c.guide, c.cdf_len, c.parts);
+    const T f = (T)c.frac;
+    return ((T)cn * ((T)1 - f) + (T)ch * f) * c.weight;
   }
   if (c.kind == BS_CH_CURRENT) {  // dc_generator: amplitude * weight inside [tmin, tmax)
     const int j = u.start + lane;
@@ -1578,7 +1589,9 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
         dc.tmax = ch.t_max_step;
         dc.seed = ch.seed;
         dc.delay = ch.delay_steps;
+        dc.frac = ch.delay_frac;
         if (ch.delay_steps < 0) BAD("Poisson delay must be >= 0");
+        if (!(ch.delay_frac >= 0.0 && ch.delay_frac < 1.0)) BAD("Poisson delay_frac must be in [0, 1)");
         if (ch.lam == 0.0) dc.tmax = -1;  // rate <= 0: all-zero output
         if (ch.gen_index) {
           int* g = nullptr;
@@ -1917,7 +1930,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
         const DevPop& dp = net->pops[p];
         for (int c = dp.ch_begin; c < dp.ch_begin + dp.ch_count; ++c) {
           DevChan& dc = net->chans[c];
-          if (dc.kind != BS_CH_POISSON) continue;
+          if (dc.kind != BS_CH_POISSON || dc.frac != 0.0) continue;  // lerped channels draw in place
           if ((long long)(dc.cdf_len - 1) * dc.parts > 65535) continue;
           dc.pre = 1;
           net->pop_pc[p] = 1;

@@ -143,6 +143,13 @@ class StaticProjection:
         self.weight = weight                         # float or (E,) array
         self.delay = delay                           # int steps or (E,) int array
         self.receptor = receptor
+        # fractional part of an InputDelay read (scalar or (E,) array in the
+        # network dtype; None when every delay is whole steps): the edge then
+        # delivers from frames delay and delay + 1 (delay.py:114-128), as two
+        # events of its weight (comm='sparse', binary) or as the weighted
+        # parts w (1 - frac) and w frac (comm='dense' and receptor scatter)
+        self.delay_frac = None
+        self.graded = False
 
     @property
     def pre_idx(self):
@@ -200,6 +207,7 @@ class _GenChannel:
         self.delay = delay
         self.receptor = receptor
         self.seed = seed
+        self.delay_frac = 0.0   # InputDelay lerp weight of frame delay + 1
 
 
 # --------------------------------------------------------------------- result
@@ -335,9 +343,15 @@ class Simulator:
                 allow_multapses=True, seed=None):
         """Connect ``pre`` to ``post`` or register a recorder tap (simulator.py:606-683).
 
-        ``delay`` may also be a per-edge array (one entry per realised edge, in
-        the rule's edge order) or a :mod:`~brainpy.state_amd.dist` distribution
-        sampled per edge; both must be integer multiples of ``dt``.
+        A scalar ``delay``, or an array with one entry per pre neuron (axonal),
+        is read as the reference's InputDelay reads it (delay.py:114-128):
+        ``delay / dt`` in the network dtype, frames ``floor`` and ``floor + 1``
+        mixed by the fraction (``comm='dense'`` and receptor inputs: graded
+        parts ``w (1 - f)`` and ``w f``; ``comm='sparse'``: an event in each
+        frame). ``delay`` may also be a per-edge array (one entry per realised
+        edge, in the rule's edge order) or a :mod:`~brainpy.state_amd.dist`
+        distribution sampled per edge, rounded to whole steps as the
+        reference's per-edge delays are (``_kernel/delays.py:50-108``).
         """
         if comm not in ('dense', 'sparse'):
             raise ValueError(f"comm must be 'dense' or 'sparse', got {comm!r}")
@@ -367,7 +381,7 @@ class Simulator:
         for pre_seg, w_seg in zip(pre.segments, seg_weights):
             for post_seg in post.segments:
                 pr = self._connect_pair(pre_seg, post_seg, rule, w_seg, delay, allow_autapses,
-                                        allow_multapses, seed, receptor_type, synapse)
+                                        allow_multapses, seed, receptor_type, synapse, comm)
                 if pr is not None:
                     projs.append(pr)
         self._invalidate()
@@ -402,7 +416,7 @@ class Simulator:
         return 0
 
     def _connect_pair(self, pre_seg, post_seg, rule, weight, delay, allow_autapses,
-                      allow_multapses, seed, receptor_type, synapse):
+                      allow_multapses, seed, receptor_type, synapse, comm='dense'):
         ordinal = next(self._proj_counter)
         post_pop = post_seg.population
         if post_pop is None or not isinstance(post_pop, Neuron):
@@ -436,10 +450,11 @@ class Simulator:
             w = 1.0 if weight is None else u.magnitude(weight, wdim)
             if np.ndim(w) != 0:
                 raise NotImplementedError('generator connections take a scalar weight')
-            rec = _GenChannel(ordinal, gen, post_pop, post_idx, float(w),
-                              delay_to_steps(delay, self.dt), receptor, params['rng_seed'])
-            if np.ndim(rec.delay) != 0:
+            lo, frac = input_delay_steps(delay, self.dt, self.dtype)
+            if np.ndim(lo) != 0:
                 raise NotImplementedError('generator connections take a scalar delay')
+            rec = _GenChannel(ordinal, gen, post_pop, post_idx, float(w), lo, receptor, params['rng_seed'])
+            rec.delay_frac = float(frac)
             self._records.append(rec)
             return rec
         pre_pop = pre_seg.population
@@ -486,21 +501,52 @@ class Simulator:
             w = float(w.reshape(()))
         elif w.shape != (E,):
             raise ValueError(f'parameter array shape {w.shape} does not match target {(E,)}')
-        # delays: scalar, per-pre-neuron (axonal), per-edge, or a distribution
+        # delays: per edge (a distribution, or an array with one entry per
+        # edge) are whole steps by NEST's rounding (the reference's per-edge
+        # path, _kernel/delays.py:50-108); homogeneous or per pre neuron
+        # (axonal) delays are read as InputDelay reads them, lerping between
+        # the two bracketing frames when delay/dt is not integral
         if synapse is not None and delay is None:
             delay = synapse.delay
+        frac = None
         if isinstance(delay, Distribution):
-            dv = delay.sample((E,), dseed + 104729)
-            d = delay_to_steps(dv, self.dt)
+            d = nest_delay_steps(delay.sample((E,), dseed + 104729), self.dt)
         else:
-            d = delay_to_steps(delay, self.dt)
-            if np.ndim(d) == 1:
-                if d.shape[0] == population_size(pre_pop) and d.shape[0] != E:
-                    if pre_l is None:
-                        pre_l, post_l = dev.host()
-                    d = d[pre_l]             # axonal: one delay per pre neuron
-                elif d.shape[0] != E:
+            dm = None if delay is None else np.asarray(u.magnitude(delay, 'time'), dtype=np.float64)
+            # an array as long as the pre population is axonal, as the
+            # reference's InputDelay over the pre-population vector reads it
+            # (event_proj.py:196-197), also when it happens to equal E
+            if dm is not None and dm.ndim == 1 and dm.shape[0] == population_size(pre_pop):
+                if pre_l is None:
+                    pre_l, post_l = dev.host()
+                lo, fr = input_delay_steps(dm, self.dt, self.dtype)
+                d, frac = lo[pre_l], fr[pre_l]       # axonal: one delay per pre neuron
+            elif dm is not None and dm.ndim >= 1:
+                if dm.ndim != 1 or dm.shape[0] != E:
                     raise ValueError('delay array must have one entry per pre neuron or per edge')
+                d = nest_delay_steps(dm, self.dt)
+            else:
+                d, frac = input_delay_steps(delay, self.dt, self.dtype)
+            if frac is not None and not np.any(frac != 0):
+                frac = None
+        graded = comm == 'dense' or isinstance(post_pop, aeif_cond_alpha)
+        if frac is not None:
+            steps = np.asarray(u.magnitude(delay, 'time'), dtype=np.float64) / self.dt
+            if synapse is not None:
+                raise NotImplementedError(
+                    f'a plastic projection needs a delay of whole steps; delay / dt = {np.ravel(steps)[0]!r} '
+                    'is fractional in the network dtype, where the reference lerps the delayed pre '
+                    'spikes (delay.py:114-128)')
+            if not graded and pre_pop.ref_count(self.dt) < 1:
+                raise NotImplementedError(
+                    "comm='sparse' with a fractional delay needs a presynaptic refractory period of at "
+                    'least one step (an event is the OR of the two bracketing frames, delay.py:114-128 '
+                    'with BinaryArray, and a neuron that can fire on consecutive steps would put a '
+                    'spike in both)')
+            if dev is not None:                  # the two-frame edges are built from host arrays
+                if pre_l is None:
+                    pre_l, post_l = dev.host()
+                dev = None
         if synapse is not None:
             if pre_l is None:                # plastic projections keep host edge arrays
                 pre_l, post_l = dev.host()
@@ -510,6 +556,8 @@ class Simulator:
         else:
             pr = StaticProjection(ordinal, pre_pop, post_pop, pre_l, post_l, w, d, receptor,
                                   device_edges=dev)
+            pr.delay_frac = frac
+            pr.graded = graded
         self._records.append(pr)
         return pr
 
@@ -665,6 +713,48 @@ def _index_channel(val, i, k):
 
 
 # ------------------------------------------------------------- descriptor
+class _TwoFrameEdges:
+    """The edges a projection with fractional InputDelay reads delivers
+    through (delay.py:114-128): every edge whose delay/dt has a fraction
+    delivers from frame ``lo`` and from frame ``lo + 1``. With
+    ``comm='sparse'`` (BinaryArray, a binary event matmul) both frames carry
+    the whole weight, as two events; with ``comm='dense'`` and receptor scatter
+    (graded inputs) they carry ``w * (1 - frac)`` and ``w * frac``, each one
+    product in the network dtype, summed exactly as per-edge weights."""
+    plastic = False
+    device_edges = None
+
+    def __init__(self, r, tdtype):
+        pre = np.asarray(r.pre_idx, np.int32)
+        post = np.asarray(r.post_idx, np.int32)
+        E = pre.shape[0]
+        d = np.broadcast_to(np.asarray(r.delay, np.int64), (E,))
+        fr = np.broadcast_to(np.asarray(r.delay_frac, tdtype), (E,))
+        lerp = fr != 0
+        w = np.broadcast_to(np.asarray(r.weight, np.float64), (E,))
+        if r.graded:
+            wt = w.astype(tdtype)
+            one = tdtype(1.0)
+            w_lo = np.where(lerp, (wt * (one - fr)).astype(tdtype), wt).astype(np.float64)
+            w_hi = (wt * fr).astype(tdtype).astype(np.float64)[lerp]
+            self.weight = np.concatenate([w_lo, w_hi])
+        else:
+            self.weight = float(r.weight) if r.homogeneous else np.concatenate([w, w[lerp]])
+        self.pre_idx = np.concatenate([pre, pre[lerp]])
+        self.post_idx = np.concatenate([post, post[lerp]])
+        self.delay = np.concatenate([d, d[lerp] + 1]).astype(np.int32)
+        self.receptor = r.receptor
+        self.pre_pop, self.post_pop = r.pre_pop, r.post_pop
+
+    @property
+    def n_edges(self):
+        return int(self.pre_idx.shape[0])
+
+    @property
+    def homogeneous(self):
+        return np.ndim(self.weight) == 0
+
+
 class _Desc:
     """ctypes descriptor plus the NumPy arrays it points into (kept alive)."""
 
@@ -801,6 +891,7 @@ class _NetBuilder:
                     ch.t_max_step = int(min(tmax, np.iinfo(np.int64).max))
                     ch.seed = int(r.seed) & 0xFFFFFFFF
                     ch.delay_steps = int(r.delay)
+                    ch.delay_frac = float(r.delay_frac)
                     if len(r.post_idx) == pop.n and np.array_equal(r.post_idx, np.arange(pop.n)):
                         ch.gen_index = None
                     else:
@@ -809,6 +900,9 @@ class _NetBuilder:
                         ch.gen_index = D.arr(gi, C.c_int32)
                     table_off.append(-1)
                 else:
+                    r_id = id(r)
+                    if getattr(r, 'delay_frac', None) is not None:
+                        r = _TwoFrameEdges(r, tdtype)
                     pre_p = D.pop_index[id(r.pre_pop)]
                     E = r.n_edges
                     dev = None if (self.host_edges or r.plastic) else getattr(r, 'device_edges', None)
@@ -879,7 +973,7 @@ class _NetBuilder:
                         pj.stdp.mu_plus, pj.stdp.mu_minus, pj.stdp.Wmax = s.mu_plus, s.mu_minus, s.Wmax
                         pj.stdp.decay_plus = float(np.exp(-dt / s.tau_plus))
                         pj.stdp.decay_minus = float(np.exp(-dt / s.tau_minus))
-                    D.proj_index[id(r)] = len(projs)
+                    D.proj_index[r_id] = len(projs)
                     projs.append(pj)
                     D.keep.append(pj)
                 c += 1

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define BS_ABI_VERSION 6
+#define BS_ABI_VERSION 7
 
 /* ---- return codes ------------------------------------------------------ */
 #define BS_OK 0
@@ -123,6 +123,10 @@ typedef struct bs_channel {
   const int32_t* gen_index;/* [n_post_pop] generator element feeding each post,
                               -1 if none; NULL = identity (POISSON, CURRENT, AC) */
   double ac[4];            /* AC: amplitude, omega (rad/ms), phase (rad), offset  */
+  double delay_frac;       /* POISSON: fraction of delay/dt in the network dtype
+                              (InputDelay, delay.py:114-128): value =
+                              (c[step-1-d] * (1 - f) + c[step-2-d] * f) * weight
+                              in the network dtype; 0 = whole steps              */
 } bs_channel;
 
 /* STDP parameters (stdp_synapse.py:115-179): additive/multiplicative pair rule. */

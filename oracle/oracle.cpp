@@ -355,7 +355,8 @@ extern "C" int orc_create(const bs_net_desc* d, orc_net** out) {
   net->Ntot = g;
   int dmax = 0;
   for (int c = 0; c < d->n_channels; ++c)
-    if (d->channels[c].kind == BS_CH_POISSON) dmax = std::max(dmax, d->channels[c].delay_steps);
+    if (d->channels[c].kind == BS_CH_POISSON)
+      dmax = std::max(dmax, d->channels[c].delay_steps + (d->channels[c].delay_frac != 0.0 ? 1 : 0));
   for (int q = 0; q < d->n_projs; ++q) {
     const bs_projection& pj = d->projs[q];
     OrcProj P;
@@ -732,7 +733,19 @@ static int run_t(orc_net* net, int64_t n_steps, double* vm_out) {
         const long long src = n - 1 - ch.delay_steps;
         if (src < 0) continue;
         const std::vector<double>& x = net->ghist[c][(int)(src % Lh)];
-        for (int j = 0; j < npost; ++j) yin[c][j] = (T)x[j] * (T)ch.weight;
+        if (ch.delay_frac == 0.0) {
+          for (int j = 0; j < npost; ++j) yin[c][j] = (T)x[j] * (T)ch.weight;
+        } else {
+          // InputDelay (delay.py:114-128) with a fractional delay/dt: frames
+          // lo and lo + 1 mixed as x_lo * (1 - frac) + x_hi * frac, in the
+          // network dtype; frames before step 0 hold zeros
+          const T f = (T)ch.delay_frac;
+          const std::vector<double>* xh = src >= 1 ? &net->ghist[c][(int)((src - 1) % Lh)] : nullptr;
+          for (int j = 0; j < npost; ++j) {
+            const T hi = xh ? (T)(*xh)[j] : (T)0;
+            yin[c][j] = ((T)x[j] * ((T)1 - f) + hi * f) * (T)ch.weight;
+          }
+        }
       } else if (ch.kind == BS_CH_CURRENT) {
         // phase 1b: dc_generator current, amplitude * weight inside its window
         // [t_min_step, t_max_step), scattered onto its targets

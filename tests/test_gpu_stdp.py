@@ -25,9 +25,13 @@ def _plastic_net(model=iaf_psc_exp, n=400, delay=('uniform', 1.0, 3.0), dtype='f
     wE, wI = (3.0, 6.0) if cond else (40.0, -160.0)
     sim.connect(noise, e + i, weight=wE, delay=1.0, **rt)
     if isinstance(delay, tuple):
-        # axonal delays: one per pre neuron, whole steps in [lo, hi] ms
+        # axonal delays: one per pre neuron in [lo, hi] ms, among the values
+        # whose ratio to dt is whole in float64 (a plastic projection needs
+        # whole-step delays; 1.2 / 0.1 = 11.999999999999998 would lerp)
         rng = np.random.default_rng(11)
-        delay = np.round(rng.uniform(delay[1], delay[2], n), 1) * u.ms
+        lo, hi = int(round(delay[1] * 10)), int(round(delay[2] * 10))
+        whole = [k for k in range(lo, hi + 1) if (k / 10) / 0.1 == k]
+        delay = rng.choice(whole, n) / 10 * u.ms
     syn = stdp_synapse(weight=wE, tau_plus=20., tau_minus=20., lambda_=0.05,
                        alpha=1.1, mu_plus=mu[0], mu_minus=mu[1], Wmax=3 * wE)
     proj = sim.connect(e, e, rule=fixed_indegree(20), synapse=syn, delay=delay, seed=3, **rt)

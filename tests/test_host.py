@@ -12,7 +12,7 @@ from brainpy.state_amd import (Simulator, all_to_all, explicit_edges, fixed_inde
                                pairwise_bernoulli, poisson_generator, spike_recorder, units as u,
                                voltmeter, aeif_cond_alpha)
 from brainpy.state_amd import _lib as L
-from brainpy.state_amd.simulator import delay_to_steps, seq_sum_table
+from brainpy.state_amd.simulator import input_delay_steps, nest_delay_steps, seq_sum_table
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -99,11 +99,25 @@ def test_rules_misc():
 
 
 def test_delay_conversion():
-    assert delay_to_steps(1.5 * u.ms, 0.1) == 15
-    assert delay_to_steps(None, 0.1) == 0
-    assert np.array_equal(delay_to_steps(np.array([1.0, 2.0]) * u.ms, 0.1), [10, 20])
+    """Homogeneous / axonal delays are read as InputDelay reads them
+    (delay.py:114-128): frames floor(d/dt) and the next one, with the fraction
+    computed in the network dtype; per-edge delays take NEST's rounding
+    (_kernel/delays.py:50-108, whose docstring example is restated here)."""
+    assert input_delay_steps(1.5 * u.ms, 0.1) == (15, 0.0)
+    assert input_delay_steps(None, 0.1) == (0, 0.0)
+    lo, fr = input_delay_steps(np.array([1.0, 2.0]) * u.ms, 0.1)
+    assert np.array_equal(lo, [10, 20]) and np.all(fr == 0)
+    assert input_delay_steps(1.5 * u.ms, 1.0) == (1, 0.5)          # the reference's 1.5-step KAT
+    lo, fr = input_delay_steps(1.2 * u.ms, 0.1)                      # 11.999999999999998 in float64
+    assert lo == 11 and fr == 1.2 / 0.1 - 11.0 and 0.99 < fr < 1.0
+    lo32, fr32 = input_delay_steps(1.2 * u.ms, 0.1, np.float32)      # whole in float32
+    assert (lo32, fr32) == (12, 0.0) and isinstance(fr32, np.float32)
+    lo32, fr32 = input_delay_steps(1.3 * u.ms, 0.1, np.float32)      # 12.999999 in float32
+    assert lo32 == 12 and fr32 == np.float32(1.3) / np.float32(0.1) - np.float32(12) and fr32 > 0.99
     with pytest.raises(ValueError):
-        delay_to_steps(0.15 * u.ms, 0.1)
+        input_delay_steps(-0.1 * u.ms, 0.1)
+    assert nest_delay_steps(np.array([0.0, 0.04, 0.05, 0.15, 1.5]) * u.ms, 0.1).tolist() == [0, 0, 1, 2, 15]
+    assert nest_delay_steps(np.array([1.2, 0.7, 2.9]), 0.1).tolist() == [12, 7, 29]
 
 
 def test_seq_sum_table_is_sequential_accumulation():

@@ -353,3 +353,76 @@ def test_fixed_point_delivery_is_the_exact_sum_within_quantisation():
         assert abs(y[j] - exact) <= bound, (j, y[j], exact, bound)
         # and the quantisation bound is far below the magnitude of the sum
         assert bound <= 1e-12 * np.abs(ws).sum() + math.ulp(exact)
+
+
+def _passive_post(sim, n):
+    """iaf_psc_delta with decay exactly 1, E_L = 0 and no reachable
+    threshold: V_m is the running sum of the delivered inputs."""
+    return sim.create(iaf_psc_delta, n, E_L=0., V_th=1e300, V_reset=0., tau_m=1e30, t_ref=2.,
+                      V_initializer=Constant(0.))
+
+
+@pytest.mark.parametrize('comm', ['dense', 'sparse'])
+def test_input_delay_fractional_frames_kat(comm):
+    """delay_test.py:62-71: a delay of 1.5 steps reads 0.5 in[t-1] +
+    0.5 in[t-2]. A neuron that fires once at step 0 reaches a passive
+    target at steps 2 and 3 (one step of holder lag): with comm='dense'
+    (x @ W, graded) as w/2 and w/2; with comm='sparse' (BinaryArray, any
+    nonzero read is an event) as w and w. An axonal delay of 2.0 steps on the
+    second source arrives whole at step 3."""
+    from brainpy.state_amd import Simulator, spike_recorder, voltmeter
+    from brainpy.state_amd import units as u
+    sim = Simulator(dt=1.0 * u.ms, dtype='float64')
+    pre = sim.create(iaf_psc_delta, 2, E_L=0., V_th=20., V_reset=0., tau_m=10., t_ref=1e6,
+                     V_initializer=Constant(30.))
+    post = _passive_post(sim, 2)
+    sim.connect(pre[0], post[0], weight=3.0 * u.mV, delay=1.5 * u.ms, comm=comm)
+    sim.connect(pre, post[1], weight=np.array([3.0, 5.0]) * u.mV, delay=np.array([1.5, 2.0]) * u.ms,
+                comm=comm)
+    sim.connect(pre, sim.create(spike_recorder))
+    vm = sim.create(voltmeter)
+    sim.connect(vm, post)
+    desc = sim.build_desc()
+    trace, g, s = oracle.OracleNet(desc).run(6)
+    assert sorted(g.tolist()) == [0, 1] and np.all(s == 0)
+    if comm == 'dense':
+        np.testing.assert_array_equal(trace[:, 0], [0, 0, 1.5, 3.0, 3.0, 3.0])
+        np.testing.assert_array_equal(trace[:, 1], [0, 0, 1.5, 3.0 + 5.0, 8.0, 8.0])
+    else:
+        np.testing.assert_array_equal(trace[:, 0], [0, 0, 3.0, 6.0, 6.0, 6.0])
+        np.testing.assert_array_equal(trace[:, 1], [0, 0, 3.0, 6.0 + 5.0, 11.0, 11.0])
+
+
+def test_input_delay_fractional_generator_kat():
+    """A poisson_generator behind a 1.5-step delay (delay.py:114-128): the
+    target receives (c[n-2] * 0.5 + c[n-3] * 0.5) * w, c[m] the generator's
+    count of step m (zero outside its window), restated here from the
+    KAT-pinned sampler; and at a whole-step delay, c[n-2] * w."""
+    from brainpy.state_amd import Simulator, poisson_generator, voltmeter
+    from brainpy.state_amd import units as u
+    sim = Simulator(dt=1.0 * u.ms, dtype='float64')
+    post = _passive_post(sim, 6)
+    gen = sim.create(poisson_generator, rate=900. * u.Hz, start=3. * u.ms)
+    p_frac = sim.connect(gen, post[:3], weight=2.0 * u.mV, delay=1.5 * u.ms)
+    p_whole = sim.connect(gen, post[3:], weight=2.0 * u.mV, delay=1.0 * u.ms)
+    vm = sim.create(voltmeter)
+    sim.connect(vm, post)
+    desc = sim.build_desc()
+    T = 40
+    trace, _, _ = oracle.OracleNet(desc).run(T)
+    for k, (ch_i, frac) in enumerate([(0, 0.5), (1, 0.0)]):
+        ch = desc.c.channels[ch_i]
+        assert ch.delay_frac == frac and ch.delay_steps == 1
+
+        def c(m, j):
+            if m < 0 or not (ch.t_min_step < m <= ch.t_max_step):
+                return 0.0
+            return float(oracle.poisson(ch.seed, m, j, ch.lam))
+        for j in range(3):
+            v, ref = 0.0, []
+            for n in range(T):
+                y = (c(n - 2, j) * 0.5 + c(n - 3, j) * 0.5) * 2.0 if frac else c(n - 2, j) * 2.0
+                v = v + y
+                ref.append(v)
+            np.testing.assert_array_equal(trace[:, 3 * k + j], ref)
+        assert trace[-1, 3 * k:3 * k + 3].min() > 0
