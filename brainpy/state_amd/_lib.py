@@ -118,6 +118,8 @@ _SIGS = {
                                          _P, _P, _P, _P]),
     'bs_conn_bernoulli_count': (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_int32, _P,
                                           C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    'bs_conn_bernoulli_count_noalloc': (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_int32, _P,
+                                                  C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     'bs_conn_bernoulli_fill': (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_int32, _P, _P, _P,
                                          _P, _P]),
     'bs_debug_math': (C.c_int, [C.c_int32, C.c_int64, C.POINTER(C.c_double),

@@ -25,12 +25,17 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 #include "../../../include/brainspike_hip.h"
 #include "bs_math.h"
 
 int bs_internal_fail(int code, const std::string& msg);
+
+// Scratch of bs_conn_bernoulli_count_noalloc (16 MiB per GPU, see there).
+constexpr size_t kScratchBytes = 16u << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
 
@@ -127,6 +132,103 @@ uint64_t bern_threshold(double p) {
   return (uint64_t)ceil(p * 4294967296.0);
 }
 
+// ---- scratch of bs_conn_bernoulli_count_noalloc: a file-scope device array
+// (each GPU holds its own copy), addressed from the host only through the
+// pointer hipGetSymbolAddress returns for the current device. Layout for a
+// call: row counts int64 [n_pre + 1], chunk sums int64 [ceil((n_pre + 1) /
+// 1024)], post counts int32 [n_post], largest post count int32 [1], each part
+// 256-byte aligned.
+constexpr int kNoAllocMaxRows = 1 << 20;  // n_pre and n_post limits
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};
+
+size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+__global__ void k_fill_i32(int32_t* __restrict__ p, long long n, int32_t v) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+    p[k] = v;
+}
+__global__ void k_fill_i64(long long* __restrict__ p, long long n, long long v) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+    p[k] = v;
+}
+
+// Exclusive prefix sum over n int64 values, chunks of 1024 per block (four
+// per thread): out[k] = in[0] + ... + in[k - 1] within the chunk, and the
+// chunk's total into csum[chunk].
+__global__ __launch_bounds__(256) void k_scan_chunks(const long long* __restrict__ in, long long n,
+                                                     long long* __restrict__ out, long long* __restrict__ csum) {
+  __shared__ long long wsum[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long long base = (long long)blockIdx.x * 1024 + (long long)tid * 4;
+  long long v[4], t = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = base + k < n ? in[base + k] : 0;
+    t += v[k];
+  }
+  long long inc = t;  // inclusive scan of the thread totals over the wave
+  for (int o = 1; o < 64; o <<= 1) {
+    const long long y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  long long woff = 0;
+  for (int w = 0; w < wave; ++w) woff += wsum[w];
+  long long ex = woff + inc - t;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < n) out[base + k] = ex;
+    ex += v[k];
+  }
+  if (tid == 255) csum[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// In place exclusive prefix of the nc chunk totals: one block, 256 at a time
+// with a running carry.
+__global__ __launch_bounds__(256) void k_scan_csum(long long* __restrict__ csum, int nc) {
+  __shared__ long long wsum[4];
+  __shared__ long long carry;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < nc; c0 += 256) {
+    const long long t = c0 + tid < nc ? csum[c0 + tid] : 0;
+    long long inc = t;
+    for (int o = 1; o < 64; o <<= 1) {
+      const long long y = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += y;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    long long woff = carry;
+    for (int w = 0; w < wave; ++w) woff += wsum[w];
+    if (c0 + tid < nc) csum[c0 + tid] = woff + inc - t;
+    __syncthreads();
+    if (tid == 0) carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(long long* __restrict__ out, long long n,
+                                                  const long long* __restrict__ csum) {
+  const long long base = (long long)blockIdx.x * 1024 + (long long)threadIdx.x * 4;
+  const long long add = csum[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (base + k < n) out[base + k] += add;
+}
+
+// Largest of n non-negative int32 values into *out (which starts at 0).
+__global__ __launch_bounds__(256) void k_max_i32(const int32_t* __restrict__ x, long long n, int32_t* __restrict__ out) {
+  int m = 0;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+    m = max(m, x[k]);
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
 }  // namespace
 
 extern "C" {
@@ -208,6 +310,76 @@ int bs_conn_bernoulli_fill(int64_t seed, int32_t n_pre, int32_t n_post, double p
                      post);
   CCHK(hipGetLastError());
   CCHK(hipStreamSynchronize(0));
+  return BS_OK;
+}
+
+int bs_conn_bernoulli_count_noalloc(int64_t seed, int32_t n_pre, int32_t n_post, double p, int32_t exclude_self,
+                                    int64_t* row_ptr, int64_t* n_edges, int32_t* max_indegree) {
+  if (n_pre < 0 || n_post < 0 || !(p >= 0.0))
+    return bs_internal_fail(BS_EINVAL, "bs_conn_bernoulli_count_noalloc: bad sizes or p");
+  if (n_pre > kNoAllocMaxRows || n_post > kNoAllocMaxRows)
+    return bs_internal_fail(BS_EINVAL, "bs_conn_bernoulli_count_noalloc: n_pre and n_post must be at most " +
+                                           std::to_string(kNoAllocMaxRows) + " (the scratch holds " +
+                                           std::to_string(kScratchBytes >> 20) + " MiB)");
+  if (!row_ptr || !n_edges || !max_indegree)
+    return bs_internal_fail(BS_EINVAL, "bs_conn_bernoulli_count_noalloc: null argument");
+  *n_edges = 0;
+  *max_indegree = 0;
+  const long long rows = (long long)n_pre + 1;
+  const int nchunk = (int)((rows + 1023) / 1024);
+  const size_t off_csum = al256(sizeof(long long) * (size_t)rows);
+  const size_t off_col = off_csum + al256(sizeof(long long) * (size_t)nchunk);
+  const size_t off_max = off_col + al256(sizeof(int32_t) * (size_t)std::max(1, n_post));
+  if (off_max + 256 > kScratchBytes)  // cannot happen within the limits above
+    return bs_internal_fail(BS_EINVAL, "bs_conn_bernoulli_count_noalloc: scratch too small");
+  int dev = 0;
+  CCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return bs_internal_fail(BS_EINVAL, "bs_conn_bernoulli_count_noalloc: device index >= 64");
+  unsigned char* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {  // once per device, with that device current
+      void* ptr = nullptr;
+      CCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+      g_scratch_ptr[dev] = ptr;
+    }
+    base = (unsigned char*)g_scratch_ptr[dev];
+  }
+  long long* cnt = (long long*)base;
+  long long* csum = (long long*)(base + off_csum);
+  int32_t* col = (int32_t*)(base + off_col);
+  int32_t* colmax = (int32_t*)(base + off_max);
+  uint32_t k0, k1;
+  conn_key(seed, k0, k1);
+  const uint64_t thr = bern_threshold(p);
+  // the scratch keeps the last call's contents: clear what must start at zero
+  // (fill kernels: the row counts cnt[0 .. n_pre) are all written below)
+  const unsigned fb = (unsigned)std::min<long long>(1024, (std::max(1, n_post) + 255) / 256);
+  hipLaunchKernelGGL(k_fill_i32, dim3(fb), dim3(256), 0, 0, col, (long long)std::max(1, n_post), 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, 0, colmax, 1ll, 0);
+  hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, 0, cnt + n_pre, 1ll, 0ll);
+  CCHK(hipGetLastError());
+  if (n_pre > 0 && n_post > 0) {
+    hipLaunchKernelGGL(k_conn_bern_count, dim3((unsigned)n_pre), dim3(256), 0, 0, k0, k1, (int)n_post, thr,
+                       exclude_self ? 1 : 0, cnt, col);
+  } else if (n_pre > 0) {
+    const unsigned rb = (unsigned)std::min<long long>(1024, (n_pre + 255) / 256);
+    hipLaunchKernelGGL(k_fill_i64, dim3(rb), dim3(256), 0, 0, cnt, (long long)n_pre, 0ll);
+  }
+  hipLaunchKernelGGL(k_scan_chunks, dim3((unsigned)nchunk), dim3(256), 0, 0, (const long long*)cnt, rows,
+                     (long long*)row_ptr, csum);
+  hipLaunchKernelGGL(k_scan_csum, dim3(1), dim3(256), 0, 0, csum, nchunk);
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nchunk), dim3(256), 0, 0, (long long*)row_ptr, rows,
+                     (const long long*)csum);
+  if (n_post > 0)
+    hipLaunchKernelGGL(k_max_i32, dim3(fb), dim3(256), 0, 0, (const int32_t*)col, (long long)n_post, colmax);
+  CCHK(hipGetLastError());
+  long long total = 0;
+  int mx = 0;
+  CCHK(hipMemcpy(&total, row_ptr + n_pre, sizeof(long long), hipMemcpyDeviceToHost));
+  CCHK(hipMemcpy(&mx, colmax, sizeof(int), hipMemcpyDeviceToHost));
+  *n_edges = total;
+  *max_indegree = mx;
   return BS_OK;
 }
 

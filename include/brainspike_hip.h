@@ -254,6 +254,17 @@ int bs_conn_fixed_indegree(int64_t seed, int32_t n_pre, int32_t n_post, int32_t 
  * edges into one post) are host outputs. */
 int bs_conn_bernoulli_count(int64_t seed, int32_t n_pre, int32_t n_post, double p, int32_t exclude_self,
                             int64_t* row_ptr, int64_t* n_edges, int32_t* max_indegree);
+/* The count pass without any allocation (no counterpart in the reference:
+ * for callers that cannot allocate device memory, such as code under HIP
+ * graph capture or allocation-free serving loops). Same arguments and
+ * results as bs_conn_bernoulli_count, bit for bit; its partial results (row
+ * and post counts, chunk sums) live in a fixed 16 MiB device array of the
+ * library that every GPU holds until the process ends, so n_pre and n_post
+ * are limited to 2^20 each (BS_EINVAL beyond). Every call on one GPU shares
+ * that array: calls that overlap on one GPU race. The caller still provides
+ * row_ptr (device, n_pre + 1 entries). */
+int bs_conn_bernoulli_count_noalloc(int64_t seed, int32_t n_pre, int32_t n_post, double p, int32_t exclude_self,
+                                    int64_t* row_ptr, int64_t* n_edges, int32_t* max_indegree);
 /* Fill pass: the edges of the count pass with the same arguments, written
  * from row_ptr into pre / post (n_edges entries each). */
 int bs_conn_bernoulli_fill(int64_t seed, int32_t n_pre, int32_t n_post, double p, int32_t exclude_self,
