@@ -1061,6 +1061,8 @@ struct bs_net {
   int rec_par = 0;                             // record buffer the next launches write
   hipStream_t stream2 = nullptr;               // record sorts and their copies
   hipEvent_t ev_chunk[2] = {nullptr, nullptr};
+  hipEvent_t ev_join[2] = {nullptr, nullptr};  // bs_net_run_device: caller stream -> ours -> caller's
+  int* d_tap_col = nullptr;                    // device column of each voltmeter tap, caller order
   // pinned host memory of the run loop: per-buffer count, flags, chunk base
   // step and previous window length, then the sorted records and V_m rows
   unsigned long long* h_cnt = nullptr;
@@ -1149,7 +1151,7 @@ static void free_net(bs_net* net) {
   F(net->d_tiles); F(net->d_pops); F(net->d_chans); F(net->d_gproj); F(net->d_tables);
   F(net->d_state); F(net->d_ref); F(net->d_sb); F(net->d_sb_count);
   F(net->d_rec[0]); F(net->d_rec[1]); F(net->d_rec_sorted); F(net->d_sort_tmp);
-  F(net->d_rec_count); F(net->d_flags); F(net->d_vm_col); F(net->d_vm);
+  F(net->d_rec_count); F(net->d_flags); F(net->d_vm_col); F(net->d_vm); F(net->d_tap_col);
   for (void* p : net->owned) F(p);
   F(net->d_base); F(net->d_blob); F(net->d_ring_cnt); F(net->d_ring_fx); F(net->d_prevlen);
   F(net->d_stdp);
@@ -1159,6 +1161,7 @@ static void free_net(bs_net* net) {
   for (int k = 0; k < 2; ++k) {
     if (net->graph[k]) (void)hipGraphExecDestroy(net->graph[k]);
     if (net->ev_chunk[k]) (void)hipEventDestroy(net->ev_chunk[k]);
+    if (net->ev_join[k]) (void)hipEventDestroy(net->ev_join[k]);
   }
   auto FH = [](void* p) {
     if (p) (void)hipHostFree(p);
@@ -1481,6 +1484,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   HCHK(hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking));
   HCHK(hipStreamCreateWithFlags(&net->stream2, hipStreamNonBlocking));
   for (int k = 0; k < 2; ++k) HCHK(hipEventCreateWithFlags(&net->ev_chunk[k], hipEventDisableTiming));
+  for (int k = 0; k < 2; ++k) HCHK(hipEventCreateWithFlags(&net->ev_join[k], hipEventDisableTiming));
   HCHK(hipHostMalloc((void**)&net->h_cnt, 2 * sizeof(unsigned long long), hipHostMallocDefault));
   HCHK(hipHostMalloc((void**)&net->h_flags, 2 * sizeof(int), hipHostMallocDefault));
   HCHK(hipHostMalloc((void**)&net->h_base_p, 2 * sizeof(long long), hipHostMallocDefault));
@@ -2249,6 +2253,9 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     }
   }
   if (gid) HCHK(hipMemcpy(net->d_vm_col, vm_col.data(), 4 * gid, hipMemcpyHostToDevice));
+  CHK(dmalloc(&net->d_tap_col, std::max<size_t>(1, net->vm_tap_col.size())));
+  if (!net->vm_tap_col.empty())
+    HCHK(hipMemcpy(net->d_tap_col, net->vm_tap_col.data(), 4 * net->vm_tap_col.size(), hipMemcpyHostToDevice));
   HCHK(hipFuncSetAttribute((const void*)k_window<double>,
                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget));
   HCHK(hipFuncSetAttribute((const void*)k_window<float>,
@@ -2405,6 +2412,29 @@ struct ChunkJob {
   int par = 0, nwin = 0;
 };
 
+// The captured graph of one full chunk (nwin window launches, plus the STDP
+// kernels of plastic networks) writing record buffer par; its kernels read the
+// chunk's first step and the previous window's length from device memory.
+static int chunk_graph(bs_net* net, int par, int nwin) {
+  hipGraphExec_t& G = net->graph[par];
+  if (G) return BS_OK;
+  net->rec_par = par;
+  hipGraph_t g;
+  HIPCHK(hipStreamBeginCapture(net->stream, hipStreamCaptureModeThreadLocal));
+  for (int w = 0; w < nwin; ++w) {
+    if (!net->stdp.empty()) launch_stdp(net, 0, net->d_base, w);
+    if (net->tsize == 8)
+      launch_window<double>(net, 0, net->W, 0, net->W, net->d_base, w);
+    else
+      launch_window<float>(net, 0, net->W, 0, net->W, net->d_base, w);
+  }
+  HIPCHK(hipStreamEndCapture(net->stream, &g));
+  hipError_t e = hipGraphInstantiate(&G, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(BS_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  return BS_OK;
+}
+
 // Queues one chunk of windows writing record buffer j.par (a graph replay when
 // the chunk is full), then the copies of its record count, the flags and, with
 // pin_vm, its V_m rows into pinned memory, and marks its end with
@@ -2423,22 +2453,9 @@ static int enqueue_chunk(bs_net* net, const ChunkJob& j, bool pin_vm, std::vecto
   const bool full = j.chunk == net->vm_cap_steps;
   if (full && !net->timing) {
     // replay the captured chunk: K window launches reading the base step
+    int rc = chunk_graph(net, j.par, j.nwin);
+    if (rc != BS_OK) return rc;
     hipGraphExec_t& G = net->graph[j.par];
-    if (!G) {
-      hipGraph_t g;
-      HIPCHK(hipStreamBeginCapture(net->stream, hipStreamCaptureModeThreadLocal));
-      for (int w = 0; w < j.nwin; ++w) {
-        if (!net->stdp.empty()) launch_stdp(net, 0, net->d_base, w);
-        if (net->tsize == 8)
-          launch_window<double>(net, 0, net->W, 0, net->W, net->d_base, w);
-        else
-          launch_window<float>(net, 0, net->W, 0, net->W, net->d_base, w);
-      }
-      HIPCHK(hipStreamEndCapture(net->stream, &g));
-      hipError_t e = hipGraphInstantiate(&G, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (e != hipSuccess) return fail(BS_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    }
     // pinned slots of this buffer: rewritten two chunks later, after this
     // chunk's event has been waited on
     net->h_base_p[j.par] = j.c0;
@@ -2680,6 +2697,241 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
   (void)hipEventDestroy(ev_all0);
   (void)hipEventDestroy(ev_all1);
   return rc;
+}
+
+// ---- device-resident, stream-ordered run ---------------------------------
+__global__ void k_run_base(long long* base, long long b, int* prev, int p) {
+  base[0] = b;
+  prev[0] = p;
+}
+__global__ void k_fill_u32(uint32_t* __restrict__ p, long long n, uint32_t v) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+    p[k] = v;
+}
+__global__ void k_zero_u64(unsigned long long* p) { p[0] = 0ull; }
+// The chunk's spike records (key = step - chunk start << gid_bits | gid) as
+// raster bits of rows row0 + (step - chunk start).
+__global__ void k_rec_raster(const unsigned long long* __restrict__ rec, const unsigned long long* __restrict__ cnt,
+                             long long cap, int gid_bits, long long row0, long long words,
+                             uint32_t* __restrict__ raster) {
+  const long long n = (long long)min((unsigned long long)cap, *cnt);
+  const unsigned long long gmask = (1ull << gid_bits) - 1ull;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
+    const unsigned long long key = rec[k];
+    const long long g = (long long)(key & gmask), row = row0 + (long long)(key >> gid_bits);
+    atomicOr(&raster[row * words + (g >> 5)], 1u << (g & 31));
+  }
+}
+// V_m rows of the chunk in caller tap order.
+__global__ void k_vm_taps(const double* __restrict__ vbuf, int n_vm_dev, const int* __restrict__ tap_col, int ntaps,
+                          long long rows, long long row0, double* __restrict__ out) {
+  const long long n = rows * ntaps;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
+    const long long t = k / ntaps;
+    const int c = (int)(k - t * ntaps);
+    out[(row0 + t) * ntaps + c] = vbuf[t * n_vm_dev + tap_col[c]];
+  }
+}
+__global__ void k_run_status(const int* __restrict__ flags, int32_t* __restrict__ status) {
+  const int f = flags[0];
+  status[0] = (f & (FLAG_OVERFLOW | FLAG_COUNT)) ? BS_ECAPACITY : ((f & FLAG_UNSTABLE) ? BS_ENUMERIC : BS_OK);
+}
+
+int bs_net_run_device(bs_net* net, int64_t n_steps, void* stream, uint32_t* raster, double* vm, int32_t* status) {
+  if (!net) return fail(BS_EINVAL, "bs_net_run_device: null handle");
+  if (n_steps < 0) return fail(BS_EINVAL, "bs_net_run_device: n_steps must be >= 0");
+  if (n_steps > 0 && !raster) return fail(BS_EINVAL, "bs_net_run_device: null raster");
+  if (net->step + n_steps >= (1ll << 31)) return fail(BS_EINVAL, "bs_net_run_device: step counter overflow");
+  HIPCHK(hipSetDevice(net->device));
+  hipStream_t cs = (hipStream_t)stream;
+  hipStream_t st = net->stream;
+  HIPCHK(hipEventRecord(net->ev_join[0], cs));
+  HIPCHK(hipStreamWaitEvent(st, net->ev_join[0], 0));
+  const long long words = (net->Ntot + 31) / 32;
+  const int ntaps = (int)net->vm_tap_col.size();
+  auto grid = [](long long n) { return dim3((unsigned)std::max<long long>(1, std::min<long long>(1024, (n + 255) / 256))); };
+  if (n_steps > 0) {
+    hipLaunchKernelGGL(k_fill_u32, grid(n_steps * words), dim3(256), 0, st, raster, n_steps * words, 0u);
+    if (net->n_prod > 0) launch_pois_fill(net, (int)net->step);
+  }
+  const long long chunk_max = net->vm_cap_steps;
+  long long done = 0;
+  int par = 0;
+  while (done < n_steps) {
+    const long long c0 = net->step;
+    const long long chunk = std::min<long long>(chunk_max, n_steps - done);
+    const int nwin = (int)((chunk + net->W - 1) / net->W);
+    net->rec_par = par;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, net->d_rec_count + par);
+    if (chunk == chunk_max) {  // a full chunk: replay its graph from this base step
+      int rc = chunk_graph(net, par, nwin);
+      if (rc != BS_OK) return rc;
+      net->rec_par = par;
+      hipLaunchKernelGGL(k_run_base, dim3(1), dim3(1), 0, st, net->d_base, (long long)c0, net->d_prevlen,
+                         net->last_win);
+      HIPCHK(hipGraphLaunch(net->graph[par], st));
+      net->last_win = net->W;
+    } else {
+      for (int w = 0; w < nwin; ++w) {
+        const long long n0 = c0 + (long long)w * net->W;
+        const int Wc = (int)std::min<long long>(net->W, c0 + chunk - n0);
+        if (!net->stdp.empty()) launch_stdp(net, (int)n0, nullptr, 0);
+        if (net->tsize == 8)
+          launch_window<double>(net, (int)n0, Wc, c0, net->last_win);
+        else
+          launch_window<float>(net, (int)n0, Wc, c0, net->last_win);
+        net->last_win = Wc;
+      }
+    }
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_rec_raster, grid(net->rec_cap), dim3(256), 0, st, (const unsigned long long*)net->d_rec[par],
+                       (const unsigned long long*)(net->d_rec_count + par), net->rec_cap, net->gid_bits, done, words,
+                       raster);
+    if (vm && ntaps > 0)
+      hipLaunchKernelGGL(k_vm_taps, grid(chunk * ntaps), dim3(256), 0, st, (const double*)net->d_vm, net->n_vm_dev,
+                         (const int*)net->d_tap_col, ntaps, chunk, done, vm);
+    HIPCHK(hipGetLastError());
+    net->step += chunk;
+    done += chunk;
+    par ^= 1;
+  }
+  if (status) hipLaunchKernelGGL(k_run_status, dim3(1), dim3(1), 0, st, (const int*)net->d_flags, status);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(net->ev_join[1], st));
+  HIPCHK(hipStreamWaitEvent(cs, net->ev_join[1], 0));
+  net->n_out_blocks = 0;  // this run's spikes are in the caller's raster
+  net->n_out = 0;
+  return BS_OK;
+}
+
+// ---- whole-engine checkpoint -------------------------------------------------
+namespace {
+struct StateHdr {
+  char magic[8];
+  int abi, dtype, n_pops, Ls, Lr, NCW, NFX, W, n_stdp, last_win;
+  long long Ntot, step, payload;
+};
+struct Part {
+  void* p;
+  size_t bytes;
+};
+std::vector<Part> state_parts(bs_net* net) {
+  const size_t N = (size_t)net->Ntot;
+  std::vector<Part> v;
+  v.push_back({net->d_state, (size_t)BS_NSTATE * N * net->tsize});
+  v.push_back({net->d_ref, 4 * N});
+  v.push_back({net->d_sb, 4 * (size_t)net->Ls * N});
+  v.push_back({net->d_sb_count, 4 * (size_t)net->Ls * net->pops.size()});
+  v.push_back({net->d_ring_cnt, 4 * (size_t)net->Lr * net->NCW * N});
+  if (net->NFX > 0) v.push_back({net->d_ring_fx, 8 * (size_t)net->Lr * net->NFX * N});
+  if (net->d_spk_hist) v.push_back({net->d_spk_hist, (size_t)net->Ls * N});
+  for (size_t q = 0; q < net->stdp.size(); ++q) {
+    const StdpDev& S = net->stdp[q];
+    v.push_back({S.w, 8 * net->w0[q].size()});
+    v.push_back({S.U, 8 * (size_t)S.Lu * S.n_pre});
+    v.push_back({S.V, 8 * (size_t)S.n_post});
+    v.push_back({S.pacc, 8 * (size_t)S.n_post});
+  }
+  return v;
+}
+StateHdr state_hdr(bs_net* net) {
+  StateHdr h;
+  memset(&h, 0, sizeof(h));
+  memcpy(h.magic, "BSNETST", 8);
+  h.abi = BS_ABI_VERSION;
+  h.dtype = net->dtype;
+  h.n_pops = (int)net->pops.size();
+  h.Ls = net->Ls;
+  h.Lr = net->Lr;
+  h.NCW = net->NCW;
+  h.NFX = net->NFX;
+  h.W = net->W;
+  h.n_stdp = (int)net->stdp.size();
+  h.last_win = net->last_win;
+  h.Ntot = net->Ntot;
+  h.step = net->step;
+  long long pay = 0;
+  for (const Part& p : state_parts(net)) pay += (long long)p.bytes;
+  h.payload = pay;
+  return h;
+}
+}  // namespace
+
+int bs_net_state_bytes(bs_net* net, int64_t* bytes) {
+  if (!net || !bytes) return fail(BS_EINVAL, "bs_net_state_bytes: null argument");
+  *bytes = (int64_t)sizeof(StateHdr) + state_hdr(net).payload;
+  return BS_OK;
+}
+
+int bs_net_save_state(bs_net* net, void* buf, int64_t cap) {
+  if (!net || !buf) return fail(BS_EINVAL, "bs_net_save_state: null argument");
+  const StateHdr h = state_hdr(net);
+  if (cap < (int64_t)sizeof(StateHdr) + h.payload) return fail(BS_ECAPACITY, "bs_net_save_state: buffer too small");
+  HIPCHK(hipSetDevice(net->device));
+  HIPCHK(hipStreamSynchronize(net->stream));
+  unsigned char* o = (unsigned char*)buf;
+  memcpy(o, &h, sizeof(h));
+  o += sizeof(h);
+  for (const Part& p : state_parts(net)) {
+    if (p.bytes) HIPCHK(hipMemcpy(o, p.p, p.bytes, hipMemcpyDeviceToHost));
+    o += p.bytes;
+  }
+  return BS_OK;
+}
+
+int bs_net_load_state(bs_net* net, const void* buf, int64_t size) {
+  if (!net || !buf) return fail(BS_EINVAL, "bs_net_load_state: null argument");
+  if (size < (int64_t)sizeof(StateHdr)) return fail(BS_EINVAL, "bs_net_load_state: not a snapshot");
+  StateHdr h;
+  memcpy(&h, buf, sizeof(h));
+  StateHdr mine = state_hdr(net);
+  if (memcmp(h.magic, "BSNETST", 8) != 0 || h.abi != BS_ABI_VERSION)
+    return fail(BS_EINVAL, "bs_net_load_state: not a snapshot of this library version");
+  if (h.dtype != mine.dtype || h.n_pops != mine.n_pops || h.Ls != mine.Ls || h.Lr != mine.Lr || h.NCW != mine.NCW ||
+      h.NFX != mine.NFX || h.W != mine.W || h.n_stdp != mine.n_stdp || h.Ntot != mine.Ntot ||
+      h.payload != mine.payload || size != (int64_t)sizeof(StateHdr) + h.payload)
+    return fail(BS_EINVAL, "bs_net_load_state: the snapshot belongs to a different network");
+  if (h.step < 0 || h.last_win < 0 || h.last_win > net->W)
+    return fail(BS_EINVAL, "bs_net_load_state: corrupt snapshot header");
+  HIPCHK(hipSetDevice(net->device));
+  HIPCHK(hipStreamSynchronize(net->stream));
+  const unsigned char* o = (const unsigned char*)buf + sizeof(h);
+  for (const Part& p : state_parts(net)) {
+    if (p.bytes) HIPCHK(hipMemcpy(p.p, o, p.bytes, hipMemcpyHostToDevice));
+    o += p.bytes;
+  }
+  net->step = h.step;
+  net->last_win = h.last_win;
+  net->n_out_blocks = 0;
+  net->n_out = 0;
+  return BS_OK;
+}
+
+int bs_net_set_state(bs_net* net, int32_t pop, int32_t var, const double* in) {
+  if (!net || !in) return fail(BS_EINVAL, "bs_net_set_state: null argument");
+  if (pop < 0 || pop >= (int)net->pops.size()) return fail(BS_EINVAL, "bs_net_set_state: bad population");
+  if (var < -1 || var >= BS_NSTATE) return fail(BS_EINVAL, "bs_net_set_state: bad variable");
+  const DevPop& p = net->pops[pop];
+  HIPCHK(hipSetDevice(net->device));
+  HIPCHK(hipStreamSynchronize(net->stream));
+  if (p.n == 0) return BS_OK;
+  if (var == -1) {
+    std::vector<int> r(p.n);
+    for (int i = 0; i < p.n; ++i) {
+      if (!(in[i] >= 0.0 && in[i] < 2147483648.0)) return fail(BS_EINVAL, "bs_net_set_state: bad refractory count");
+      r[i] = (int)in[i];
+    }
+    HIPCHK(hipMemcpy(net->d_ref + p.gid0, r.data(), 4 * (size_t)p.n, hipMemcpyHostToDevice));
+    return BS_OK;
+  }
+  const size_t off = (size_t)var * net->Ntot + p.gid0;
+  if (net->tsize == 8) {
+    HIPCHK(hipMemcpy((double*)net->d_state + off, in, 8 * (size_t)p.n, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> f(in, in + p.n);
+    HIPCHK(hipMemcpy((float*)net->d_state + off, f.data(), 4 * (size_t)p.n, hipMemcpyHostToDevice));
+  }
+  return BS_OK;
 }
 
 int bs_net_window(bs_net* net, int32_t* W) {

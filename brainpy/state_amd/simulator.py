@@ -608,6 +608,31 @@ class Simulator:
             self.reset_rollout()
         return self._run_window(duration)
 
+    def save_state(self) -> bytes:
+        """Snapshot of the running rollout: every piece of engine state the
+        next step depends on (neurons, refractory counters, in-flight spikes
+        and arrival rings, plastic traces and weights) and the step counter,
+        which positions every random stream. ``load_state`` then ``cont``
+        continues from it bit for bit. (The reference keeps this state inside
+        the Simulator between ``cont`` calls, simulator.py:1822-1839; here it
+        can also be stored, restored and moved to another Simulator built
+        the same way.)"""
+        if not self._rollout_ready:
+            self.reset_rollout()
+        blob = self._engine.save_state()
+        return int(self._base_step).to_bytes(8, 'little', signed=True) + blob
+
+    def load_state(self, blob: bytes):
+        """Restore a snapshot of ``save_state`` (of a network built the same
+        way; another network's snapshot raises EngineError)."""
+        if self._edited_mid_rollout or self._engine is None:
+            self._engine = None
+            self._edited_mid_rollout = False
+        eng = self._ensure_engine()
+        eng.load_state(blob[8:])
+        self._base_step = int.from_bytes(blob[:8], 'little', signed=True)
+        self._rollout_ready = True
+
     def n_steps_of(self, duration) -> int:
         T = u.scalar(duration, 'time')
         return int(math.ceil(T / self.dt))   # len(arange(0, T, dt))
@@ -1057,6 +1082,40 @@ class _Engine:
         a, b, n = C.c_double(), C.c_double(), C.c_int64()
         L.check(self.lib.bs_net_last_bytes(self.h, C.byref(a), C.byref(b), C.byref(n)))
         return dict(deliver_bytes=a.value, update_bytes=b.value, n_events=n.value)
+
+    def set_state(self, pop_index, var, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        L.check(self.lib.bs_net_set_state(self.h, pop_index, var, v.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def save_state(self) -> bytes:
+        """Snapshot of the whole engine state (bs_net_save_state)."""
+        n = C.c_int64()
+        L.check(self.lib.bs_net_state_bytes(self.h, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        L.check(self.lib.bs_net_save_state(self.h, buf, n.value))
+        return buf.raw
+
+    def load_state(self, blob: bytes):
+        buf = C.create_string_buffer(bytes(blob), len(blob))
+        L.check(self.lib.bs_net_load_state(self.h, buf, len(blob)))
+
+    def run_device(self, n_steps, stream=None):
+        """Queue ``n_steps`` on a torch/HIP stream (the current one by
+        default) with device outputs: ``(raster, vm, status)`` torch tensors,
+        raster uint32 bits [n_steps][ceil(N / 32)] of every neuron, vm f64
+        [n_steps][taps], status int32 [1] (0 = OK). Nothing is synchronised."""
+        import torch
+        st = torch.cuda.current_stream() if stream is None else stream
+        words = (int(self.desc.ntot) + 31) // 32
+        with torch.cuda.stream(st):
+            raster = torch.empty((n_steps, words), dtype=torch.int32, device='cuda')
+            vm = torch.empty((n_steps, max(self.desc.n_taps, 0)), dtype=torch.float64, device='cuda')
+            status = torch.empty(1, dtype=torch.int32, device='cuda')
+        vm_ptr = C.c_void_p(vm.data_ptr()) if self.desc.n_taps > 0 and n_steps > 0 else None
+        L.check(self.lib.bs_net_run_device(self.h, int(n_steps), C.c_void_p(st.cuda_stream),
+                                           C.c_void_p(raster.data_ptr()) if n_steps > 0 else None, vm_ptr,
+                                           C.c_void_p(status.data_ptr())))
+        return raster, vm, status
 
     def state(self, pop_index, var, n):
         out = np.zeros(max(1, n))

@@ -197,8 +197,46 @@ int bs_net_step(bs_net* net, int64_t* step);
 /* Spikes recorded by the last bs_net_run, sorted by (step, gid). */
 int bs_net_spike_count(bs_net* net, int64_t* n);
 int bs_net_copy_spikes(bs_net* net, int32_t* gid, int32_t* step, int64_t cap);
-/* Neuron state after the last step: var in [0, BS_NSTATE), values as f64. */
+/* Neuron state after the last step: var in [0, BS_NSTATE) (values as f64) or
+ * -1 (refractory step counter). Replaces reading a model's state
+ * (iaf_psc_delta.V etc., _base.py) and SimulationResult.trace. */
 int bs_net_get_state(bs_net* net, int32_t pop, int32_t var, double* out);
+/* Overwrite one state variable of a population (var as in bs_net_get_state;
+ * values rounded to the network dtype, the counter to int32): the reference's
+ * state assignment before a run (model.V.value = ..., Simulator.cont). */
+int bs_net_set_state(bs_net* net, int32_t pop, int32_t var, const double* in);
+
+/* ---- whole-engine checkpoint / resume ------------------------------------
+ * Replaces the carried state of Simulator.cont (simulator.py:1822-1839) and a
+ * BPTT restart from (state0, n0) (bptt.py:536-571): a snapshot of everything
+ * the next step depends on -- neuron state and refractory counters, the
+ * spike buckets of the delay horizon, the arrival rings, spike history and
+ * plastic traces, weights and deposits, and the step counter (the position
+ * of every counter-based random stream). Restoring it into a handle created
+ * from the same descriptor continues the run bit for bit. The snapshot is an
+ * opaque host byte string of bs_net_state_bytes() bytes; a snapshot of a
+ * different network is rejected (BS_EINVAL). */
+int bs_net_state_bytes(bs_net* net, int64_t* bytes);
+int bs_net_save_state(bs_net* net, void* buf, int64_t cap);
+int bs_net_load_state(bs_net* net, const void* buf, int64_t size);
+
+/* ---- device-resident, stream-ordered run -----------------------------------
+ * What an XLA FFI handler (or torch op) needs from Simulator._run_window
+ * (simulator.py:1841-1883): the run is queued on the caller's HIP stream
+ * `stream` (a hipStream_t; NULL = the null stream) and writes only device
+ * buffers the caller owns; the call returns once the work is queued.
+ *   raster  uint32 [n_steps][ceil(Ntot / 32)] (device): bit g % 32 of word
+ *           g / 32 of row t is set iff neuron gid g fired at step t of the
+ *           run, every population included (the reference's (T, n) 0/1
+ *           raster, packed); the call clears it first
+ *   vm      f64 [n_steps][n_taps] (device) or NULL: the voltmeter taps in
+ *           caller order (bs_net_run's vm_out, on the device)
+ *   status  int32 [1] (device) or NULL: 0, or the bs_net_run error class of
+ *           a device-side failure (BS_ECAPACITY, BS_ENUMERIC)
+ * The handle's own stream joins the caller's stream before and after the
+ * run (events), so calls on one handle stay ordered with each other. */
+int bs_net_run_device(bs_net* net, int64_t n_steps, void* stream, uint32_t* raster, double* vm,
+                      int32_t* status);
 /* Plastic weights of projection proj in the reference's CSR (sorted-by-pre,
  * stable) edge order (event_plastic.py:293-296). */
 int bs_net_get_weights(bs_net* net, int32_t proj, double* out);
