@@ -1063,6 +1063,18 @@ struct bs_net {
   hipEvent_t ev_chunk[2] = {nullptr, nullptr};
   hipEvent_t ev_join[2] = {nullptr, nullptr};  // bs_net_run_device: caller stream -> ours -> caller's
   int* d_tap_col = nullptr;                    // device column of each voltmeter tap, caller order
+  // record_weight taps: plastic projections whose weights are copied after
+  // every step into a device trajectory, then into host memory per chunk
+  struct WTap {
+    int q = 0;                                // stdp index
+    long long E = 0;
+    long long* d_ref = nullptr;               // reference (CSR) position -> stored position
+    double* d_traj = nullptr;                 // [wtap_rows][E]
+    std::vector<double> host;                 // [n_steps][E] of the last run
+  };
+  std::vector<WTap> wtaps;
+  long long wtap_rows = 0;                    // device trajectory rows (steps per chunk)
+  long long wtap_steps = 0;                   // steps in the host trajectories
   // pinned host memory of the run loop: per-buffer count, flags, chunk base
   // step and previous window length, then the sorted records and V_m rows
   unsigned long long* h_cnt = nullptr;
@@ -1153,6 +1165,10 @@ static void free_net(bs_net* net) {
   F(net->d_rec[0]); F(net->d_rec[1]); F(net->d_rec_sorted); F(net->d_sort_tmp);
   F(net->d_rec_count); F(net->d_flags); F(net->d_vm_col); F(net->d_vm); F(net->d_tap_col);
   for (void* p : net->owned) F(p);
+  for (auto& t : net->wtaps) {
+    F(t.d_ref);
+    F(t.d_traj);
+  }
   F(net->d_base); F(net->d_blob); F(net->d_ring_cnt); F(net->d_ring_fx); F(net->d_prevlen);
   F(net->d_stdp);
   F(net->d_spk_hist);
@@ -2400,6 +2416,13 @@ static void launch_stdp(bs_net* net, int n, const long long* base_ptr, int widx)
   hipLaunchKernelGGL(k_stdp_events, dim3(512), dim3(256), 0, net->stream, a);
 }
 
+// One step's weights of a plastic projection in the reference's CSR order.
+__global__ void k_weight_tap(const double* __restrict__ w, const long long* __restrict__ ref, long long E,
+                             double* __restrict__ row) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < E; k += (long long)gridDim.x * blockDim.x)
+    row[k] = w[ref[k]];
+}
+
 int bs_net_set_timing(bs_net* net, int on) {
   if (!net) return fail(BS_EINVAL, "bs_net_set_timing: null handle");
   net->timing = on != 0;
@@ -2451,7 +2474,7 @@ static int enqueue_chunk(bs_net* net, const ChunkJob& j, bool pin_vm, std::vecto
     }
   }
   const bool full = j.chunk == net->vm_cap_steps;
-  if (full && !net->timing) {
+  if (full && !net->timing && net->wtaps.empty()) {
     // replay the captured chunk: K window launches reading the base step
     int rc = chunk_graph(net, j.par, j.nwin);
     if (rc != BS_OK) return rc;
@@ -2479,6 +2502,11 @@ static int enqueue_chunk(bs_net* net, const ChunkJob& j, bool pin_vm, std::vecto
       else
         launch_window<float>(net, (int)n0, Wc, j.c0, net->last_win);
       net->last_win = Wc;
+      for (const bs_net::WTap& t : net->wtaps)  // record_weight: plastic networks step one at a time
+        if (t.E > 0)
+          hipLaunchKernelGGL(k_weight_tap, dim3((unsigned)std::min<long long>(1024, (t.E + 255) / 256)), dim3(256), 0,
+                             net->stream, (const double*)net->stdp[t.q].w, (const long long*)t.d_ref, t.E,
+                             t.d_traj + (size_t)w * t.E);
       if (net->timing) HIPCHK(hipEventRecord(evs[2 * w + 1], net->stream));
       HIPCHK(hipGetLastError());
       net->n_update++;
@@ -2589,6 +2617,10 @@ static int drain_chunk(bs_net* net, const ChunkJob& j, bool pin_vm, double* vm_o
   // gather: each spike's bucket entry and segment bounds, 4 B per edge; 8 B record
   net->b_deliver += 8.0 * (double)cnt + 4.0 * (double)ev;
   net->b_update += 8.0 * (double)cnt;
+  for (bs_net::WTap& t : net->wtaps)
+    if (t.E > 0 && j.chunk > 0)
+      HIPCHK(hipMemcpy(t.host.data() + (size_t)j.done * t.E, t.d_traj, sizeof(double) * (size_t)j.chunk * t.E,
+                       hipMemcpyDeviceToHost));
   const int ntaps = (int)net->vm_tap_col.size();
   if (vm_out && ntaps > 0) {
     const double* hv;
@@ -2619,14 +2651,18 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
   net->b_deliver = net->b_update = 0;
   net->n_events = 0;
   const int ntaps = (int)net->vm_tap_col.size();
-  const long long chunk_max = net->vm_cap_steps;  // = K * W
+  // = K * W; record_weight taps bound it by their device trajectory rows
+  const long long chunk_max = net->wtaps.empty() ? net->vm_cap_steps : std::min(net->vm_cap_steps, net->wtap_rows);
+  for (bs_net::WTap& t : net->wtaps) t.host.assign((size_t)n_steps * t.E, 0.0);
+  net->wtap_steps = n_steps;
   // Pipelined drain: chunk c + 1 is queued before chunk c is drained, so the
   // device never waits on the host's sort, copy and decode. Timed runs keep
   // one chunk in flight (their per-window events are read at each drain), and
   // so do runs whose V_m rows would need more than 256 MiB of pinned memory.
   const size_t vm_bytes = sizeof(double) * (size_t)chunk_max * std::max(1, net->n_vm_dev);
   const bool want_vm = vm_out && ntaps > 0;
-  const bool pipelined = !net->timing && n_steps > chunk_max && (!want_vm || vm_bytes <= (256u << 20));
+  const bool pipelined = !net->timing && net->wtaps.empty() && n_steps > chunk_max &&
+                         (!want_vm || vm_bytes <= (256u << 20));
   const bool pin_vm = pipelined && want_vm;
   if (pin_vm && net->h_vm_cap < 2 * vm_bytes) {
     if (net->h_vm_p) HIPCHK(hipHostFree(net->h_vm_p));
@@ -2931,6 +2967,46 @@ int bs_net_set_state(bs_net* net, int32_t pop, int32_t var, const double* in) {
     std::vector<float> f(in, in + p.n);
     HIPCHK(hipMemcpy((float*)net->d_state + off, f.data(), 4 * (size_t)p.n, hipMemcpyHostToDevice));
   }
+  return BS_OK;
+}
+
+int bs_net_set_weight_taps(bs_net* net, int32_t n, const int32_t* projs) {
+  if (!net || n < 0 || (n > 0 && !projs)) return fail(BS_EINVAL, "bs_net_set_weight_taps: bad arguments");
+  for (int k = 0; k < n; ++k)
+    if (projs[k] < 0 || projs[k] >= (int)net->proj_stdp.size() || net->proj_stdp[projs[k]] < 0)
+      return fail(BS_EINVAL, "bs_net_set_weight_taps: projection is not plastic");
+  HIPCHK(hipSetDevice(net->device));
+  HIPCHK(hipStreamSynchronize(net->stream));
+  for (auto& t : net->wtaps) {
+    (void)hipFree(t.d_ref);
+    (void)hipFree(t.d_traj);
+  }
+  net->wtaps.clear();
+  net->wtap_steps = 0;
+  long long Esum = 0;
+  for (int k = 0; k < n; ++k) Esum += (long long)net->w0[net->proj_stdp[projs[k]]].size();
+  // trajectory rows per chunk: 256 MiB of device memory over all taps
+  net->wtap_rows = std::max<long long>(1, std::min<long long>(net->vm_cap_steps, (32ll << 20) / std::max(1ll, Esum)));
+  for (int k = 0; k < n; ++k) {
+    bs_net::WTap t;
+    t.q = net->proj_stdp[projs[k]];
+    t.E = (long long)net->w0[t.q].size();
+    net->wtaps.push_back(t);
+    bs_net::WTap& b = net->wtaps.back();
+    int rc = dmalloc(&b.d_ref, (size_t)std::max(1ll, b.E));
+    if (rc) return rc;
+    rc = dmalloc(&b.d_traj, (size_t)std::max(1ll, b.E * net->wtap_rows));
+    if (rc) return rc;
+    if (b.E) HIPCHK(hipMemcpy(b.d_ref, net->w_ref[b.q].data(), 8 * (size_t)b.E, hipMemcpyHostToDevice));
+  }
+  return BS_OK;
+}
+
+int bs_net_copy_weight_trace(bs_net* net, int32_t k, double* out) {
+  if (!net || !out) return fail(BS_EINVAL, "bs_net_copy_weight_trace: null argument");
+  if (k < 0 || k >= (int)net->wtaps.size()) return fail(BS_EINVAL, "bs_net_copy_weight_trace: no such tap");
+  const std::vector<double>& h = net->wtaps[k].host;
+  if (!h.empty()) memcpy(out, h.data(), sizeof(double) * h.size());
   return BS_OK;
 }
 

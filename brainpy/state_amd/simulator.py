@@ -305,6 +305,7 @@ class Simulator:
             raise ValueError(f"connectivity must be 'host' or 'device', got {connectivity!r}")
         self.connectivity = connectivity
         self._pops = []              # neuron populations, creation order
+        self._devices = []           # recorders and meters, kept alive: their ids key the taps
         self._taps = {}              # id(spike_recorder) -> (pop, idx)
         self._analog_taps = {}       # id(multimeter) -> (pop, idx, names)
         self._weight_taps = {}
@@ -330,6 +331,9 @@ class Simulator:
             return NodeView([_GenSegment(_GeneratorSpec(model_cls, p))])
         mod = model_cls(size, **p)
         if isinstance(mod, (spike_recorder, multimeter)):
+            # the taps are keyed by id(device): the Simulator holds every device
+            # so that no id is reused by a later one while the taps exist
+            self._devices.append(mod)
             return NodeView([Selection(mod, np.arange(1))])
         if not isinstance(mod, Neuron):
             raise TypeError(f'{model_cls!r} is not a supported neuron model or device')
@@ -664,24 +668,14 @@ class Simulator:
         T = u.scalar(duration, 'time')
         n = self.n_steps_of(duration)
         weights = {}
-        if self._weight_taps:
-            # weight taps read the post-update weights every step (simulator.py:1868)
-            taps = list(self._weight_taps.items())
-            traj = {rid: np.zeros((n, p.n_edges)) for rid, p in taps}
-            vms, gl, sl = [], [], []
-            for t in range(n):
-                vm1, (g1, s1) = eng.run(1)
-                vms.append(vm1)
-                gl.append(g1)
-                sl.append(s1)
-                for rid, p in taps:
-                    traj[rid][t] = eng.weights(eng.desc.proj_index[id(p)], p.n_edges)
-            vm = np.concatenate(vms) if vms else np.zeros((0, eng.desc.n_taps))
-            gids = np.concatenate(gl) if gl else np.zeros(0, np.int32)
-            steps = np.concatenate(sl) if sl else np.zeros(0, np.int32)
-            weights = {rid: u.Quantity(traj[rid], u.pA) for rid in traj}
-        else:
-            vm, (gids, steps) = eng.run(n)
+        taps = list(self._weight_taps.items())
+        # weight taps read the post-update weights every step (simulator.py:1868),
+        # copied on the device after each step and brought back once per chunk
+        eng.set_weight_taps([eng.desc.proj_index[id(p)] for _, p in taps])
+        vm, (gids, steps) = eng.run(n)
+        for k, (rid, p) in enumerate(taps):
+            unit = {'voltage': u.mV, 'current': u.pA, 'conductance': u.nS}[_weight_dim(p.post_pop)]
+            weights[rid] = u.Quantity(eng.weight_trace(k, n, p.n_edges), unit)
         res = self._result(eng.desc, n, T, vm, gids, steps)
         res._weights = weights
         return res
@@ -1082,6 +1076,16 @@ class _Engine:
         a, b, n = C.c_double(), C.c_double(), C.c_int64()
         L.check(self.lib.bs_net_last_bytes(self.h, C.byref(a), C.byref(b), C.byref(n)))
         return dict(deliver_bytes=a.value, update_bytes=b.value, n_events=n.value)
+
+    def set_weight_taps(self, proj_indices):
+        arr = np.ascontiguousarray(proj_indices, dtype=np.int32)
+        L.check(self.lib.bs_net_set_weight_taps(self.h, len(arr), arr.ctypes.data_as(C.POINTER(C.c_int32))))
+
+    def weight_trace(self, k, n_steps, n_edges):
+        out = np.zeros((n_steps, n_edges), dtype=np.float64)
+        if out.size:
+            L.check(self.lib.bs_net_copy_weight_trace(self.h, k, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
 
     def set_state(self, pop_index, var, values):
         v = np.ascontiguousarray(values, dtype=np.float64)

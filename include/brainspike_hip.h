@@ -240,6 +240,13 @@ int bs_net_run_device(bs_net* net, int64_t n_steps, void* stream, uint32_t* rast
 /* Plastic weights of projection proj in the reference's CSR (sorted-by-pre,
  * stable) edge order (event_plastic.py:293-296). */
 int bs_net_get_weights(bs_net* net, int32_t proj, double* out);
+/* record_weight taps (simulator.py:884-910): every following bs_net_run
+ * copies the weights of the listed plastic projections after every step (in
+ * the order of bs_net_get_weights) on the device, into host memory once per
+ * chunk; n = 0 removes the taps. bs_net_copy_weight_trace(k) returns tap k's
+ * trajectory of the last run, [n_steps][n_edges] f64. */
+int bs_net_set_weight_taps(bs_net* net, int32_t n, const int32_t* projs);
+int bs_net_copy_weight_trace(bs_net* net, int32_t k, double* out);
 /* Per-kernel HIP-event timing of the following runs (off by default: the
  * events add launch overhead). */
 int bs_net_set_timing(bs_net* net, int on);

@@ -93,3 +93,29 @@ def test_stdp_chunks_and_continue():
     assert np.array_equal(g_b, g_o) and np.array_equal(s_b, s_o)
     np.testing.assert_allclose(sim.get_weights(proj),
                                orc.weights(desc.proj_index[id(proj)], proj.n_edges), rtol=1e-12)
+
+
+@pytest.mark.parametrize('model,unit', [(iaf_psc_exp, 'pA'), (aeif_cond_alpha, 'nS')])
+def test_record_weight_trajectory_equals_the_oracle_step_by_step(model, unit):
+    """record_weight (simulator.py:884-910): the (n_steps, n_edges) weight
+    trajectory of a plastic projection, post-update every step, in the CSR
+    edge order, in the weight unit of the target model; every row equals the
+    oracle's weights after that step."""
+    sim, proj = _plastic_net(model=model, n=200, delay=1.0)
+    sim.record_weight(proj)
+    n = 400
+    res = sim.simulate(n * 0.1 * u.ms)
+    tr = res.weight_trace(proj)
+    assert tr.unit == getattr(u, unit)
+    w = np.asarray(tr.value)
+    assert w.shape == (n, proj.n_edges)
+    np.testing.assert_array_equal(w[-1], sim.get_weights(proj))
+    desc = sim.build_desc()
+    orc = oracle.OracleNet(desc)
+    k = desc.proj_index[id(proj)]
+    rows = []
+    for _ in range(n):
+        orc.run(1)
+        rows.append(orc.weights(k, proj.n_edges))
+    np.testing.assert_allclose(w, np.array(rows), rtol=1e-12, atol=0)
+    assert np.any(w[-1] != w[0]), 'no plasticity happened'

@@ -238,3 +238,24 @@ def test_nodeview_algebra_and_indexing():
         v[8]
     with pytest.raises(TypeError):
         va + 1
+
+
+def test_recorders_created_inline_keep_distinct_taps():
+    """Recorders created and dropped inside a loop (the usual
+    ``sim.connect(pop, sim.create(spike_recorder))``) each keep their tap:
+    the Simulator holds the devices, so a later device never reuses an
+    earlier one's id (which keys the tap)."""
+    import gc
+    sim = Simulator(dt=0.1 * u.ms)
+    pops = [sim.create(iaf_psc_delta, 3 + k) for k in range(6)]
+    for p in pops:
+        sim.connect(p, sim.create(spike_recorder))
+        gc.collect()
+    assert len(sim._taps) == 6
+    vm = [sim.create(voltmeter) for _ in range(2)]
+    sim.connect(vm[0], pops[0])
+    sim.connect(sim.create(voltmeter), pops[1])
+    gc.collect()
+    sim.connect(sim.create(voltmeter), pops[2])
+    assert len(sim._analog_taps) == 3
+    assert sim.build_desc().n_taps == 3 + 4 + 5
