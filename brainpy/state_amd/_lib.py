@@ -15,12 +15,12 @@ __all__ = ['lib', 'EngineUnavailable', 'EngineError', 'check', 'BS_NPARAM', 'BS_
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libbrainspike_hip.so')
 
-BS_ABI_VERSION = 7
+BS_ABI_VERSION = 8
 BS_NPARAM = 40
 BS_NSTATE = 9
 BS_F64, BS_F32 = 0, 1
 BS_IAF_PSC_DELTA, BS_IAF_PSC_EXP, BS_IAF_PSC_ALPHA, BS_AEIF_COND_ALPHA = 1, 2, 3, 4
-BS_CH_COUNT, BS_CH_FIXED, BS_CH_POISSON, BS_CH_CURRENT, BS_CH_AC = 1, 2, 3, 4, 5
+BS_CH_COUNT, BS_CH_FIXED, BS_CH_POISSON, BS_CH_CURRENT, BS_CH_AC, BS_CH_SCHEDULE = 1, 2, 3, 4, 5, 6
 
 _ERRORS = {-1: 'invalid argument', -2: 'out of memory', -3: 'HIP error',
            -4: 'capacity exceeded', -5: 'numerical instability', -6: 'no device'}
@@ -51,7 +51,8 @@ class Channel(C.Structure):
                 ('max_count', C.c_int32), ('fixed_shift', C.c_int32), ('lam', C.c_double),
                 ('t_min_step', C.c_int64), ('t_max_step', C.c_int64), ('seed', C.c_uint32),
                 ('delay_steps', C.c_int32), ('gen_index', C.POINTER(C.c_int32)),
-                ('ac', C.c_double * 4), ('delay_frac', C.c_double)]
+                ('ac', C.c_double * 4), ('sched', C.POINTER(C.c_int64)), ('sched_val', C.POINTER(C.c_double)),
+                ('sched_len', C.c_int32), ('_pad_sched', C.c_int32), ('delay_frac', C.c_double)]
 
 
 class Stdp(C.Structure):

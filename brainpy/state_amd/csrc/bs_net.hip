@@ -115,6 +115,8 @@ struct DevChan {
   const unsigned short* guide;   // POISSON: 64-entry guide table
   long long* pacc;               // FIXED fed by a plastic projection: deposits per post
   double frac;                   // POISSON: InputDelay lerp weight of frame delay + 1 (0: whole steps)
+  const long long* sched;        // SCHEDULE: [2 * cdf_len] plateau start, stop steps
+  const double* sval;            // SCHEDULE: [cdf_len] plateau values
 };
 
 // One block's tile: everything its prologue needs before the population's
@@ -272,6 +274,8 @@ struct ChReg {
   const int* gen_index;
   long long* pacc;              // plastic deposits (FIXED channel of a plastic projection)
   double frac;                  // POISSON: lerp weight of the frame one step further back
+  const long long* sched;       // SCHEDULE: plateau steps and values (global memory)
+  const double* sval;
 };
 
 template <typename T>
@@ -296,6 +300,8 @@ __device__ __forceinline__ ChReg<T> load_ch(const DevChan& ch, const unsigned ch
   c.gen_index = ch.gen_index;
   c.pacc = ch.pacc;
   c.frac = ch.frac;
+  c.sched = ch.sched;
+  c.sval = ch.sval;
   const bool staged = ch.lds_off >= 0;
   c.tab = staged ? (const T*)(tab + ch.lds_off) : tables + ch.table_off;
   c.tabg = tables + ch.table_off;
@@ -323,7 +329,19 @@ struct UpdCtx {
   int nrec, Wm;
   const unsigned short* pc;  // [Wm][TS] drawn-ahead Poisson counts of this tile
   int ncur;                  // 1: row nrec of inp holds the summed current input
+  const double* schv;        // [Wm] the current SCHEDULE channel's value per window step
 };
+
+// Value of a step_current_generator schedule at step n: the plateau with the
+// last start <= n, if n is before its stop; 0 otherwise (binary search).
+__device__ __forceinline__ double sched_at(const long long* se, const double* sv, int len, long long n) {
+  int lo = 0, hi = len;  // first plateau with start > n
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (se[2 * mid] <= n) lo = mid + 1; else hi = mid;
+  }
+  return (lo > 0 && n < se[2 * (lo - 1) + 1]) ? sv[lo - 1] : 0.0;
+}
 
 // Input of one channel at window step w for lane tid (neuron j).
 template <typename T>
@@ -359,6 +377,11 @@ __device__ __forceinline__ T chan_value(const UpdCtx<T>& u, const ChReg<T>& c, i
     const int j = u.start + lane;
     const int e = c.gen_index ? c.gen_index[j] : j;
     return (e >= 0 && (long long)step >= c.tmin && (long long)step < c.tmax) ? c.weight : (T)0;
+  }
+  if (c.kind == BS_CH_SCHEDULE) {  // step_current_generator: staged per window step
+    const int j = u.start + lane;
+    const int e = c.gen_index ? c.gen_index[j] : j;
+    return e >= 0 ? (T)u.schv[w] : (T)0;
   }
   if (c.kind == BS_CH_AC) {  // ac_generator: (A sin(omega t + phi) + offset) * weight
     const int j = u.start + lane;
@@ -775,8 +798,9 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   // the summation order.
   {
     const T* tables = (const T*)a.tables;
+    __shared__ double schv[32];  // a SCHEDULE channel's value per window step (Wmax <= 32)
     UpdCtx<T> u{a, cnt, fxa, spl, spn, chs, tab, tables, TS, ncw, nfx, nch, tl.start, tid,
-                tid < tl.count, (long long)gpop.gid0 + tl.start + tid, inp, nrec, Wm, pc, tl.ncur};
+                tid < tl.count, (long long)gpop.gid0 + tl.start + tid, inp, nrec, Wm, pc, tl.ncur, schv};
     constexpr int kIT = 4;
     const int nit = W * TS;
     // current inputs are summed apart from the delta inputs, starting from
@@ -796,13 +820,18 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
       }
       for (int c = 0; c < nch; ++c) {
         const ChReg<T> cc = load_ch<T>(chs[c], tab, tables);
+        if (cc.kind == BS_CH_SCHEDULE) {  // block-uniform: one lookup per window step
+          __syncthreads();
+          if (tid < W) schv[tid] = sched_at(cc.sched, cc.sval, cc.cdf_len, (long long)a.n0 + tid);
+          __syncthreads();
+        }
         T v[kIT];
 #pragma unroll
         for (int i = 0; i < kIT; ++i) {
           const int idx = base0 + tid + 256 * i, w = idx >> lgTS, l = idx & (TS - 1);
           v[i] = ok[i] ? chan_value<T>(u, cc, w, a.n0 + w, l) : (T)0;
         }
-        if (cc.kind == BS_CH_CURRENT || cc.kind == BS_CH_AC) {
+        if (cc.kind == BS_CH_CURRENT || cc.kind == BS_CH_AC || cc.kind == BS_CH_SCHEDULE) {
 #pragma unroll
           for (int i = 0; i < kIT; ++i) accC[i] = accC[i] + v[i];
         } else if (cc.receptor == 0) {
@@ -831,7 +860,7 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   if (tid < TS) {
     const T* tables = (const T*)a.tables;
     UpdCtx<T> u{a, cnt, fxa, spl, spn, chs, tab, tables, TS, ncw, nfx, nch, tl.start, tid,
-                tid < tl.count, (long long)gpop.gid0 + tl.start + tid, inp, nrec, Wm, pc, tl.ncur};
+                tid < tl.count, (long long)gpop.gid0 + tl.start + tid, inp, nrec, Wm, pc, tl.ncur, nullptr};
     // each model with and without current inputs (a compile-time branch)
 #define BS_UPDATE(CUR)                                                                              \
     switch (gpop.model) {                                                                          \
@@ -1624,6 +1653,29 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
         dc.weight = ch.weight;
         dc.tmin = ch.t_min_step;
         dc.tmax = ch.t_max_step;
+        if (ch.gen_index) {
+          int* g = nullptr;
+          CHK(upload(net, &g, ch.gen_index, (size_t)sp.n));
+          dc.gen_index = g;
+        }
+        ++dp.n_cur;
+      } else if (ch.kind == BS_CH_SCHEDULE) {
+        if (ch.receptor != 0) BAD("current channels feed the current-input seam (receptor 0)");
+        if (ch.sched_len < 1 || !ch.sched || !ch.sched_val) BAD("schedule channel needs sched_len >= 1 plateaus");
+        for (int k = 0; k < ch.sched_len; ++k) {
+          if (!(ch.sched[2 * k] < ch.sched[2 * k + 1])) BAD("schedule plateaus must have start < stop");
+          if (k > 0 && ch.sched[2 * k] < ch.sched[2 * k - 1]) BAD("schedule plateaus must be sorted and disjoint");
+          if (!(ch.sched_val[k] == ch.sched_val[k])) BAD("schedule values must not be NaN");
+        }
+        long long* dse = nullptr;
+        double* dsv = nullptr;
+        CHK(upload(net, &dse, (const long long*)ch.sched, 2 * (size_t)ch.sched_len));
+        CHK(upload(net, &dsv, ch.sched_val, (size_t)ch.sched_len));
+        dc.sched = dse;
+        dc.sval = dsv;
+        dc.cdf_len = ch.sched_len;
+        dc.tmin = ch.sched[0];
+        dc.tmax = ch.sched[2 * ch.sched_len - 1];
         if (ch.gen_index) {
           int* g = nullptr;
           CHK(upload(net, &g, ch.gen_index, (size_t)sp.n));

@@ -18,7 +18,7 @@ import numpy as np
 
 from . import units as u
 from . import _lib as L
-from .devices import dc_generator, multimeter, poisson_generator, spike_recorder
+from .devices import dc_generator, multimeter, poisson_generator, spike_recorder, step_current_generator
 from .dist import Distribution
 from .models import Neuron, aeif_cond_alpha, iaf_psc_delta
 from .nodeview import NodeView, Selection, population_size
@@ -828,8 +828,13 @@ class _NetBuilder:
             q = D.pop_index[id(r.post_pop)]
             if q in self.remote:              # a remote population's inputs are its own shard's
                 continue
-            if isinstance(r, _CurChannel):    # one current channel per plateau, in schedule order
-                per_pop[q].extend((r, piece) for piece in r.gen.plateaus(dt))
+            if isinstance(r, _CurChannel):
+                pieces = r.gen.plateaus(dt)
+                if isinstance(r.gen, step_current_generator):
+                    if pieces:                # the whole schedule in one channel
+                        per_pop[q].append((r, ('sched', pieces)))
+                else:                         # dc / ac: one current channel
+                    per_pop[q].extend((r, piece) for piece in pieces)
             else:
                 per_pop[q].append(r)
         chans = (L.Channel * max(1, sum(len(x) for x in per_pop)))()
@@ -880,7 +885,27 @@ class _NetBuilder:
                 if isinstance(r, tuple):
                     r, piece = r
                 ch.receptor = r.receptor
-                if piece is not None:
+                if piece is not None and piece[0] == 'sched':
+                    # step_current_generator: its plateaus as one schedule,
+                    # each value device.update() * weight (simulator.py:1738-1740)
+                    pcs = piece[1]
+                    ch.kind = L.BS_CH_SCHEDULE
+                    ch.weight = 1.0
+                    se = np.array([[n0, min(n1, np.iinfo(np.int64).max)] for _, n0, n1 in pcs], dtype=np.int64)
+                    sv = np.array([a if r.weight is None else a * r.weight for a, _, _ in pcs], dtype=np.float64)
+                    ch.sched = D.arr(se.reshape(-1), C.c_int64)
+                    ch.sched_val = D.arr(sv, C.c_double)
+                    ch.sched_len = len(pcs)
+                    ch.t_min_step = int(se[0, 0])
+                    ch.t_max_step = int(se[-1, 1])
+                    if len(r.post_idx) == pop.n and np.array_equal(r.post_idx, np.arange(pop.n)):
+                        ch.gen_index = None
+                    else:
+                        gi = np.full(pop.n, -1, dtype=np.int32)
+                        gi[r.post_idx] = np.arange(len(r.post_idx), dtype=np.int32)
+                        ch.gen_index = D.arr(gi, C.c_int32)
+                    table_off.append(-1)
+                elif piece is not None:
                     amp, n0, n1 = piece
                     if amp == 'ac':   # the sinusoid, evaluated on the device
                         ch.kind = L.BS_CH_AC

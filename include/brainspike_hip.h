@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define BS_ABI_VERSION 7
+#define BS_ABI_VERSION 8
 
 /* ---- return codes ------------------------------------------------------ */
 #define BS_OK 0
@@ -71,7 +71,7 @@ enum bs_channel_kind {
                         inputs. iaf_psc_delta adds the sum to I_e in the same
                         step; iaf_psc_exp / iaf_psc_alpha / aeif_cond_alpha
                         latch it for the next step (i_0 / y0 / I_stim)          */
-  BS_CH_AC = 5       /* ac_generator current (ac_generator.py:264-325), a
+  BS_CH_AC = 5,      /* ac_generator current (ac_generator.py:264-325), a
                         current channel like BS_CH_CURRENT: value =
                         (ac[0] * sin(ac[1] * t + ac[2]) + ac[3]) * weight with
                         t = step * dt (ms), ac = {amplitude (pA), omega (rad/ms)
@@ -80,6 +80,12 @@ enum bs_channel_kind {
                         sine is the engine's own (Cody-Waite reduction by
                         pi/2 and Taylor series), within a few ulp of the
                         exact sine for |argument| < 1.6e6 rad                   */
+  BS_CH_SCHEDULE = 6 /* step_current_generator (step_current_generator.py:269-335),
+                        a current channel like BS_CH_CURRENT for a whole
+                        schedule: sched_len disjoint plateaus sorted by start,
+                        value = sched_val[k] at steps sched[2k] <= step <
+                        sched[2k + 1], else 0 (a binary search per window on
+                        the device, any schedule length)                      */
 };
 
 /* Parameter block lengths (doubles) per model; layouts are documented in
@@ -123,6 +129,11 @@ typedef struct bs_channel {
   const int32_t* gen_index;/* [n_post_pop] generator element feeding each post,
                               -1 if none; NULL = identity (POISSON, CURRENT, AC) */
   double ac[4];            /* AC: amplitude, omega (rad/ms), phase (rad), offset  */
+  const int64_t* sched;    /* SCHEDULE: [2 * sched_len] plateau start, stop steps */
+  const double* sched_val; /* SCHEDULE: [sched_len] plateau values (amplitude times
+                              the connection weight)                             */
+  int32_t sched_len;
+  int32_t _pad_sched;
   double delay_frac;       /* POISSON: fraction of delay/dt in the network dtype
                               (InputDelay, delay.py:114-128): value =
                               (c[step-1-d] * (1 - f) + c[step-2-d] * f) * weight

@@ -225,6 +225,8 @@ struct orc_net {
   std::vector<bs_channel> chans;
   std::vector<std::vector<int>> gen_index;  // per channel (POISSON)
   std::vector<PoissonTable> ptab;           // per channel (POISSON)
+  std::vector<std::vector<int64_t>> sched;  // per channel (SCHEDULE): plateau [start, stop) steps
+  std::vector<std::vector<double>> sched_val;
   std::vector<OrcProj> projs;
   std::vector<int> chan_proj;               // channel -> projection (COUNT/FIXED)
   std::vector<long long> gid0;
@@ -331,6 +333,8 @@ extern "C" int orc_create(const bs_net_desc* d, orc_net** out) {
   net->chan_proj.assign(d->n_channels, -1);
   net->gen_index.resize(d->n_channels);
   net->ptab.resize(d->n_channels);
+  net->sched.resize(d->n_channels);
+  net->sched_val.resize(d->n_channels);
   long long g = 0;
   for (int p = 0; p < d->n_pops; ++p) {
     const bs_population& sp = d->pops[p];
@@ -344,7 +348,11 @@ extern "C" int orc_create(const bs_net_desc* d, orc_net** out) {
     for (int k = 0; k < sp.n_vm; ++k) net->taps.emplace_back(p, sp.vm_idx[k]);
     for (int c = sp.ch_begin; c < sp.ch_begin + sp.ch_count; ++c) {
       const bs_channel& ch = d->channels[c];
-      if (ch.kind == BS_CH_POISSON || ch.kind == BS_CH_CURRENT || ch.kind == BS_CH_AC) {
+      if (ch.kind == BS_CH_SCHEDULE) {
+        net->sched[c].assign(ch.sched, ch.sched + 2 * (size_t)std::max(0, ch.sched_len));
+        net->sched_val[c].assign(ch.sched_val, ch.sched_val + std::max(0, ch.sched_len));
+      }
+      if (ch.kind == BS_CH_POISSON || ch.kind == BS_CH_CURRENT || ch.kind == BS_CH_AC || ch.kind == BS_CH_SCHEDULE) {
         std::vector<int> gi(sp.n);
         for (int i = 0; i < sp.n; ++i) gi[i] = ch.gen_index ? ch.gen_index[i] : i;
         net->gen_index[c] = gi;
@@ -414,7 +422,8 @@ extern "C" int orc_create(const bs_net_desc* d, orc_net** out) {
       net->chan_pop[c] = p;
       if (d->channels[c].kind == BS_CH_POISSON)
         net->ghist[c].assign(net->Lh, std::vector<double>(sp.n, 0.0));
-      else if (d->channels[c].kind == BS_CH_CURRENT || d->channels[c].kind == BS_CH_AC)
+      else if (d->channels[c].kind == BS_CH_CURRENT || d->channels[c].kind == BS_CH_AC ||
+               d->channels[c].kind == BS_CH_SCHEDULE)
         continue;  // no arrival ring: the current enters in its own step
       else if (d->channels[c].kind == BS_CH_COUNT)
         net->cring[c].assign(net->Lh, std::vector<double>(sp.n, 0.0));
@@ -753,6 +762,15 @@ static int run_t(orc_net* net, int64_t n_steps, double* vm_out) {
         const bool active = ch.t_min_step <= n && n < ch.t_max_step;
         const std::vector<int>& gi = net->gen_index[c];
         for (int j = 0; j < npost; ++j) yin[c][j] = (active && gi[j] >= 0) ? (T)ch.weight : (T)0;
+      } else if (ch.kind == BS_CH_SCHEDULE) {
+        // step_current_generator (step_current_generator.py:269-335): the
+        // plateau holding step n, by a plain scan of the schedule
+        double v = 0.0;
+        const std::vector<int64_t>& se = net->sched[c];
+        for (size_t k = 0; k < net->sched_val[c].size(); ++k)
+          if (se[2 * k] <= n && n < se[2 * k + 1]) v = net->sched_val[c][k];
+        const std::vector<int>& gi = net->gen_index[c];
+        for (int j = 0; j < npost; ++j) yin[c][j] = gi[j] >= 0 ? (T)v : (T)0;
       } else if (ch.kind == BS_CH_AC) {
         // phase 1b: ac_generator current amplitude * sin(omega t + phi) + offset
         // at t = n dt, times the connection weight, inside [t_min_step,
@@ -829,7 +847,8 @@ static int run_t(orc_net* net, int64_t n_steps, double* vm_out) {
         // sum_current_inputs(init): I_e + x for iaf_psc_delta, x for the others (x = 0)
         T cur = sp.model == BS_IAF_PSC_DELTA ? (T)P[34] + (T)0 : (T)0;
         for (int c = sp.ch_begin; c < sp.ch_begin + sp.ch_count; ++c) {
-          if (net->chans[c].kind == BS_CH_CURRENT || net->chans[c].kind == BS_CH_AC)
+          if (net->chans[c].kind == BS_CH_CURRENT || net->chans[c].kind == BS_CH_AC ||
+              net->chans[c].kind == BS_CH_SCHEDULE)
             cur = cur + yin[c][j];
           else
             acc[net->chans[c].receptor] = acc[net->chans[c].receptor] + yin[c][j];

@@ -104,3 +104,30 @@ def test_step_and_ac_currents_float32_against_float32_oracle():
     assert len(g_o) > 20
     assert np.array_equal(g_g, g_o) and np.array_equal(s_g, s_o)
     np.testing.assert_allclose(vm_g, vm_o, rtol=1e-6, atol=1e-4)
+
+
+@pytest.mark.parametrize('model', [iaf_psc_delta, iaf_psc_exp, aeif_cond_alpha])
+def test_long_schedules_bit_exact(model):
+    """step_current_generator schedules of 2000 changes (one channel each, any
+    length: the device looks the plateau up once per window step), on a whole
+    population and on a weighted subset."""
+    from brainpy.state_amd import NodeView
+    sim = _net(model)
+    pops = [NodeView.of(p) for p in sim._pops]
+    rng = np.random.default_rng(9)
+    t = np.round(np.arange(2000) * 0.1 + 0.3, 1)
+    sim.connect(sim.create(step_current_generator, amplitude_times=t,
+                           amplitude_values=rng.uniform(-100.0, 400.0, 2000)), pops[0])
+    sim.connect(sim.create(step_current_generator, amplitude_times=t[::2] + 0.05,
+                           amplitude_values=rng.uniform(0.0, 600.0, 1000), start=3.0, stop=150.0),
+                pops[1], weight=0.7)
+    desc = sim.build_desc()
+    assert sum(desc.c.channels[k].kind == 6 for k in range(desc.c.n_channels)) == 2
+    eng = sim._ensure_engine()
+    eng.reset()
+    n = 2500
+    vm_g, (g_g, s_g) = eng.run(n)
+    vm_o, g_o, s_o = oracle.OracleNet(desc).run(n)
+    assert len(g_o) > 20, 'network too quiet for a useful check'
+    assert np.array_equal(g_g, g_o) and np.array_equal(s_g, s_o)
+    np.testing.assert_allclose(vm_g, vm_o, rtol=1e-12, atol=0)

@@ -134,3 +134,32 @@ def test_schedule_equals_its_dc_windows():
                **NEST_DELTA)
     assert np.array_equal(a, b)
     assert np.any(a[50:400] != -70.0)
+
+
+def test_long_schedule_closed_form():
+    """A schedule of 1500 changes (one every 0.2 ms, values of both signs)
+    into a NEST-default iaf_psc_delta: the oracle's one schedule channel
+    against the exact recurrence with the reference's look-up (the plateau of
+    the last amplitude_times[k] <= t, step_current_generator.py:269-335)."""
+    n_steps, dt = 3200, 0.1
+    rng = np.random.default_rng(4)
+    amp_times = np.round(np.arange(1500) * 0.2 + 1.0, 1)
+    amp_values = rng.uniform(-150.0, 450.0, 1500)
+    gen = (step_current_generator, dict(amplitude_times=amp_times * u.ms, amplitude_values=amp_values * u.pA))
+    v = _trace(iaf_psc_delta, n_steps, [gen], **NEST_DELTA)
+    P33 = math.exp(-dt / 10.0)
+    P30 = (10.0 / 250.0) * (1.0 - P33)
+    V, ref, out = -70.0, 0, np.empty(n_steps)
+    for step in range(n_steps):
+        t = step * dt
+        k = int(np.searchsorted(amp_times, t, side='right')) - 1
+        I = amp_values[k] if k >= 0 else 0.0
+        if ref == 0:
+            V = -70.0 + (V + 70.0) * P33 + I * P30
+        else:
+            ref -= 1
+        if V >= -55.0 and ref == 0:
+            ref, V = math.ceil(2.0 / dt), -70.0
+        out[step] = V
+    np.testing.assert_allclose(v, out, atol=1e-9, rtol=0)
+    assert np.ptp(v) > 5.0
