@@ -42,6 +42,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -1104,6 +1105,9 @@ struct bs_net {
   std::vector<WTap> wtaps;
   long long wtap_rows = 0;                    // device trajectory rows (steps per chunk)
   long long wtap_steps = 0;                   // steps in the host trajectories
+  // ac_generator channels: {omega, phi, dt, t_min_step, t_max_step}; a run
+  // whose sine arguments would leave the sine's exact range is refused
+  std::vector<std::array<double, 5>> ac_args;
   // pinned host memory of the run loop: per-buffer count, flags, chunk base
   // step and previous window length, then the sorted records and V_m rows
   unsigned long long* h_cnt = nullptr;
@@ -1693,6 +1697,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
         dc.weight = ch.weight;
         dc.tmin = ch.t_min_step;
         dc.tmax = ch.t_max_step;
+        net->ac_args.push_back({ch.ac[1], ch.ac[2], d->dt_ms, (double)ch.t_min_step, (double)ch.t_max_step});
         if (ch.gen_index) {
           int* g = nullptr;
           CHK(upload(net, &g, ch.gen_index, (size_t)sp.n));
@@ -2691,10 +2696,30 @@ static int drain_chunk(bs_net* net, const ChunkJob& j, bool pin_vm, double* vm_o
   return BS_OK;
 }
 
+// The engine's sine (sin_p) reduces its argument by pi/2 in three parts whose
+// products with the quadrant count are exact only while |argument| < 2^20 *
+// pi/2 (about 1.6e6 rad); a run whose ac_generator arguments omega * n * dt +
+// phi could leave that range is refused rather than evaluated inexactly.
+static int check_ac_range(bs_net* net, long long n_end) {
+  for (const auto& a : net->ac_args) {
+    const double last = std::min((double)n_end, a[4]);
+    if (last <= a[3]) continue;  // never active in this run
+    const double arg = std::fabs(a[0]) * last * a[2] + std::fabs(a[1]);
+    if (!(arg < 1.6e6))
+      return fail(BS_EINVAL, "ac_generator: omega * t + phi reaches " + std::to_string(arg) +
+                                 " rad in this run, beyond the 1.6e6 rad the engine's sine reduces exactly");
+  }
+  return BS_OK;
+}
+
 int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
   if (!net) return fail(BS_EINVAL, "bs_net_run: null handle");
   if (n_steps < 0) return fail(BS_EINVAL, "bs_net_run: n_steps must be >= 0");
   if (net->step + n_steps >= (1ll << 31)) return fail(BS_EINVAL, "bs_net_run: step counter overflow");
+  {
+    int rc = check_ac_range(net, net->step + n_steps);
+    if (rc != BS_OK) return rc;
+  }
   HIPCHK(hipSetDevice(net->device));
   net->n_out_blocks = 0;
   net->n_out = 0;
@@ -2830,6 +2855,10 @@ int bs_net_run_device(bs_net* net, int64_t n_steps, void* stream, uint32_t* rast
   if (n_steps < 0) return fail(BS_EINVAL, "bs_net_run_device: n_steps must be >= 0");
   if (n_steps > 0 && !raster) return fail(BS_EINVAL, "bs_net_run_device: null raster");
   if (net->step + n_steps >= (1ll << 31)) return fail(BS_EINVAL, "bs_net_run_device: step counter overflow");
+  {
+    int rc = check_ac_range(net, net->step + n_steps);
+    if (rc != BS_OK) return rc;
+  }
   HIPCHK(hipSetDevice(net->device));
   hipStream_t cs = (hipStream_t)stream;
   hipStream_t st = net->stream;

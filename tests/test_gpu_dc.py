@@ -131,3 +131,19 @@ def test_long_schedules_bit_exact(model):
     assert len(g_o) > 20, 'network too quiet for a useful check'
     assert np.array_equal(g_g, g_o) and np.array_equal(s_g, s_o)
     np.testing.assert_allclose(vm_g, vm_o, rtol=1e-12, atol=0)
+
+
+def test_ac_run_beyond_the_exact_sine_range_is_refused():
+    """The engine's sine is exact for |argument| < 1.6e6 rad; a run whose
+    ac_generator argument would pass that is refused before any launch."""
+    from brainpy.state_amd._lib import EngineError
+    sim = Simulator(dt=0.1 * u.ms)
+    p = sim.create(iaf_psc_delta, 4)
+    sim.connect(sim.create(ac_generator, amplitude=10.0, frequency=1000.0), p)
+    eng = sim._ensure_engine()
+    eng.reset()
+    eng.run(100)                          # 2 pi rad/ms * 10 ms: fine
+    with pytest.raises(EngineError, match='1.6e6'):
+        eng.run(3_000_000)                # 300 s at 2 pi rad/ms: 1.9e6 rad
+    eng.close()
+    sim._engine = None
