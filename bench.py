@@ -74,6 +74,9 @@ def main():
     ap.add_argument('--ensemble', type=int, default=0, metavar='M',
                     help='instead: the batched-ensemble workload with M networks (256 = full size)')
     ap.add_argument('--ensemble-steps', type=int, default=5000)
+    ap.add_argument('--ens-tile', type=int, default=0,
+                    help='ensemble launch shape without autotuning (with --ens-window)')
+    ap.add_argument('--ens-window', type=int, default=0)
     ap.add_argument('--connectivity', default='device', choices=['host', 'device'],
                     help='where the ensemble samples its random edges (the networks differ; with '
                          'device edges an engine builds in well under a second, and the autotuner '
@@ -290,7 +293,7 @@ def run_cpu_leg(args):
             'loadavg': [round(la0[0], 2), round(la1[0], 2)]}
 
 
-def pmc_traffic(kernel, n_last):
+def pmc_traffic(kernel, n_last, prof_args=None):
     """HBM bytes per launch of ``kernel`` from two rocprofv3 counter passes
     (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) over a profile-only run of
     the same network (settle, one warm-up window, one timed window), each
@@ -307,8 +310,8 @@ def pmc_traffic(kernel, n_last):
         return None, 'rocprofv3 not found'
     tmp = tempfile.mkdtemp(prefix='bench_pmc_', dir='/tmp')
     env = dict(os.environ, TMPDIR='/tmp')
-    prof = [os.path.join(ROOT, 'bench.py'), '--steps', '1', '--warmup', '1', '--profile-only',
-            '--no-cpu', '--bptt-steps', '0', '--no-pmc']
+    prof = [os.path.join(ROOT, 'bench.py')] + (prof_args or [
+        '--steps', '1', '--warmup', '1', '--profile-only', '--no-cpu', '--bptt-steps', '0', '--no-pmc'])
     copy = ['-c', "import torch; x = torch.ones(1 << 28, dtype=torch.uint8, device='cuda'); "
                   "y = torch.empty_like(x); [y.copy_(x) for _ in range(4)]; torch.cuda.synchronize()"]
 
@@ -413,14 +416,24 @@ def run_ensemble(args):
         f'built in {time.time() - t0:.1f}s')
     rng = np.random.default_rng(1)
     sample = sorted(rng.choice(M, size=min(16, M), replace=False).tolist())
-    tune_on = sims if args.connectivity == 'device' else [sims[k] for k in sample]
-    best, table = Ensemble.autotune(tune_on, steps=300)
-    log(f'autotuned on {len(tune_on)} networks')
-    log('autotune (tile_size, max_window) -> ms/step: ' +
-        ', '.join(f'{k}: {v * 1e3:.3f}' for k, v in sorted(table.items())))
+    if args.ens_tile:
+        best, table, tune_on = (args.ens_tile, args.ens_window), {}, []
+    else:
+        tune_on = sims if args.connectivity == 'device' else [sims[k] for k in sample]
+        best, table = Ensemble.autotune(tune_on, steps=300)
+        log(f'autotuned on {len(tune_on)} networks')
+        log('autotune (tile_size, max_window) -> ms/step: ' +
+            ', '.join(f'{k}: {v * 1e3:.3f}' for k, v in sorted(table.items())))
     t1 = time.time()
     ens = Ensemble(sims, tile_size=best[0], max_window=best[1])
     log(f'engine created in {time.time() - t1:.1f}s (tile {best[0]}, window bound {best[1]})')
+    if args.profile_only:
+        ens.reset()
+        ens.run_raw(200)
+        _, (g_p, _) = ens.run_raw(steps)
+        log(f'profile run: {steps} steps, {len(g_p)} spikes')
+        ens.close()
+        return {'profile_only': True}
     ens.reset()
     ens.run_raw(200)
     t2 = time.perf_counter()
@@ -430,7 +443,27 @@ def run_ensemble(args):
     parts = ens.split(vm, g_all, st_all)
     n_sp = len(g_all)
     counts_ens = {k: len(parts[k][1]) for k in sample}
+    # window kernel: HIP events around each launch over 320 steady-state steps
+    ens._engine.set_timing(True)
+    ens.run_raw(320)
+    tm, by2 = ens._engine.timing(), ens.engine_bytes()
+    ens._engine.set_timing(False)
+    n_launch = max(1, tm['n_update'])
+    kms = tm['update_ms'] / n_launch
+    del_b, upd_b = by2['deliver_bytes'] / n_launch, by2['update_bytes'] / n_launch
+    achieved = (del_b + upd_b) / (kms * 1e-3) / 1e9
     ens.close()
+    traffic, tnote = (None, 'skipped (--no-pmc)') if args.no_pmc else pmc_traffic(
+        'k_window', n_launch, ['--ensemble', str(M), '--ensemble-steps', '320', '--profile-only',
+                               '--ens-tile', str(best[0]), '--ens-window', str(best[1]),
+                               '--connectivity', args.connectivity, '--no-pmc'])
+    roofline = {'bound': 'hbm', 'kernel': 'k_window', 'achieved': round(achieved, 2), 'peak': 8000.0,
+                'unit': 'GB/s', 'frac': round(achieved / 8000.0, 5), 'traffic': traffic, 'traffic_note': tnote,
+                'bytes_per_launch': round(del_b + upd_b), 'delivery_bytes_per_launch': round(del_b),
+                'update_bytes_per_launch': round(upd_b), 'us_per_launch': round(kms * 1e3, 3),
+                'launches': n_launch}
+    log(f'k_window (ensemble): {kms * 1e3:.1f} us/launch, {(del_b + upd_b) / 1e6:.2f} MB/launch, '
+        f'{achieved:.1f} GB/s; traffic {traffic} ({tnote})')
     log(f'ensemble: {steps} steps in {wall:.3f}s, {n_sp} spikes, {by["n_events"]:.3e} events')
     # the sample networks one by one (plain runs), same steps after the same warm-up
     seq = 0.0
@@ -461,6 +494,7 @@ def run_ensemble(args):
             'config': {'workload': 'brunel iaf_psc_exp ensemble', 'networks': M, 'connectivity': args.connectivity,
                        'neurons': int(prm['N'].sum()), 'synapses_approx': syn, 'tile_size': best[0],
                        'max_window': best[1]},
+            'roofline': roofline,
             'autotuned_on_networks': len(tune_on),
             'autotune_ms_per_step': {f'{k[0]}/{k[1]}': round(v * 1e3, 4) for k, v in table.items()},
             'sample': {'networks': len(sample), 'sequential_s': round(seq, 4),
