@@ -134,11 +134,13 @@ static_assert(sizeof(Tile) % 16 == 0, "tile records are read as 16-byte words");
 
 // An incoming static projection of a post population. Its edges are grouped
 // by (pre neuron, post tile): seg[i * (nt + 1) + t] .. seg[i * (nt + 1) + t + 1]
-// are the edges from pre neuron i into tile t; an edge word holds the post
-// lane within the tile (bits 0..7) and the delay in steps (bits 8..31).
+// are the edges from pre neuron i into tile t; an edge word holds the edge's
+// offset in the post tile's LDS arrival counters, delay * RS + lane, with RS
+// = (counter words or fixed-point channels of the post population) * TS the
+// stride of one arrival step, so the gather adds it to a per-spike base.
 struct GProj {
   int pre_pop, chan, d_min, d_max, nt;
-  int pad;
+  int rs;  // step stride of the edge words (LDS counter rows of the post population)
   const uint32_t* seg;
   const uint32_t* edges;
   const long long* q;  // FIXED channels: 2^F fixed-point weights per edge
@@ -215,7 +217,7 @@ struct GInfo {
   const uint32_t* seg;
   const uint32_t* edges;
   const long long* q;
-  int pre_gid0, nt1, kind, cw, half, fx, s_lo, pad;
+  int pre_gid0, nt1, kind, cw, half, fx, s_lo, rs;  // rs: the edge words' step stride
 };
 
 struct WinLayout {
@@ -716,8 +718,8 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
     }
     const uint4* ep[kQ];
     const long long* qp[kQ];
-    int dn[kQ], kc[kQ], cwq[kQ], fxq[kQ];
-    uint32_t incq[kQ], a0[kQ], nchk[kQ];
+    int dn[kQ], kc[kQ], cwq[kQ], fxq[kQ], rsq[kQ], baseq[kQ];
+    uint32_t incq[kQ], a0[kQ], nchk[kQ], limq[kQ];
     int rounds = 0;
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
@@ -727,8 +729,13 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
       kc[q] = gi[g].kind == BS_CH_COUNT;
       cwq[q] = gi[g].cw;
       fxq[q] = gi[g].fx;
+      rsq[q] = gi[g].rs;
       incq[q] = 1u << (16 * gi[g].half);
-      dn[q] = ss[q] + 1 - a.n0;
+      dn[q] = ss[q] + 1 - a.n0;  // <= 0: the spike's step relative to the window
+      // an edge word off = d * RS + lane lands in LDS slot base + off when its
+      // arrival offset dn + d is below the horizon H, i.e. off < (H - dn) RS
+      baseq[q] = dn[q] * rsq[q] + (kc[q] ? cwq[q] : fxq[q]) * TS;
+      limq[q] = (uint32_t)(H - dn[q]) * (uint32_t)rsq[q];
       a0[q] = e0[q] & ~3u;
       nchk[q] = e1[q] > e0[q] ? (e1[q] - a0[q] + 3u) >> 2 : 0u;
       rounds = max(rounds, (int)((nchk[q] + kCH - 1) / kCH));
@@ -750,17 +757,17 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
 #pragma unroll
           for (int m = 0; m < 4; ++m) {
             const uint32_t e = a0[q] + (uint32_t)((rd * kCH + c) * 4 + m);
-            if (e >= e0[q] && e < e1[q]) {
-              const int w = dn[q] + (int)(wv[m] >> 8);
-              const int ln = (int)(wv[m] & 255u);
-              if (w < H) {  // an LDS counter (this window's steps, and later ones when they fit)
+            if (e - e0[q] < e1[q] - e0[q]) {
+              const uint32_t off = wv[m];
+              if (off < limq[q]) {  // an LDS counter (this window's steps, and later ones when they fit)
                 if (kc[q])
-                  atomicAdd(&cnt[(w * ncw + cwq[q]) * TS + ln], incq[q]);
+                  atomicAdd(&cnt[baseq[q] + (int)off], incq[q]);
                 else
-                  atomicAdd((unsigned long long*)&fxa[(w * nfx + fxq[q]) * TS + ln],
-                            (unsigned long long)qp[q][e]);
+                  atomicAdd((unsigned long long*)&fxa[baseq[q] + (int)off], (unsigned long long)qp[q][e]);
               } else {  // beyond the LDS horizon: this tile's slot of the arrival ring
-                const size_t slot = (size_t)((a.n0 + w) % a.Lr);
+                const int d = (int)(off / (uint32_t)rsq[q]);
+                const int ln = (int)(off - (uint32_t)d * (uint32_t)rsq[q]);
+                const size_t slot = (size_t)((a.n0 + dn[q] + d) % a.Lr);
                 if (kc[q])
                   __hip_atomic_fetch_add(&a.ring_cnt[(slot * a.NCW + cwq[q]) * a.Ntot + tile_gid0 + ln],
                                          incq[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1387,7 +1394,7 @@ __global__ void k_seg_bounds(const uint32_t* __restrict__ keys, long long E, int
 
 __global__ void k_seg_edges(const uint32_t* __restrict__ order, long long E, const int32_t* __restrict__ post,
                             const int32_t* __restrict__ delay, int delay_steps, const double* __restrict__ w,
-                            int F, int TS, uint32_t* __restrict__ ew, long long* __restrict__ eq) {
+                            int F, int TS, int RS, uint32_t* __restrict__ ew, long long* __restrict__ eq) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= E + 4) return;
   if (k >= E) {  // the gather reads whole 16-byte chunks past a segment's end
@@ -1396,7 +1403,7 @@ __global__ void k_seg_edges(const uint32_t* __restrict__ order, long long E, con
   }
   const uint32_t e = order[k];
   const int dd = delay ? delay[e] : delay_steps;
-  ew[k] = (uint32_t)(post[e] % TS) | ((uint32_t)dd << 8);
+  ew[k] = (uint32_t)dd * (uint32_t)RS + (uint32_t)(post[e] % TS);
   if (eq) eq[k] = (long long)llrint(ldexp(w[e], F));
 }
 
@@ -1420,8 +1427,8 @@ struct SegScratch {
 // Builds G.seg / G.edges / G.q of a static projection with device-resident
 // edges on net->stream; adds the out-degrees to net->outdeg and returns the
 // projection's delay range. Messages match the host build's.
-static int build_segments_device(bs_net* net, const bs_projection& pj, int kind, int F, int npre, int npost,
-                                 long long g0, SegScratch& S, GProj& G, int& pdmin, int& pdmax) {
+static int build_segments_device(bs_net* net, const bs_projection& pj, int kind, int F, int RS, int npre,
+                                 int npost, long long g0, SegScratch& S, GProj& G, int& pdmin, int& pdmax) {
   const int TS = net->TS;
   const long long E = pj.n_edges;
   const int nt = (npost + TS - 1) / TS;
@@ -1486,9 +1493,6 @@ static int build_segments_device(bs_net* net, const bs_projection& pj, int kind,
                        dseg, S.deg);
     HIPCHK(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_seg_edges, dim3((unsigned)((E + 4 + 255) / 256)), dim3(256), 0, st, S.kv[3], E, pj.post,
-                     pj.delay, pj.delay_steps, kind == BS_CH_FIXED ? pj.weight : nullptr, F, TS, dew, deq);
-  HIPCHK(hipGetLastError());
   int stats[4];
   std::vector<int> deg((size_t)npre);
   HIPCHK(hipMemcpyAsync(stats, S.stats, sizeof(stats), hipMemcpyDeviceToHost, st));
@@ -1496,7 +1500,6 @@ static int build_segments_device(bs_net* net, const bs_projection& pj, int kind,
   HIPCHK(hipStreamSynchronize(st));
   if (stats[0] & 1) return fail(BS_EINVAL, "edge index out of range");
   if (stats[0] & 2) return fail(BS_EINVAL, "delay must be in [0, 2^24) steps");
-  for (int a = 0; a < npre; ++a) net->outdeg[g0 + a] += deg[a];
   if (E == 0) {
     pdmin = pdmax = 0;
   } else if (!pj.delay) {
@@ -1505,6 +1508,13 @@ static int build_segments_device(bs_net* net, const bs_projection& pj, int kind,
     pdmin = stats[1];
     pdmax = stats[2];
   }
+  // edge words delay * RS + lane must fit 32 bits
+  if ((unsigned long long)std::max(0, pdmax) * (unsigned long long)RS + (unsigned long long)TS > 0xFFFFFFFFull)
+    return fail(BS_EINVAL, "delay too long for this population's arrival counters (delay * stride >= 2^32)");
+  hipLaunchKernelGGL(k_seg_edges, dim3((unsigned)((E + 4 + 255) / 256)), dim3(256), 0, st, S.kv[3], E, pj.post,
+                     pj.delay, pj.delay_steps, kind == BS_CH_FIXED ? pj.weight : nullptr, F, TS, RS, dew, deq);
+  HIPCHK(hipGetLastError());
+  for (int a = 0; a < npre; ++a) net->outdeg[g0 + a] += deg[a];
   G.seg = dseg;
   G.edges = dew;
   G.q = deq;
@@ -1737,7 +1747,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   // threads (the order inside a segment is the edge order, as before).
   struct StaticSeg {
     std::string err;
-    int pdmin = 0, pdmax = 0, nt = 0;
+    int pdmin = 0, pdmax = 0, nt = 0, rs = 0;
     std::vector<uint32_t> seg, ew;
     std::vector<long long> eq;
   };
@@ -1777,6 +1787,13 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
         __atomic_fetch_add(&net->outdeg[g0 + a], 1ll, __ATOMIC_RELAXED);
       }
       if (pj.n_edges == 0) pdmin = pdmax = 0;
+      // edge words delay * RS + lane, RS the post population's arrival-counter step stride
+      const int RS = (kind == BS_CH_COUNT ? net->pops[pj.post_pop].n_cw : net->pops[pj.post_pop].n_fx) * TS;
+      if ((unsigned long long)pdmax * (unsigned long long)RS + (unsigned long long)TS > 0xFFFFFFFFull) {
+        S.err = "delay too long for this population's arrival counters (delay * stride >= 2^32)";
+        return;
+      }
+      S.rs = RS;
       // exclusive prefix over (pre, tile) in row-major order; entry (a, nt) is the row end
       S.seg.assign((size_t)npre * (nt + 1), 0);
       uint32_t acc = 0;
@@ -1795,7 +1812,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
         const int a = pj.pre[e], b = pj.post[e];
         const int dd = pj.delay ? pj.delay[e] : pj.delay_steps;
         const uint32_t k = pos[(size_t)a * (nt + 1) + b / TS]++;
-        S.ew[k] = (uint32_t)(b % TS) | ((uint32_t)dd << 8);
+        S.ew[k] = (uint32_t)dd * (uint32_t)RS + (uint32_t)(b % TS);
         if (kind == BS_CH_FIXED) S.eq[k] = (long long)std::llrint(std::ldexp(pj.weight[e], F));
       }
       S.pdmin = pdmin;
@@ -1945,14 +1962,17 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     G.chan = pj.channel;
     int pdmin = 0, pdmax = 0;
     if (pj.on_device) {
-      CHK(build_segments_device(net, pj, kind, d->channels[pj.channel].fixed_shift, d->pops[pj.pre_pop].n,
+      const int RS = (kind == BS_CH_COUNT ? net->pops[pj.post_pop].n_cw : net->pops[pj.post_pop].n_fx) * TS;
+      CHK(build_segments_device(net, pj, kind, d->channels[pj.channel].fixed_shift, RS, d->pops[pj.pre_pop].n,
                                 d->pops[pj.post_pop].n, gid0_of[pj.pre_pop], seg_scratch, G, pdmin, pdmax));
+      G.rs = RS;
     } else {
       StaticSeg& S = sseg[q];
       if (!S.err.empty()) BAD(S.err);
       pdmin = S.pdmin;
       pdmax = S.pdmax;
       G.nt = S.nt;
+      G.rs = S.rs;
       uint32_t *dseg = nullptr, *dew = nullptr;
       long long* deq = nullptr;
       CHK(upload(net, &dseg, S.seg.data(), S.seg.size()));
@@ -2114,6 +2134,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
         I.cw = ch.cw;
         I.half = ch.half;
         I.fx = ch.fx;
+        I.rs = G.rs;
         memcpy(&B[o + L.gi + g * sizeof(GInfo)], &I, sizeof(GInfo));
       }
       for (int c = dp.ch_begin; c < dp.ch_begin + dp.ch_count; ++c) {
