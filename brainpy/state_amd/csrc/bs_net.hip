@@ -346,6 +346,28 @@ __device__ __forceinline__ double sched_at(const long long* se, const double* sv
   return (lo > 0 && n < se[2 * (lo - 1) + 1]) ? sv[lo - 1] : 0.0;
 }
 
+// Poisson drive of neuron j at step `step`: the generator's count of step
+// step - 1 - delay times the weight, or with a fractional delay the InputDelay
+// lerp (delay.py:114-128) of frames d and d + 1, x_lo (1 - frac) + x_hi frac,
+// then times the weight. cdf and guide point at the staged LDS copy or at
+// global memory.
+template <typename T>
+__device__ __forceinline__ T pois_value(const ChReg<T>& c, const double* cdf, const unsigned short* guide,
+                                        int j, int step) {
+  const int e = c.gen_index ? c.gen_index[j] : j;
+  const long long sgs = (long long)step - 1 - c.delay;
+  uint32_t cn = 0;
+  if (e >= 0 && sgs >= 0 && sgs > c.tmin && sgs <= c.tmax)
+    cn = poisson_inv(c.seed, (uint32_t)sgs, (uint32_t)e, cdf, guide, c.cdf_len, c.parts);
+  if (c.frac == 0.0) return (T)cn * c.weight;
+  const long long sgh = sgs - 1;
+  uint32_t ch = 0;
+  if (e >= 0 && sgh >= 0 && sgh > c.tmin && sgh <= c.tmax)
+    ch = poisson_inv(c.seed, (uint32_t)sgh, (uint32_t)e, cdf, guide, c.cdf_len, c.parts);
+  const T f = (T)c.frac;
+  return ((T)cn * ((T)1 - f) + (T)ch * f) * c.weight;
+}
+
 // Input of one channel at window step w for lane tid (neuron j).
 template <typename T>
 __device__ __forceinline__ T chan_value(const UpdCtx<T>& u, const ChReg<T>& c, int w, int step,
@@ -360,21 +382,7 @@ __device__ __forceinline__ T chan_value(const UpdCtx<T>& u, const ChReg<T>& c, i
   }
   if (c.kind == BS_CH_POISSON) {
     if (c.pre) return (T)u.pc[w * u.TS + lane] * c.weight;
-    const int j = u.start + lane;
-    const int e = c.gen_index ? c.gen_index[j] : j;
-    const long long sgs = (long long)step - 1 - c.delay;
-    uint32_t cn = 0;
-    if (e >= 0 && sgs >= 0 && sgs > c.tmin && sgs <= c.tmax)
-      cn = poisson_inv(c.seed, (uint32_t)sgs, (uint32_t)e, c.cdf, c.guide, c.cdf_len, c.parts);
-    if (c.frac == 0.0) return (T)cn * c.weight;
-    // InputDelay lerp (delay.py:114-128): frames d and d + 1 of the
-    // generator's output, x_lo * (1 - frac) + x_hi * frac, then * weight
-    const long long sgh = sgs - 1;
-    uint32_t ch = 0;
-    if (e >= 0 && sgh >= 0 && sgh > c.tmin && sgh <= c.tmax)
-      ch = poisson_inv(c.seed, (uint32_t)sgh, (uint32_t)e, c.cdf, c.guide, c.cdf_len, c.parts);
-    const T f = (T)c.frac;
-    return ((T)cn * ((T)1 - f) + (T)ch * f) * c.weight;
+    return pois_value<T>(c, c.cdf, c.guide, u.start + lane, step);
   }
   if (c.kind == BS_CH_CURRENT) {  // dc_generator: amplitude * weight inside [tmin, tmax)
     const int j = u.start + lane;
@@ -833,11 +841,40 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
           if (tid < W) schv[tid] = sched_at(cc.sched, cc.sval, cc.cdf_len, (long long)a.n0 + tid);
           __syncthreads();
         }
+        // one code path per channel kind (the kind is block-uniform), with
+        // the staged tables addressed in LDS
         T v[kIT];
+        const long long loff = chs[c].lds_off;
+        if (cc.kind == BS_CH_COUNT) {
+          const T* tl_ = (const T*)(tab + (loff >= 0 ? loff : 0));
+          const uint32_t nl = loff >= 0 ? (uint32_t)cc.tab_n : 0u;
 #pragma unroll
-        for (int i = 0; i < kIT; ++i) {
-          const int idx = base0 + tid + 256 * i, w = idx >> lgTS, l = idx & (TS - 1);
-          v[i] = ok[i] ? chan_value<T>(u, cc, w, a.n0 + w, l) : (T)0;
+          for (int i = 0; i < kIT; ++i) {
+            const int idx = base0 + tid + 256 * i, w = idx >> lgTS, l = idx & (TS - 1);
+            v[i] = (T)0;
+            if (ok[i]) {
+              uint32_t k = (cnt[(w * ncw + cc.cw) * TS + l] >> (16 * cc.half)) & 0xFFFFu;
+              if (k > (uint32_t)cc.max_count) {  // cannot happen for consistent edges
+                atomicOr(a.flags, FLAG_COUNT);
+                k = (uint32_t)cc.max_count;
+              }
+              v[i] = k < nl ? tl_[k] : cc.tabg[k];
+            }
+          }
+        } else if (cc.kind == BS_CH_POISSON && !cc.pre && loff >= 0) {
+          const double* cdf = (const double*)(tab + loff);
+          const unsigned short* guide = (const unsigned short*)(cdf + cc.cdf_len);
+#pragma unroll
+          for (int i = 0; i < kIT; ++i) {
+            const int idx = base0 + tid + 256 * i, w = idx >> lgTS, l = idx & (TS - 1);
+            v[i] = ok[i] ? pois_value<T>(cc, cdf, guide, tl.start + l, a.n0 + w) : (T)0;
+          }
+        } else {
+#pragma unroll
+          for (int i = 0; i < kIT; ++i) {
+            const int idx = base0 + tid + 256 * i, w = idx >> lgTS, l = idx & (TS - 1);
+            v[i] = ok[i] ? chan_value<T>(u, cc, w, a.n0 + w, l) : (T)0;
+          }
         }
         if (cc.kind == BS_CH_CURRENT || cc.kind == BS_CH_AC || cc.kind == BS_CH_SCHEDULE) {
 #pragma unroll
