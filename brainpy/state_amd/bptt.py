@@ -11,7 +11,8 @@ spike rule of ``_kernel/spiking.py:364-488`` (``count_ste`` forward/backward,
 (``bptt.py:170-215``).
 
 The forward pass, the checkpoint recomputation, the reverse-time adjoint, the
-transposed delayed delivery (one rocBLAS GEMM per window) and the weight
+transposed delayed delivery (one hand-written matrix-core GEMM per window,
+``k_bptt_tdeliver``) and the weight
 gradients all run on the device through ``bs_bptt_*`` of the C ABI. There is
 no CPU path: without the engine or a device every call raises
 :class:`~brainpy.state_amd._lib.EngineUnavailable`.

@@ -29,7 +29,7 @@ def source_hash() -> str:
 FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared', '-ffp-contract=off',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
 # rocBLAS: the dense transposed-delivery GEMM of the BPTT backward pass
-LIBS = ['-L/opt/rocm/lib', '-lrocblas', '-Wl,-rpath,/opt/rocm/lib']
+LIBS = ['-L/opt/rocm/lib', '-Wl,-rpath,/opt/rocm/lib']
 
 
 def build(verbose: bool = False) -> str:

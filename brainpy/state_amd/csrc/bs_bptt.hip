@@ -23,11 +23,16 @@
 //                                        arriving at (step, trial, neuron)
 //   sbar           float [W][B][N]       gradient w.r.t. the spikes of a window:
 //                                        W_rec . ibar(n + d), one GEMM per window
+//   ibar operands  [T][B][Kp] per E/I    the GEMM's copies of ibar, K padded to a
+//                                        multiple of 64 with zeros: bf16 hi and lo
+//                                        parts (default) or float32
+//   weight operand [N][Kp]               W_rec split (or copied) the same way,
+//                                        once per forward
+// The GEMM (k_bptt_tdeliver) is hand-written for the gfx950 matrix cores.
 // Every sum runs in a fixed order (no float atomics), so results repeat
 // bit-for-bit from run to run.
 
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 #include <stdint.h>
 
 #include <algorithm>
@@ -47,14 +52,6 @@ namespace {
     if (e_ != hipSuccess)                                                                  \
       return bs_internal_fail(BS_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
-#define RBCHK(expr)                                                                      \
-  do {                                                                                   \
-    rocblas_status s_ = (expr);                                                          \
-    if (s_ != rocblas_status_success)                                                    \
-      return bs_internal_fail(BS_EHIP, std::string(#expr) + ": rocblas status " +        \
-                                           std::to_string((int)s_));                     \
-  } while (0)
-
 constexpr int kTile = 256;       // neurons per block
 constexpr int kMaxWords = 128;   // N <= 8192
 constexpr int kMaxIn = 512;      // input sources
@@ -428,20 +425,196 @@ __device__ __forceinline__ unsigned short bf16_bits(float x) {
 }
 __device__ __forceinline__ float bf16_float(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
-// Row of the three-term split: [hi | lo | hi] for the ibar side, [hi | hi | lo]
-// for the weight side, so that one GEMM with K = 3N sums hi*hi + hi*lo + lo*hi.
-__device__ __forceinline__ void split3(float x, unsigned short* row, int N, int j, bool weight_side) {
-  const unsigned short hi = bf16_bits(x);
-  const unsigned short lo = bf16_bits(x - bf16_float(hi));
-  row[j] = hi;
-  row[N + j] = weight_side ? hi : lo;
-  row[2 * N + j] = weight_side ? lo : hi;
+// Operand rows of the transposed delivery: [rows][Kp], K padded with zeros.
+// mode 0: the bf16 parts hi = bf16(x), lo = bf16(x - hi) in two arrays; mode 1:
+// float32 copies.
+__global__ __launch_bounds__(kTile) void k_bptt_prep_w(int N, int Kp, int mode, const float* w,
+                                                       unsigned short* wh, unsigned short* wl, float* wf) {
+  const int p = (int)blockIdx.y;
+  for (int k = (int)blockIdx.x * kTile + (int)threadIdx.x; k < Kp; k += (int)gridDim.x * kTile) {
+    const float x = k < N ? w[(size_t)p * N + k] : 0.f;
+    const size_t o = (size_t)p * Kp + k;
+    if (mode == 0) {
+      const unsigned short hi = bf16_bits(x);
+      wh[o] = hi;
+      wl[o] = bf16_bits(x - bf16_float(hi));
+    } else {
+      wf[o] = x;
+    }
+  }
 }
 
-__global__ __launch_bounds__(kTile) void k_bptt_split_w(int N, const float* w, unsigned short* w3) {
-  const int p = (int)blockIdx.y;
-  const int j = (int)blockIdx.x * kTile + (int)threadIdx.x;
-  if (j < N) split3(w[(size_t)p * N + j], w3 + (size_t)p * 3 * N, N, j, true);
+// ---- transposed delayed delivery on the matrix cores
+// out[m][i] = sum_k IB_p[m][k] W[i][k] for the window's rows m = (step, trial)
+// and presynaptic neurons i, p = 0 (excitatory rows, IB = ibar of I_ex) or 1
+// (inhibitory rows, IB = ibar of I_in). Mode 0 computes each product as
+// hi*hi + hi*lo + lo*hi of the bf16 parts (v_mfma_f32_32x32x16_bf16, float32
+// accumulation: about float32 accuracy at the bf16 rate); mode 1 multiplies the
+// float32 values (v_mfma_f32_32x32x2f32). Each output is summed in one fixed
+// order, so results repeat bit for bit.
+//
+// Block: 4 waves (2 over m x 2 over i), tile 64 m x 128 i; a wave owns 32 m x
+// 64 i (two 32x32 accumulator tiles). K advances in stages of 32: every stage
+// of both operands is loaded (16-byte loads, 8 k per load) into registers one
+// stage ahead and stored into the other half of a double-buffered LDS area.
+// Blocks are dealt round-robin over the 8 XCDs (block b on XCD b % 8), so each
+// XCD is given one contiguous range of (i-tile, m-tile) pairs, i-tile major:
+// the blocks of an XCD share their weight rows and the window's ibar rows in
+// its L2, and HBM sees each operand byte about once per launch.
+constexpr int kTdBM = 64, kTdBI = 128, kTdKC = 32;
+
+struct TdArgs {
+  const void* w0;         // mode 0: W hi parts, mode 1: W float32   [N][Kp]
+  const void* w1;         // mode 0: W lo parts
+  const void* b0[2];      // per part: IB hi parts / float32, rows m [M][Kp]
+  const void* b1[2];      // per part: IB lo parts
+  float* out;             // [M][N]
+  int M, N, Kp, n_exc;
+  int nti0, nti1, ntm;    // i-tiles of the excitatory and inhibitory rows, m-tiles
+};
+
+// LDS slot (16-byte unit) of group g of row r in a stage holding KB groups per
+// row: the group index is XOR-ed with row bits so that staging stores (threads
+// t..t+KB-1: one row's groups) and fragment reads (lanes: consecutive rows,
+// one group) both spread over the 64 four-byte banks.
+template <int KB>
+__device__ __forceinline__ int td_slot(int r, int g) {
+  constexpr int sh = KB == 4 ? 2 : 1;  // rows per 256 bytes: 4 (KB = 4), 2 (KB = 8)
+  return r * KB + (g ^ ((r >> sh) & (KB - 1)));
+}
+
+__device__ __forceinline__ uint4 td_mask(uint4 v, unsigned m) {
+  return make_uint4(v.x & m, v.y & m, v.z & m, v.w & m);
+}
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 td_bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float td_f32x16;
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_bptt_tdeliver(TdArgs a) {
+  constexpr int BM = kTdBM, BI = kTdBI, KC = kTdKC;
+  constexpr int KB = MODE == 0 ? KC / 8 : KC / 4;   // 16-byte groups per row and array
+  constexpr int NA = MODE == 0 ? 2 : 1;             // arrays per operand
+  constexpr int NWL = BI * KB / 256, NBL = BM * KB / 256;  // 16-byte loads per thread and array
+  constexpr int ESZ = MODE == 0 ? 2 : 4;            // element bytes
+  constexpr int STG = NA * (BI + BM) * KB;          // 16-byte slots per stage
+  __shared__ uint4 lds[2 * STG];
+  const int total = (a.nti0 + a.nti1) * a.ntm;
+  const int per = (total + 7) / 8;
+  const int L = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+  if (L >= total) return;
+  const int it = L / a.ntm, mt = L - it * a.ntm;
+  const int part = it < a.nti0 ? 0 : 1;
+  const int i0 = part == 0 ? it * BI : a.n_exc + (it - a.nti0) * BI;
+  const int i1 = part == 0 ? a.n_exc : a.N;
+  const int m0 = mt * BM;
+  const int tid = (int)threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wm = wv & 1, wi = wv >> 1;
+  const unsigned char* W0 = (const unsigned char*)a.w0;
+  const unsigned char* W1 = (const unsigned char*)a.w1;
+  const unsigned char* B0 = (const unsigned char*)a.b0[part];
+  const unsigned char* B1 = (const unsigned char*)a.b1[part];
+  const size_t rowb = (size_t)a.Kp * ESZ;
+  // loads are unconditional (rows clamped into range) so that the compiler can
+  // count them; rows past the tile's end are zeroed when stored
+  uint4 rw[NA][NWL], rb[NA][NBL];
+#define TD_LOAD(k0_)                                                                    \
+  do {                                                                                  \
+    _Pragma("unroll") for (int j = 0; j < NWL; ++j) {                                   \
+      const int c = tid + 256 * j, r = c / KB, g = c % KB;                              \
+      const size_t o = (size_t)min(i0 + r, i1 - 1) * rowb + ((size_t)(k0_) * ESZ + g * 16); \
+      rw[0][j] = *(const uint4*)(W0 + o);                                               \
+      if (NA > 1) rw[NA - 1][j] = *(const uint4*)(W1 + o);                              \
+    }                                                                                   \
+    _Pragma("unroll") for (int j = 0; j < NBL; ++j) {                                   \
+      const int c = tid + 256 * j, r = c / KB, g = c % KB;                              \
+      const size_t o = (size_t)min(m0 + r, a.M - 1) * rowb + ((size_t)(k0_) * ESZ + g * 16); \
+      rb[0][j] = *(const uint4*)(B0 + o);                                               \
+      if (NA > 1) rb[NA - 1][j] = *(const uint4*)(B1 + o);                              \
+    }                                                                                   \
+  } while (0)
+#define TD_STORE(st_)                                                                   \
+  do {                                                                                  \
+    uint4* s_ = (st_);                                                                  \
+    _Pragma("unroll") for (int j = 0; j < NWL; ++j) {                                   \
+      const int c = tid + 256 * j, r = c / KB, g = c % KB;                              \
+      const unsigned mk = i0 + r < i1 ? ~0u : 0u;                                       \
+      _Pragma("unroll") for (int q = 0; q < NA; ++q)                                    \
+        s_[q * BI * KB + td_slot<KB>(r, g)] = td_mask(rw[q][j], mk);                    \
+    }                                                                                   \
+    _Pragma("unroll") for (int j = 0; j < NBL; ++j) {                                   \
+      const int c = tid + 256 * j, r = c / KB, g = c % KB;                              \
+      const unsigned mk = m0 + r < a.M ? ~0u : 0u;                                      \
+      _Pragma("unroll") for (int q = 0; q < NA; ++q)                                    \
+        s_[NA * BI * KB + q * BM * KB + td_slot<KB>(r, g)] = td_mask(rb[q][j], mk);     \
+    }                                                                                   \
+  } while (0)
+  td_f32x16 acc[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[t][r] = 0.f;
+  const int nk = a.Kp / KC;
+  TD_LOAD(0);
+  TD_STORE(lds);
+  __syncthreads();
+  const int rb_ = wm * 32 + (lane & 31);
+  for (int k = 0; k < nk; ++k) {
+    if (k + 1 < nk) TD_LOAD((k + 1) * KC);
+    const uint4* st = lds + (k & 1) * STG;
+    const uint4* sW = st;
+    const uint4* sB = st + NA * BI * KB;
+    if constexpr (MODE == 0) {
+#pragma unroll
+      for (int s2 = 0; s2 < KC / 16; ++s2) {
+        const int g = 2 * s2 + (lane >> 5);
+        const td_bf16x8 ah = __builtin_bit_cast(td_bf16x8, sB[td_slot<KB>(rb_, g)]);
+        const td_bf16x8 al = __builtin_bit_cast(td_bf16x8, sB[BM * KB + td_slot<KB>(rb_, g)]);
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          const int r = wi * 64 + t * 32 + (lane & 31);
+          const td_bf16x8 wh = __builtin_bit_cast(td_bf16x8, sW[td_slot<KB>(r, g)]);
+          const td_bf16x8 wl = __builtin_bit_cast(td_bf16x8, sW[BI * KB + td_slot<KB>(r, g)]);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, wh, acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, wl, acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, wh, acc[t], 0, 0, 0);
+        }
+      }
+    } else {
+      // lanes 0-31 hold k = 8 s2 + e, lanes 32-63 k = 8 s2 + 4 + e of their
+      // rows' float4 groups; MFMA e (k depth 2) takes element e of both operands
+#pragma unroll
+      for (int s2 = 0; s2 < KC / 8; ++s2) {
+        const int g = 2 * s2 + (lane >> 5);
+        const float4 av = __builtin_bit_cast(float4, sB[td_slot<KB>(rb_, g)]);
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          const int r = wi * 64 + t * 32 + (lane & 31);
+          const float4 wv4 = __builtin_bit_cast(float4, sW[td_slot<KB>(r, g)]);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, wv4.x, acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, wv4.y, acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, wv4.z, acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, wv4.w, acc[t], 0, 0, 0);
+        }
+      }
+    }
+    if (k + 1 < nk) TD_STORE(lds + ((k + 1) & 1) * STG);
+    __syncthreads();
+  }
+#undef TD_LOAD
+#undef TD_STORE
+  // accumulator layout of a 32x32 tile: lane l holds column i = l % 32 and
+  // rows m = 8 (r / 4) + 4 (l / 32) + r % 4 for r = 0..15
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int i = i0 + wi * 64 + t * 32 + (lane & 31);
+    if (i >= i1) continue;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int m = m0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+      if (m < a.M) a.out[(size_t)m * a.N + i] = acc[t][r];
+    }
+  }
 }
 
 struct BwdArgs {
@@ -457,7 +630,9 @@ struct BwdArgs {
   float mscale;
   float *ay, *ae, *ai;
   float *ibex, *ibin;
-  unsigned short *ib3e, *ib3i;  // bf16 split rows [T][B][3N], or null for fp32 GEMMs
+  int Kp, mode;                  // GEMM operand rows: Kp columns, mode 0 bf16 parts, 1 float32
+  unsigned short *ibh[2], *ibl[2];  // mode 0: hi / lo parts of ibar (E, I) [T][B][Kp]
+  float* ibf[2];                    // mode 1: float32 copies of ibar (E, I) [T][B][Kp]
 };
 
 // Reverse-time adjoint of one window for one (trial, neuron tile). The carry
@@ -484,10 +659,16 @@ __global__ __launch_bounds__(kTile) void k_bptt_bwd(BwdArgs a) {
     const float sb = a.sbar[((size_t)k * a.B + b) * a.N + j] + g_rate;
     a.ibex[(size_t)n * BN + bj] = ge;
     a.ibin[(size_t)n * BN + bj] = gi;
-    if (a.ib3e) {
-      const size_t row = ((size_t)n * a.B + b) * 3 * a.N;
-      split3(ge, a.ib3e + row, a.N, j, false);
-      split3(gi, a.ib3i + row, a.N, j, false);
+    const size_t ro = ((size_t)n * a.B + b) * a.Kp + j;  // the GEMM's operand rows
+    if (a.mode == 0) {
+      const unsigned short he = bf16_bits(ge), hi_ = bf16_bits(gi);
+      a.ibh[0][ro] = he;
+      a.ibl[0][ro] = bf16_bits(ge - bf16_float(he));
+      a.ibh[1][ro] = hi_;
+      a.ibl[1][ro] = bf16_bits(gi - bf16_float(hi_));
+    } else {
+      a.ibf[0][ro] = ge;
+      a.ibf[1][ro] = gi;
     }
     const float gp = (hard ? 0.f : gv) + sb * ds;  // d loss / d V before the reset
     if (ref) {
@@ -615,12 +796,6 @@ struct bs_bptt {
   Consts c;
   int N = 0, NW = 0, W = 0, nck = 0;
   hipStream_t stream = nullptr;
-  rocblas_handle blas = nullptr;
-  // the inhibitory rows' GEMM runs on a second stream beside the excitatory
-  // one (the smaller GEMM alone leaves most CUs idle); fork/join events
-  hipStream_t stream2 = nullptr;
-  rocblas_handle blas2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::vector<void*> owned;
   unsigned long long* bm = nullptr;
   unsigned long long* bmT = nullptr;
@@ -637,7 +812,11 @@ struct bs_bptt {
   float* vbuf = nullptr;
   float *ibex = nullptr, *ibin = nullptr;
   float *xin_e = nullptr, *xin_i = nullptr;  // stored summed inputs (segment replay)
-  unsigned short *w3 = nullptr, *ib3e = nullptr, *ib3i = nullptr;
+  int Kp = 0;                                           // GEMM operand row length
+  unsigned short *wh = nullptr, *wl = nullptr;          // mode 0: W_rec parts [N][Kp]
+  unsigned short *ibh[2] = {nullptr, nullptr}, *ibl[2] = {nullptr, nullptr};
+  float* wf = nullptr;                                  // mode 1: W_rec [N][Kp]
+  float* ibf[2] = {nullptr, nullptr};
 
   double* lpart_blocks = nullptr;
   float* sbar = nullptr;
@@ -670,11 +849,6 @@ int bmalloc(bs_bptt* h, X** p, size_t n) {
 void free_bptt(bs_bptt* h) {
   if (!h) return;
   for (void* p : h->owned) (void)hipFree(p);
-  if (h->blas) (void)rocblas_destroy_handle(h->blas);
-  if (h->blas2) (void)rocblas_destroy_handle(h->blas2);
-  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
-  if (h->ev_join) (void)hipEventDestroy(h->ev_join);
-  if (h->stream2) (void)hipStreamDestroy(h->stream2);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
@@ -805,10 +979,22 @@ int bs_bptt_create(const bs_bptt_desc* d, bs_bptt** out) {
     BALLOC(h->xin_e, (size_t)T * BN);
     BALLOC(h->xin_i, (size_t)T * BN);
   }
+  if (d->gemm != 0 && d->gemm != 1) {
+    free_bptt(h);
+    return bs_internal_fail(BS_EINVAL, "bs_bptt_create: gemm must be 0 (bf16 split) or 1 (float32)");
+  }
+  h->Kp = (int)((N + 63) / 64 * 64);
+  const size_t TBK = (size_t)T * B * h->Kp;
   if (d->gemm == 0) {
-    BALLOC(h->w3, (size_t)N * 3 * N);
-    BALLOC(h->ib3e, (size_t)T * BN * 3);
-    BALLOC(h->ib3i, (size_t)T * BN * 3);
+    BALLOC(h->wh, (size_t)N * h->Kp);
+    BALLOC(h->wl, (size_t)N * h->Kp);
+    for (int p = 0; p < 2; ++p) {
+      BALLOC(h->ibh[p], TBK);
+      BALLOC(h->ibl[p], TBK);
+    }
+  } else {
+    BALLOC(h->wf, (size_t)N * h->Kp);
+    for (int p = 0; p < 2; ++p) BALLOC(h->ibf[p], TBK);
   }
   BALLOC(h->lpart_blocks, kLossBlocks);
 
@@ -822,20 +1008,17 @@ int bs_bptt_create(const bs_bptt_desc* d, bs_bptt** out) {
     free_bptt(h);
     return bs_internal_fail(BS_EHIP, "bs_bptt_create: hipStreamCreate failed");
   }
-  if (hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess) {
-    free_bptt(h);
-    return bs_internal_fail(BS_EHIP, "bs_bptt_create: second stream or events");
-  }
-  if (rocblas_create_handle(&h->blas) != rocblas_status_success ||
-      rocblas_set_stream(h->blas, h->stream) != rocblas_status_success ||
-      rocblas_set_atomics_mode(h->blas, rocblas_atomics_not_allowed) != rocblas_status_success ||
-      rocblas_create_handle(&h->blas2) != rocblas_status_success ||
-      rocblas_set_stream(h->blas2, h->stream2) != rocblas_status_success ||
-      rocblas_set_atomics_mode(h->blas2, rocblas_atomics_not_allowed) != rocblas_status_success) {
-    free_bptt(h);
-    return bs_internal_fail(BS_EHIP, "bs_bptt_create: rocblas handle");
+  // the K padding of the ibar operand rows stays zero (the adjoint writes
+  // columns < N only)
+  for (int p = 0; p < 2; ++p) {
+    if (h->ibh[p] && (hipMemset(h->ibh[p], 0, 2 * TBK) != hipSuccess || hipMemset(h->ibl[p], 0, 2 * TBK) != hipSuccess)) {
+      free_bptt(h);
+      return bs_internal_fail(BS_EHIP, "bs_bptt_create: hipMemset of the GEMM operands failed");
+    }
+    if (h->ibf[p] && hipMemset(h->ibf[p], 0, 4 * TBK) != hipSuccess) {
+      free_bptt(h);
+      return bs_internal_fail(BS_EHIP, "bs_bptt_create: hipMemset of the GEMM operands failed");
+    }
   }
   *out = h;
   return BS_OK;
@@ -875,9 +1058,9 @@ int bs_bptt_forward(bs_bptt* h, const float* w_in, const float* w_rec, const uin
                      1.0 / T_s, inv_count, h->counts, h->lpart, target, h->grate, h->lpart_blocks);
   hipLaunchKernelGGL(k_bptt_loss_final, dim3(1), dim3(kTile), 0, h->stream, inv_count,
                      (const double*)h->lpart_blocks, h->d_loss);
-  if (h->w3)  // the weight side of the three-term split, once per forward
-    hipLaunchKernelGGL(k_bptt_split_w, dim3((unsigned)((N + kTile - 1) / kTile), (unsigned)N), dim3(kTile),
-                       0, h->stream, N, w_rec, h->w3);
+  // the weight operand of the transposed delivery, once per forward
+  hipLaunchKernelGGL(k_bptt_prep_w, dim3((unsigned)((h->Kp + kTile - 1) / kTile), (unsigned)N), dim3(kTile), 0,
+                     h->stream, N, h->Kp, h->d.gemm, w_rec, h->wh, h->wl, h->wf);
   BCHK(hipGetLastError());
   BCHK(hipMemcpyAsync(loss, h->d_loss, sizeof(double), hipMemcpyDeviceToHost, h->stream));
   int err = 0;
@@ -923,9 +1106,22 @@ int bs_bptt_backward(bs_bptt* h, float* g_in, float* g_rec) {
   ba.ai = h->ai;
   ba.ibex = h->ibex;
   ba.ibin = h->ibin;
-  ba.ib3e = h->ib3e;
-  ba.ib3i = h->ib3i;
-  const float one = 1.0f, zero = 0.0f;
+  ba.Kp = h->Kp;
+  ba.mode = h->d.gemm;
+  for (int p = 0; p < 2; ++p) {
+    ba.ibh[p] = h->ibh[p];
+    ba.ibl[p] = h->ibl[p];
+    ba.ibf[p] = h->ibf[p];
+  }
+  TdArgs ta{};
+  ta.w0 = h->d.gemm == 0 ? (const void*)h->wh : (const void*)h->wf;
+  ta.w1 = h->wl;
+  ta.out = h->sbar;
+  ta.N = N;
+  ta.Kp = h->Kp;
+  ta.n_exc = nE;
+  ta.nti0 = (nE + kTdBI - 1) / kTdBI;
+  ta.nti1 = (nI + kTdBI - 1) / kTdBI;
   for (int seg = h->nck - 1; seg >= 0; --seg) {
     const int s0 = seg * C, s1 = std::min(T, s0 + C);
     fa.seg0 = s0;
@@ -946,39 +1142,19 @@ int bs_bptt_backward(bs_bptt* h, float* g_in, float* g_rec) {
                             h->stream));
       if (kmax > 0) {
         const int M = kmax * B;
-        const size_t src = (size_t)(w0 + d) * BN;
-        // I rows on the second stream, E rows here; disjoint rows of sbar
-        if (nI > 0) {
-          BCHK(hipEventRecord(h->ev_fork, h->stream));
-          BCHK(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+        const size_t src = (size_t)(w0 + d) * B * h->Kp;  // first operand row: (step w0 + d, trial 0)
+        for (int p = 0; p < 2; ++p) {
+          ta.b0[p] = h->d.gemm == 0 ? (const void*)(h->ibh[p] + src) : (const void*)(h->ibf[p] + src);
+          ta.b1[p] = h->d.gemm == 0 ? (const void*)(h->ibl[p] + src) : nullptr;
         }
-        if (h->w3) {  // bf16 three-term split on the matrix cores, fp32 accumulate
-          const int K3 = 3 * N;
-          if (nE > 0)
-            RBCHK(rocblas_gemm_ex(h->blas, rocblas_operation_transpose, rocblas_operation_none, nE, M, K3,
-                                  &one, h->w3, rocblas_datatype_bf16_r, K3, h->ib3e + src * 3,
-                                  rocblas_datatype_bf16_r, K3, &zero, h->sbar, rocblas_datatype_f32_r, N,
-                                  h->sbar, rocblas_datatype_f32_r, N, rocblas_datatype_f32_r,
-                                  rocblas_gemm_algo_standard, 0, 0));
-          if (nI > 0)
-            RBCHK(rocblas_gemm_ex(h->blas2, rocblas_operation_transpose, rocblas_operation_none, nI, M, K3,
-                                  &one, h->w3 + (size_t)nE * K3, rocblas_datatype_bf16_r, K3,
-                                  h->ib3i + src * 3, rocblas_datatype_bf16_r, K3, &zero, h->sbar + nE,
-                                  rocblas_datatype_f32_r, N, h->sbar + nE, rocblas_datatype_f32_r, N,
-                                  rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0));
-        } else {
-          if (nE > 0)
-            RBCHK(rocblas_sgemm(h->blas, rocblas_operation_transpose, rocblas_operation_none, nE, M, N,
-                                &one, h->w_rec, N, h->ibex + src, N, &zero, h->sbar, N));
-          if (nI > 0)
-            RBCHK(rocblas_sgemm(h->blas2, rocblas_operation_transpose, rocblas_operation_none, nI, M, N,
-                                &one, h->w_rec + (size_t)nE * N, N, h->ibin + src, N, &zero,
-                                h->sbar + nE, N));
-        }
-        if (nI > 0) {
-          BCHK(hipEventRecord(h->ev_join, h->stream2));
-          BCHK(hipStreamWaitEvent(h->stream, h->ev_join, 0));
-        }
+        ta.M = M;
+        ta.ntm = (M + kTdBM - 1) / kTdBM;
+        const int tiles = (ta.nti0 + ta.nti1) * ta.ntm;
+        const unsigned tgrid = (unsigned)((tiles + 7) / 8 * 8);
+        if (h->d.gemm == 0)
+          hipLaunchKernelGGL(k_bptt_tdeliver<0>, dim3(tgrid), dim3(256), 0, h->stream, ta);
+        else
+          hipLaunchKernelGGL(k_bptt_tdeliver<1>, dim3(tgrid), dim3(256), 0, h->stream, ta);
       }
       ba.n0 = w0;
       ba.nsteps = nsteps;
