@@ -15,7 +15,7 @@ __all__ = ['lib', 'EngineUnavailable', 'EngineError', 'check', 'BS_NPARAM', 'BS_
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libbrainspike_hip.so')
 
-BS_ABI_VERSION = 8
+BS_ABI_VERSION = 9
 BS_NPARAM = 40
 BS_NSTATE = 9
 BS_F64, BS_F32 = 0, 1
@@ -75,7 +75,7 @@ class NetDesc(C.Structure):
                 ('max_window', C.c_int32), ('pops', C.POINTER(Population)),
                 ('channels', C.POINTER(Channel)), ('projs', C.POINTER(Projection)),
                 ('tables', C.POINTER(C.c_double)), ('table_off', C.POINTER(C.c_int64)),
-                ('n_tables', C.c_int64), ('tile_size', C.c_int32), ('_pad2', C.c_int32)]
+                ('n_tables', C.c_int64), ('tile_size', C.c_int32), ('plastic_records', C.c_int32)]
 
 
 class BpttDesc(C.Structure):

@@ -29,12 +29,19 @@
 //     records of a chunk go to one of two buffers: while the device runs
 //     chunk c, a second stream sorts chunk c - 1's records and the host
 //     decodes them from pinned memory.
-//   * Plastic (stdp_synapse) projections force W = 1 and add two kernels per
-//     step: k_stdp_traces (pre-trace history and post traces) and
-//     k_stdp_events (facilitation of the incoming edges of neurons that fired
-//     in the previous step, then depression of the edges whose pre spike
-//     arrives, depositing their new weights as exact 2^-F fixed-point sums
-//     that the window kernel reads as a channel).
+//   * Plastic (stdp_synapse) projections run inside the same window kernel.
+//     Every weight update of an edge is caused by its post neuron (a post
+//     spike facilitates) or by its own arrival at the post neuron (depression
+//     and delivery), so the thread of the post neuron owns its incoming edges
+//     for the window: the arrivals of the window are gathered into LDS lists
+//     per (step, lane), and the update loop applies, per arriving edge, the
+//     facilitations still pending from the neuron's earlier spikes in the
+//     window, then the arrival's facilitate/depress, and deposits the new
+//     weight as an exact 2^-F fixed-point sum into that step's input. After
+//     the loop the block facilitates the remaining incoming edges of the
+//     neurons that fired (those with no later arrival in the window). The pre
+//     traces are a per-neuron history ring written by the pre neuron's thread;
+//     everything read from it lies before the window (W <= d_min + 1).
 #include <hip/hip_runtime.h>
 #include <hipcub/device/device_radix_sort.hpp>
 #include <stdint.h>
@@ -83,7 +90,8 @@ int bs_internal_fail(int code, const std::string& msg) { return fail(code, msg);
   } while (0)
 
 // ------------------------------------------------------------ device types
-enum { FLAG_OVERFLOW = 1, FLAG_UNSTABLE = 2, FLAG_COUNT = 4 };
+enum { FLAG_OVERFLOW = 1, FLAG_UNSTABLE = 2, FLAG_COUNT = 4, FLAG_PLASTIC = 8 };
+#define BS_MAX_PL 4  // plastic projections into (and out of) one population
 
 struct DevPop {
   int model, n, gid0, ch_begin, ch_count;
@@ -93,6 +101,14 @@ struct DevPop {
   int gp_begin, gp_count;  // incoming static projections (gather list)
   int tile0;         // first tile of this population
   int n_cur;         // current-input (dc_generator) channels
+  // plastic projections: n_pl into this population (post side, channel
+  // order), n_plpre out of it (pre side: the neurons write pre-trace history);
+  // indices into the StdpDev array. n_tail: static channels summed after a
+  // plastic channel of the same receptor (their values wait in LDS for the
+  // update loop, which completes the sum in channel order). pl_cap: LDS
+  // arrival records per sub-window.
+  int n_pl, n_plpre, n_tail, pl_cap;
+  int pl[BS_MAX_PL], plpre[BS_MAX_PL];
   long long tab_bytes;  // bytes of tables staged into LDS by k_window
   double P[BS_NPARAM];
 };
@@ -114,7 +130,8 @@ struct DevChan {
   const int* gen_index;
   const double* cdf;             // POISSON: CDF table of Poisson(lam / parts)
   const unsigned short* guide;   // POISSON: 64-entry guide table
-  long long* pacc;               // FIXED fed by a plastic projection: deposits per post
+  int plastic;                   // FIXED of a plastic projection: its index in the population's pl list, else -1
+  int tail;                      // static channel summed after a plastic one: its LDS tail slot, else -1
   double frac;                   // POISSON: InputDelay lerp weight of frame delay + 1 (0: whole steps)
   const long long* sched;        // SCHEDULE: [2 * cdf_len] plateau start, stop steps
   const double* sval;            // SCHEDULE: [cdf_len] plateau values
@@ -153,6 +170,41 @@ struct PJob {
   int n, ch;
 };
 
+// Plastic projection (stdp_synapse.py:163-179 on event_plastic.py:454-536),
+// per-edge delays d >= 1. At step n, for edge e (pre i, post j, delay d):
+//   pre trace   tr_e(n) = U_i(n - d),  U_i(m) = U_i(m-1) P+ + s_i(m-1), U_i(0) = 0
+//   post trace  V_j(n)  = V_j(n-1) P- + s_j(n-1)
+//   post fired  s_j(n-1); the edge arrives at n iff s_i(n - 1 - d)
+//   where post fired: w <- facilitate(w, tr - arrives), then
+//   where it arrives: w <- depress(w, V - post fired), and the new w is
+//   delivered to j at step n.
+// Edges are stored grouped by (pre, post tile) like the static gather
+// segments; the position in that order indexes w, la and e_pre. The CSC index
+// by post lists the positions of each post neuron's incoming edges.
+struct StdpDev {
+  int pre_pop, post_pop, n_pre, n_post, d_min, d_max, Lu, shift, nt, pad;
+  long long pre_gid0, post_gid0;
+  const uint32_t* seg;      // [n_pre][nt + 1] first position of each (pre, tile) segment; [.][nt] row end
+  const uint32_t* ew;       // [E] delay << 8 | lane in the post tile
+  const int* e_pre;         // [E] pre neuron (local index)
+  const unsigned* csc_ptr;  // [n_post + 1]
+  const unsigned* csc_eid;  // [E] positions grouped by post, ascending
+  double* w;                // [E]
+  int* la;                  // [E] step of the edge's last arrival (-1: none yet)
+  double* U;                // [Lu][n_pre] pre-trace history, slot m % Lu holds U_i(m)
+  double* V;                // [n_post] post trace V_j of the last step run
+  double decay_plus, decay_minus, lam, alpha, mup, mum, Wmax;
+};
+
+__device__ __forceinline__ double stdp_facilitate(const StdpDev& p, double w, double kplus) {
+  const double nw = w / p.Wmax + p.lam * pow_p(1.0 - w / p.Wmax, p.mup) * kplus;
+  return nw < 1.0 ? nw * p.Wmax : p.Wmax;
+}
+__device__ __forceinline__ double stdp_depress(const StdpDev& p, double w, double kminus) {
+  const double nw = w / p.Wmax - p.alpha * p.lam * pow_p(w / p.Wmax, p.mum) * kminus;
+  return nw > 0.0 ? nw * p.Wmax : 0.0;
+}
+
 struct WinArgs {
   const Tile* tiles;
   int n_tiles;               // blocks past n_tiles are Poisson producers
@@ -175,6 +227,8 @@ struct WinArgs {
   const long long* base_ptr;  // graph replays: n0 = *base_ptr + widx * Wmax
   int widx;
   unsigned char* spk_hist;  // [Ls][Ntot] spike flags (plastic networks), or null
+  const StdpDev* stdp;      // plastic projections
+  int psp;                  // plastic gather: source steps per projection and window, at most
   int max_span;        // longest gathered step range, LDS prefix size
   int TS;              // neurons per tile
   // arrival ring of events that land after their delivery window:
@@ -275,7 +329,7 @@ struct ChReg {
   const double* cdf;            // Poisson CDF (LDS or global)
   const unsigned short* guide;  // Poisson guide table
   const int* gen_index;
-  long long* pacc;              // plastic deposits (FIXED channel of a plastic projection)
+  int plastic, tail;            // see DevChan
   double frac;                  // POISSON: lerp weight of the frame one step further back
   const long long* sched;       // SCHEDULE: plateau steps and values (global memory)
   const double* sval;
@@ -301,7 +355,8 @@ __device__ __forceinline__ ChReg<T> load_ch(const DevChan& ch, const unsigned ch
   c.tmax = ch.tmax;
   c.weight = (T)ch.weight;
   c.gen_index = ch.gen_index;
-  c.pacc = ch.pacc;
+  c.plastic = ch.plastic;
+  c.tail = ch.tail;
   c.frac = ch.frac;
   c.sched = ch.sched;
   c.sval = ch.sval;
@@ -402,14 +457,7 @@ __device__ __forceinline__ T chan_value(const UpdCtx<T>& u, const ChReg<T>& c, i
     const double t = (double)step * p[5];
     return (T)((p[0] * sin_p(p[1] * t + p[2]) + p[3]) * p[4]);
   }
-  long long q;  // BS_CH_FIXED
-  if (c.pacc) {  // plastic: this step's deposits (windows of one step), consumed here
-    const int j = u.start + lane;
-    q = c.pacc[j];
-    c.pacc[j] = 0;
-  } else {
-    q = u.fxa[(w * u.nfx + c.fx) * u.TS + lane];
-  }
+  const long long q = u.fxa[(w * u.nfx + c.fx) * u.TS + lane];  // BS_CH_FIXED
   return (T)__builtin_ldexp((double)q, -c.shift);
 }
 
@@ -446,11 +494,354 @@ __device__ __forceinline__ void update_loop(const UpdCtx<T>& u, int ns, F stepf)
   a.ref[u.gid] = r;
 }
 
+// ------------------------------------------------------------ plastic path
+// LDS of a population with plastic inputs, after the window layout: bucket
+// prefix per plastic projection over its source steps, per-(step, projection,
+// lane) arrival offsets (scanned in place, then cursors), scan partials, the
+// facilitation masks and the list of lanes that fired, the tail channel
+// values, and the arrival records (position, pre trace) of one sub-window.
+struct PlLayout {
+  size_t ppref, poff, part, fm, fl, tail, rk, rtr, end;
+};
+__host__ __device__ inline PlLayout pl_layout(size_t base, int Wm, int TS, int npl, int psp, int ntail, int tsz,
+                                              int cap) {
+  PlLayout L;
+  L.ppref = al16(base);
+  L.poff = al16(L.ppref + (size_t)npl * (psp + 1) * 4);
+  L.part = al16(L.poff + ((size_t)Wm * npl * TS + 1) * 4);
+  L.fm = al16(L.part + 256 * 4);
+  L.fl = al16(L.fm + (size_t)TS * 4);
+  L.tail = al16(L.fl + ((size_t)TS + 1) * 4);
+  L.rk = al16(L.tail + (size_t)ntail * Wm * TS * tsz);
+  L.rtr = al16(L.rk + (size_t)cap * 4);
+  L.end = al16(L.rtr + (size_t)cap * 8);
+  return L;
+}
+
+struct PlCtx {
+  int* ppref;
+  int* poff;
+  int* part;
+  unsigned* fm;
+  int* fl;
+  const void* tail;  // T [n_tail][Wm][TS]
+  uint32_t* rk;
+  double* rtr;
+  int npl, nplpre, cap, S;  // S = npl * TS keys per window step
+  int tile_in_pop;
+};
+
+// Bucket sizes of the source steps of every plastic projection into this
+// population: a spike of step s arrives at s + 1 + d, inside the window for
+// s in [n0 - 1 - d_max, n0 + W - 2 - d_min]; inclusive prefix per projection.
+__device__ __forceinline__ void pl_prefix(const WinArgs& a, const DevPop& pop, const PlCtx& c, int tid) {
+  const int P1 = a.psp + 1;
+  for (int k = tid; k < c.npl * P1; k += 256) {
+    const int p = k / P1, t = k - p * P1;
+    int v = 0;
+    if (t > 0) {
+      const StdpDev& P = a.stdp[pop.pl[p]];
+      const int s_lo = max(0, a.n0 - 1 - P.d_max), s_hi = a.n0 + a.W - 2 - P.d_min;
+      const int s = s_lo + t - 1;
+      if (s <= s_hi) v = (int)a.sb_count[(s % a.Ls) * a.n_pops + P.pre_pop];
+    }
+    c.ppref[k] = v;
+  }
+  __syncthreads();
+  if (tid < c.npl) {
+    int acc = 0;
+    for (int t = 1; t < P1; ++t) {
+      acc += c.ppref[tid * P1 + t];
+      c.ppref[tid * P1 + t] = acc;
+    }
+  }
+  __syncthreads();
+}
+
+// One walk over the plastic arrivals of window steps [wa, wb): MODE 0 counts
+// them per key (step, projection, lane); MODE 1 places each record at its
+// key's cursor. One spike per thread; a (pre, tile) segment holds a few edges.
+template <int MODE>
+__device__ __forceinline__ void pl_pass(const WinArgs& a, const DevPop& pop, const PlCtx& c, int TS, int wa, int wb, int tid) {
+  const int P1 = a.psp + 1;
+  for (int p = 0; p < c.npl; ++p) {
+    const StdpDev& P = a.stdp[pop.pl[p]];
+    const int* pr = c.ppref + p * P1;
+    const int tot = pr[P1 - 1];
+    const int s_lo = max(0, a.n0 - 1 - P.d_max);
+    for (int idx = tid; idx < tot; idx += 256) {
+      int lo = 0, hi = P1 - 1;  // pr[lo] <= idx < pr[lo + 1]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pr[mid] <= idx) lo = mid; else hi = mid;
+      }
+      const int s = s_lo + lo;
+      const int i = (int)a.sb[(long long)(s % a.Ls) * a.Ntot + P.pre_gid0 + (idx - pr[lo])];
+      const uint32_t* sg = P.seg + (long long)i * (P.nt + 1) + c.tile_in_pop;
+      const uint32_t k0 = sg[0], k1 = sg[1];
+      double tr = 0.0;
+      if (MODE == 1 && k1 > k0) tr = P.U[(long long)((s + 1) % P.Lu) * P.n_pre + i];
+      for (uint32_t k = k0; k < k1; ++k) {
+        const uint32_t word = P.ew[k];
+        const int w = s + 1 + (int)(word >> 8) - a.n0;
+        if (w < wa || w >= wb) continue;
+        const int key = (w * c.npl + p) * TS + (int)(word & 255u);
+        if (MODE == 0) {
+          atomicAdd(&c.poff[key], 1);
+        } else {
+          const int pos = atomicAdd(&c.poff[key], 1);
+          if (pos < c.cap) {
+            c.rk[pos] = k;
+            c.rtr[pos] = tr;
+          }
+        }
+      }
+    }
+  }
+}
+
+// In-place exclusive scan of x[0 .. M), x[M] = total (256 threads).
+__device__ __forceinline__ void block_exscan(int* x, int M, int* part, int tid) {
+  const int per = (M + 255) / 256;
+  const int b = min(tid * per, M), e = min(b + per, M);
+  int sum = 0;
+  for (int k = b; k < e; ++k) sum += x[k];
+  part[tid] = sum;
+  __syncthreads();
+  if (tid < 64) {  // one wave: four partials per lane, then a shuffle scan
+    int v[4], t = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      v[q] = part[tid * 4 + q];
+      t += v[q];
+    }
+    int incl = t;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(incl, o, 64);
+      if (tid >= o) incl += y;
+    }
+    int run = incl - t;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      part[tid * 4 + q] = run;
+      run += v[q];
+    }
+    if (tid == 63) x[M] = incl;
+  }
+  __syncthreads();
+  int run = part[tid];
+  for (int k = b; k < e; ++k) {
+    const int t = x[k];
+    x[k] = run;
+    run += t;
+  }
+  __syncthreads();
+}
+
+// Facilitations an edge still owes: post spikes at f - 1 for the window steps
+// f = n0 + b, b in mask (ascending), each with the pre trace U_i(f - d) and no
+// arrival at f (k+ = trace - 0).
+__device__ __forceinline__ double stdp_pending(const StdpDev& P, double w, unsigned long long m, int n0, int i,
+                                               int d) {
+  while (m) {
+    const int b = __builtin_ctzll(m);
+    m &= m - 1ull;
+    const int mu = n0 + b - d;
+    const double tr = mu >= 0 ? P.U[(long long)(mu % P.Lu) * P.n_pre + i] : 0.0;
+    w = stdp_facilitate(P, w, tr - 0.0);
+  }
+  return w;
+}
+
+// The W-step loop of a population with plastic inputs or outputs, run by all
+// 256 threads (block barriers between sub-windows). Lane j < TS advances its
+// neuron as update_loop does; in addition, at each step it updates its post
+// traces, processes the plastic edges arriving at it (pending facilitations,
+// facilitate/depress, deposit) and completes the channel sum in channel
+// order, and it writes its pre-trace history. After the loop the block
+// facilitates the incoming edges of the neurons that fired and that no later
+// arrival of the window has brought up to date.
+template <typename T, bool HAS_CUR, typename F>
+__device__ __forceinline__ void update_loop_pl(const UpdCtx<T>& u, const PlCtx& c, const DevPop& pop, int ns, F stepf) {
+  const WinArgs& a = u.a;
+  const int tid = u.tid, TS = u.TS;
+  const bool act = u.active && tid < TS;
+  T* st = (T*)a.state;
+  T s[BS_NSTATE];
+  int r = 0;
+#pragma unroll
+  for (int k = 0; k < BS_NSTATE; ++k) s[k] = (T)0;
+  double vtr[BS_MAX_PL], utr[BS_MAX_PL];
+#pragma unroll
+  for (int q = 0; q < BS_MAX_PL; ++q) vtr[q] = utr[q] = 0.0;
+  unsigned fmask = 0u;  // bit b: the neuron fired at step n0 + b - 1
+  const int il = u.start + tid;  // local index in the population
+  int vmc = -1;
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < BS_NSTATE; ++k)
+      if (k < ns) s[k] = st[(long long)k * a.Ntot + u.gid];
+    r = a.ref[u.gid];
+    vmc = a.n_vm > 0 ? a.vm_col[u.gid] : -1;
+    if (a.n0 >= 1 && a.spk_hist[(long long)((a.n0 - 1) % a.Ls) * a.Ntot + u.gid]) fmask = 1u;
+    // (constant indices into the trace arrays keep them in registers)
+#pragma unroll
+    for (int p = 0; p < BS_MAX_PL; ++p)
+      if (p < c.npl) vtr[p] = a.stdp[pop.pl[p]].V[il];
+#pragma unroll
+    for (int q = 0; q < BS_MAX_PL; ++q)
+      if (q < c.nplpre) {
+        const StdpDev& P = a.stdp[pop.plpre[q]];
+        utr[q] = P.U[(long long)(a.n0 % P.Lu) * P.n_pre + il];
+      }
+  }
+  if (c.npl > 0) {  // arrival counts of the window, scanned into offsets
+    const int M = a.W * c.S;
+    for (int k = tid; k <= M; k += 256) c.poff[k] = 0;
+    pl_prefix(a, pop, c, tid);  // (its barriers also order the zeroing)
+    pl_pass<0>(a, pop, c, TS, 0, a.W, tid);
+    __syncthreads();
+    block_exscan(c.poff, M, c.part, tid);
+  }
+  const T* tail = (const T*)c.tail;
+  for (int wa = 0; wa < a.W;) {
+    int wb = a.W;
+    if (c.npl > 0) {  // the longest run of steps whose records fit (at least one step)
+      const int base = c.poff[wa * c.S];
+      if (c.poff[(wa + 1) * c.S] - base > c.cap && tid == 0) atomicOr(a.flags, FLAG_PLASTIC);
+      wb = wa + 1;
+      while (wb < a.W && c.poff[(wb + 1) * c.S] - base <= c.cap) ++wb;
+      __syncthreads();
+      for (int k = wa * c.S + tid; k < wb * c.S; k += 256) c.poff[k] -= base;
+      __syncthreads();
+      pl_pass<1>(a, pop, c, TS, wa, wb, tid);
+      __syncthreads();
+    }
+    if (act) {
+      for (int w = wa; w < wb; ++w) {
+        const int step = a.n0 + w;
+        T acc0 = u.inp[w * TS + tid];
+        T acc1 = u.nrec > 1 ? u.inp[(u.Wm + w) * TS + tid] : (T)0;
+        T cur = (T)0;
+        if constexpr (HAS_CUR) cur = u.inp[(u.nrec * u.Wm + w) * TS + tid];
+        const bool post_fired = (fmask >> w) & 1u;
+        if (c.npl > 0) {
+#pragma unroll
+          for (int p = 0; p < BS_MAX_PL; ++p)
+            if (p < c.npl) vtr[p] = vtr[p] * a.stdp[pop.pl[p]].decay_minus + (post_fired ? 1.0 : 0.0);
+          // the channels from the first plastic one of each receptor, in order
+          for (int ch = 0; ch < u.nch; ++ch) {
+            const DevChan& dc = u.chs[ch];
+            T v;
+            if (dc.plastic >= 0) {
+              const int p = dc.plastic;
+              const StdpDev& P = a.stdp[pop.pl[p]];
+              double vt = vtr[0];
+#pragma unroll
+              for (int q = 1; q < BS_MAX_PL; ++q)
+                if (q == p) vt = vtr[q];
+              const int key = (w * c.npl + p) * TS + tid;
+              const int b = key == wa * c.S ? 0 : min(c.poff[key - 1], c.cap);
+              const int e = min(c.poff[key], c.cap);
+              long long dep = 0;
+              for (int q = b; q < e; ++q) {
+                const uint32_t k = c.rk[q];
+                double wv = P.w[k];
+                const int la = P.la[k];
+                const int i = P.e_pre[k];
+                const int d = (int)(P.ew[k] >> 8);
+                // facilitations since the edge's last arrival (or the window start)
+                const int lo = la > a.n0 - 1 ? la : a.n0 - 1;
+                const unsigned long long m =
+                    (unsigned long long)fmask & ((1ull << w) - 1ull) & ~((1ull << (lo - a.n0 + 1)) - 1ull);
+                wv = stdp_pending(P, wv, m, a.n0, i, d);
+                if (post_fired) wv = stdp_facilitate(P, wv, c.rtr[q] - 1.0);
+                wv = stdp_depress(P, wv, vt - (post_fired ? 1.0 : 0.0));
+                P.w[k] = wv;
+                P.la[k] = step;
+                dep += (long long)__builtin_rint(__builtin_ldexp(wv, P.shift));
+              }
+              v = (T)__builtin_ldexp((double)dep, -P.shift);
+            } else if (dc.tail >= 0) {
+              v = tail[(dc.tail * u.Wm + w) * TS + tid];
+            } else {
+              continue;
+            }
+            if (dc.receptor == 0) acc0 = acc0 + v;
+            else acc1 = acc1 + v;
+          }
+        }
+        const bool spike = stepf(s, r, acc0, acc1, cur);
+        if (vmc >= 0) a.vm_buf[(long long)(step - a.vm_step0) * a.n_vm + vmc] = (double)s[0];
+        a.spk_hist[(long long)(step % a.Ls) * a.Ntot + u.gid] = spike ? 1 : 0;
+#pragma unroll
+        for (int q = 0; q < BS_MAX_PL; ++q)
+          if (q < c.nplpre) {
+            const StdpDev& P = a.stdp[pop.plpre[q]];
+            utr[q] = utr[q] * P.decay_plus + (spike ? 1.0 : 0.0);
+            P.U[(long long)((step + 1) % P.Lu) * P.n_pre + il] = utr[q];
+          }
+        if (spike) {
+          if (w + 1 < a.W) fmask |= 1u << (w + 1);
+          const int pp = atomicAdd(&u.spn[w], 1);
+          u.spl[w * TS + pp] = (unsigned short)tid;
+        }
+      }
+    }
+    if (c.npl > 0) __syncthreads();
+    wa = wb;
+  }
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < BS_NSTATE; ++k)
+      if (k < ns) st[(long long)k * a.Ntot + u.gid] = s[k];
+    a.ref[u.gid] = r;
+#pragma unroll
+    for (int p = 0; p < BS_MAX_PL; ++p)
+      if (p < c.npl) a.stdp[pop.pl[p]].V[il] = vtr[p];
+  }
+  if (c.npl == 0) return;
+  // facilitation of the incoming edges of the neurons that fired
+  if (tid < TS) c.fm[tid] = act ? fmask : 0u;
+  if (tid == 0) c.fl[TS] = 0;
+  __syncthreads();
+  if (tid < TS && c.fm[tid]) c.fl[atomicAdd(&c.fl[TS], 1)] = tid;
+  __syncthreads();
+  const int nl = c.fl[TS];
+  for (int p = 0; p < c.npl; ++p) {
+    const StdpDev& P = a.stdp[pop.pl[p]];
+    for (int li = 0; li < nl; ++li) {
+      const int lane = c.fl[li];
+      const unsigned fmk = c.fm[lane];
+      const int j = u.start + lane;
+      const unsigned c0 = P.csc_ptr[j], c1 = P.csc_ptr[j + 1];
+      for (unsigned cc = c0 + tid; cc < c1; cc += 256) {
+        const uint32_t k = P.csc_eid[cc];
+        const int la = P.la[k];
+        const int lo = la > a.n0 - 1 ? la : a.n0 - 1;
+        const unsigned long long m = (unsigned long long)fmk & ~((1ull << (lo - a.n0 + 1)) - 1ull);
+        if (!m) continue;
+        P.w[k] = stdp_pending(P, P.w[k], m, a.n0, P.e_pre[k], (int)(P.ew[k] >> 8));
+      }
+    }
+  }
+}
+
+template <typename T, bool HAS_CUR, typename F>
+__device__ __forceinline__ void run_update(const UpdCtx<T>& u, bool plastic, const PlCtx& pl, const DevPop& pop,
+                                           int ns, F stepf) {
+  if (plastic)
+    update_loop_pl<T, HAS_CUR>(u, pl, pop, ns, stepf);
+  else if (u.tid < u.TS)
+    update_loop<T, HAS_CUR>(u, ns, stepf);
+}
+
 // Producer blocks: draw the Poisson counts of steps n1 .. n1 + Wmax - 1 for
 // every drawn-ahead channel (the next window's inputs) while the tile blocks
 // run the current window on the other CUs. The counts are functions of
 // (seed, step, element) only, so they equal what the inputs phase would draw.
-__device__ void produce_poisson(const WinArgs& a, int n1, int blk, int nblk) {
+__device__ __forceinline__ void produce_poisson(const WinArgs& a, int n1, int blk, int nblk) {
   const long long per = a.pjob_n;
   const long long total = per * a.Wmax;
   const int L2 = 2 * a.Wmax;
@@ -481,7 +872,9 @@ __global__ __launch_bounds__(256) void k_pois_fill(WinArgs a) {
   produce_poisson(a, a.n0, (int)blockIdx.x, (int)gridDim.x);
 }
 
-template <typename T>
+// PLASTIC: the network has plastic projections (a separate instantiation, so the
+// static kernel carries none of the plastic path's registers)
+template <typename T, bool PLASTIC>
 __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   if (a.base_ptr) {
@@ -808,6 +1201,26 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
     }
   }
 
+  // plastic inputs or outputs: their LDS lies after the window layout
+  const bool plp = PLASTIC && (gpop.n_pl > 0 || gpop.n_plpre > 0);
+  PlCtx plc{};
+  if (plp) {
+    const PlLayout PLo = pl_layout(Lo.end, Wm, TS, gpop.n_pl, a.psp, gpop.n_tail, (int)sizeof(T), gpop.pl_cap);
+    plc.ppref = (int*)(lds + PLo.ppref);
+    plc.poff = (int*)(lds + PLo.poff);
+    plc.part = (int*)(lds + PLo.part);
+    plc.fm = (unsigned*)(lds + PLo.fm);
+    plc.fl = (int*)(lds + PLo.fl);
+    plc.tail = lds + PLo.tail;
+    plc.rk = (uint32_t*)(lds + PLo.rk);
+    plc.rtr = (double*)(lds + PLo.rtr);
+    plc.npl = gpop.n_pl;
+    plc.nplpre = gpop.n_plpre;
+    plc.cap = gpop.pl_cap;
+    plc.S = gpop.n_pl * TS;
+    plc.tile_in_pop = tile_in_pop;
+  }
+
   // ---- 2. inputs of every (step, lane), channels summed in order, all threads.
   // Channel-outer over the (up to four) items of each thread, so the items'
   // LDS reads of one channel overlap; every item still adds its channels in
@@ -836,6 +1249,7 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
       }
       for (int c = 0; c < nch; ++c) {
         const ChReg<T> cc = load_ch<T>(chs[c], tab, tables);
+        if (cc.plastic >= 0) continue;  // summed by the update loop, after its arrivals
         if (cc.kind == BS_CH_SCHEDULE) {  // block-uniform: one lookup per window step
           __syncthreads();
           if (tid < W) schv[tid] = sched_at(cc.sched, cc.sval, cc.cdf_len, (long long)a.n0 + tid);
@@ -876,7 +1290,14 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
             v[i] = ok[i] ? chan_value<T>(u, cc, w, a.n0 + w, l) : (T)0;
           }
         }
-        if (cc.kind == BS_CH_CURRENT || cc.kind == BS_CH_AC || cc.kind == BS_CH_SCHEDULE) {
+        if (cc.tail >= 0) {  // summed after a plastic channel: kept for the update loop
+          T* tw = (T*)plc.tail;
+#pragma unroll
+          for (int i = 0; i < kIT; ++i) {
+            const int idx = base0 + tid + 256 * i, w = idx >> lgTS, l = idx & (TS - 1);
+            if (ok[i]) tw[(cc.tail * Wm + w) * TS + l] = v[i];
+          }
+        } else if (cc.kind == BS_CH_CURRENT || cc.kind == BS_CH_AC || cc.kind == BS_CH_SCHEDULE) {
 #pragma unroll
           for (int i = 0; i < kIT; ++i) accC[i] = accC[i] + v[i];
         } else if (cc.receptor == 0) {
@@ -901,8 +1322,9 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   }
   __syncthreads();
 
-  // ---- 3. update: the W-step loop of each neuron
-  if (tid < TS) {
+  // ---- 3. update: the W-step loop of each neuron (populations with plastic
+  // projections: all threads, for the block-wide arrival lists)
+  if (plp || tid < TS) {
     const T* tables = (const T*)a.tables;
     UpdCtx<T> u{a, cnt, fxa, spl, spn, chs, tab, tables, TS, ncw, nfx, nch, tl.start, tid,
                 tid < tl.count, (long long)gpop.gid0 + tl.start + tid, inp, nrec, Wm, pc, tl.ncur, nullptr};
@@ -911,23 +1333,23 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
     switch (gpop.model) {                                                                          \
       case BS_IAF_PSC_DELTA: {                                                                     \
         const DeltaP<T> mp = load_delta<T>(P);                                                     \
-        update_loop<T, CUR>(u, gpop.n_state, [&](T* s, int& r, T a0, T, T cur) {               \
+        run_update<T, CUR>(u, plp, plc, gpop, gpop.n_state, [&](T* s, int& r, T a0, T, T cur) {               \
           return step_delta<T, CUR>(mp, s, r, a0, cur);                                            \
         });                                                                                        \
       } break;                                                                                     \
       case BS_IAF_PSC_EXP: {                                                                       \
         const ExpP<T> mp = load_exp<T>(P);                                                         \
-        update_loop<T, CUR>(u, gpop.n_state,                                                   \
+        run_update<T, CUR>(u, plp, plc, gpop, gpop.n_state,                                                   \
                             [&](T* s, int& r, T a0, T, T cur) { return step_exp<T>(mp, s, r, a0, cur); }); \
       } break;                                                                                     \
       case BS_IAF_PSC_ALPHA: {                                                                     \
         const AlphaP<T> mp = load_alpha<T>(P);                                                     \
-        update_loop<T, CUR>(u, gpop.n_state,                                                   \
+        run_update<T, CUR>(u, plp, plc, gpop, gpop.n_state,                                                   \
                             [&](T* s, int& r, T a0, T, T cur) { return step_alpha<T>(mp, s, r, a0, cur); }); \
       } break;                                                                                     \
       case BS_AEIF_COND_ALPHA: {                                                                   \
         if constexpr (std::is_same<T, double>::value) {                                            \
-          update_loop<T, CUR>(u, gpop.n_state, [&](T* s, int& r, T a0, T a1, T cur) {           \
+          run_update<T, CUR>(u, plp, plc, gpop, gpop.n_state, [&](T* s, int& r, T a0, T a1, T cur) {           \
             int unstable = 0;                                                                      \
             const bool sp = step_aeif(P, s, r, a0, a1, cur, &unstable);                            \
             if (unstable) atomicOr(a.flags, FLAG_UNSTABLE);                                        \
@@ -968,134 +1390,6 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   }
 }
 
-// ------------------------------------------------------------ STDP kernels
-// Plastic projection state (stdp_synapse.py:163-179 on event_plastic.py:454-536),
-// per-edge delays. At arrival step n, for edge e (pre i, post j, delay d):
-//   pre trace   tr_e(n) = U_i(n - d),  U_i(m) = U_i(m-1) P+ + s_i(m-1)
-//   post trace  V_j(n)  = V_j(n-1) P- + s_j(n-1)
-//   pre_fired   s_i(n - 1 - d), post_fired s_j(n - 1)
-//   w <- facilitate(w, tr - pre_fired) where post_fired, then
-//   w <- depress(w, V - post_fired) where pre_fired; the new w of arriving
-//   edges is delivered to j at step n.
-// Edges are stored grouped by (pre, delay) with row offsets per delay, and
-// indexed by post (CSC) for the facilitation pass.
-struct StdpDev {
-  int pre_pop, post_pop, pre_gid0, post_gid0, n_pre, n_post, d_min, d_max, Lu, shift;
-  const int* e_post;
-  const int* e_pre;
-  const int* e_dly;
-  const unsigned* roff;     // [n_pre][d_max - d_min + 2]
-  const unsigned* csc_ptr;  // [n_post + 1]
-  const unsigned* csc_eid;  // edge ids grouped by post
-  double* w;
-  double* U;                // [Lu][n_pre]
-  double* V;                // [n_post]
-  long long* pacc;          // [n_post] deposits for the current step
-  double decay_plus, decay_minus, lam, alpha, mup, mum, Wmax;
-};
-
-struct StdpArgs {
-  const StdpDev* P;
-  int n_proj;
-  const unsigned char* spk_hist;  // [Ls][Ntot]
-  const uint32_t* sb;
-  const uint32_t* sb_count;
-  long long Ntot;
-  int Ls, n_pops, n;
-  const long long* base_ptr;      // graph replays: n = *base_ptr + widx
-  int widx;
-};
-
-__device__ __forceinline__ double stdp_facilitate(const StdpDev& p, double w, double kplus) {
-  const double nw = w / p.Wmax + p.lam * pow_p(1.0 - w / p.Wmax, p.mup) * kplus;
-  return nw < 1.0 ? nw * p.Wmax : p.Wmax;
-}
-__device__ __forceinline__ double stdp_depress(const StdpDev& p, double w, double kminus) {
-  const double nw = w / p.Wmax - p.alpha * p.lam * pow_p(w / p.Wmax, p.mum) * kminus;
-  return nw > 0.0 ? nw * p.Wmax : 0.0;
-}
-
-__device__ __forceinline__ int stdp_step(const StdpArgs& a) {
-  return a.base_ptr ? (int)(*a.base_ptr) + a.widx : a.n;
-}
-
-// Trace recurrences of step n from the spikes of step n-1.
-__global__ __launch_bounds__(256) void k_stdp_traces(StdpArgs a) {
-  const int n = stdp_step(a);
-  const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  const int hs = ((n - 1) % a.Ls + a.Ls) % a.Ls;
-  for (int q = 0; q < a.n_proj; ++q) {
-    const StdpDev& p = a.P[q];
-    const int cur = n % p.Lu, prv = (n - 1 + p.Lu) % p.Lu;
-    for (long long i = t0; i < p.n_pre; i += stride) {
-      const double s = n >= 1 ? (double)a.spk_hist[(long long)hs * a.Ntot + p.pre_gid0 + i] : 0.0;
-      const double u = n >= 1 ? p.U[(long long)prv * p.n_pre + i] : 0.0;
-      p.U[(long long)cur * p.n_pre + i] = u * p.decay_plus + s;
-    }
-    for (long long j = t0; j < p.n_post; j += stride) {
-      const double s = n >= 1 ? (double)a.spk_hist[(long long)hs * a.Ntot + p.post_gid0 + j] : 0.0;
-      p.V[j] = p.V[j] * p.decay_minus + s;
-    }
-  }
-}
-
-// Facilitation of post-spike edges and depression + delivery of arriving edges.
-// Work items: for every projection, (a) each post spike of step n-1, then
-// (b) each pre spike of steps n-1-d_max .. n-1-d_min; one workgroup per item.
-__global__ __launch_bounds__(256) void k_stdp_events(StdpArgs a) {
-  const int n = stdp_step(a);
-  if (n < 1) return;
-  const int hs1 = (n - 1) % a.Ls;
-  long long item = blockIdx.x;
-  for (int q = 0; q < a.n_proj; ++q) {
-    const StdpDev& p = a.P[q];
-    // (a) post spikes of step n-1
-    const long long na = a.sb_count[hs1 * a.n_pops + p.post_pop];
-    for (; item < na; item += gridDim.x) {
-      const int j = (int)a.sb[(long long)hs1 * a.Ntot + p.post_gid0 + item];
-      for (unsigned c = p.csc_ptr[j] + threadIdx.x; c < p.csc_ptr[j + 1]; c += blockDim.x) {
-        const unsigned e = p.csc_eid[c];
-        const int d = p.e_dly[e];
-        const int i = p.e_pre[e];
-        const int ms = n - 1 - d;
-        const bool pre_fired =
-            ms >= 0 && a.spk_hist[(long long)(ms % a.Ls) * a.Ntot + p.pre_gid0 + i] != 0;
-        if (pre_fired) continue;  // handled with the arrival below, facilitation first
-        const int mu = n - d;
-        const double tr = mu >= 0 ? p.U[(long long)(mu % p.Lu) * p.n_pre + i] : 0.0;
-        p.w[e] = stdp_facilitate(p, p.w[e], tr - 0.0);
-      }
-    }
-    item -= na;
-    // (b) pre spikes whose edges of delay n-1-s arrive now
-    int s_lo = n - 1 - p.d_max;
-    if (s_lo < 0) s_lo = 0;
-    const int s_hi = n - 1 - p.d_min;
-    const int nd1 = p.d_max - p.d_min + 2;
-    for (int s = s_lo; s <= s_hi; ++s) {
-      const long long ns = a.sb_count[(s % a.Ls) * a.n_pops + p.pre_pop];
-      for (; item < ns; item += gridDim.x) {
-        const int i = (int)a.sb[(long long)(s % a.Ls) * a.Ntot + p.pre_gid0 + item];
-        const int d = n - 1 - s;
-        const unsigned* ro = p.roff + (long long)i * nd1 + (d - p.d_min);
-        const double tr = p.U[(long long)((n - d) % p.Lu) * p.n_pre + i];
-        for (unsigned e = ro[0] + threadIdx.x; e < ro[1]; e += blockDim.x) {
-          const int j = p.e_post[e];
-          const bool post_fired = a.spk_hist[(long long)hs1 * a.Ntot + p.post_gid0 + j] != 0;
-          double w = p.w[e];
-          if (post_fired) w = stdp_facilitate(p, w, tr - 1.0);
-          w = stdp_depress(p, w, p.V[j] - (post_fired ? 1.0 : 0.0));
-          p.w[e] = w;
-          atomicAdd((unsigned long long*)&p.pacc[j],
-                    (unsigned long long)(long long)__builtin_rint(__builtin_ldexp(w, p.shift)));
-        }
-      }
-      item -= ns;
-    }
-  }
-}
-
 // ============================================================== host side
 struct bs_net {
   int dtype = BS_F64;
@@ -1106,7 +1400,9 @@ struct bs_net {
   // layout
   long long Ntot = 0;
   int W = 1, d_min = 0, d_max = 0, Ls = 1, max_span = 1, TS = 256;
+  int psp = 1;            // plastic gather: source steps per projection and window, at most
   size_t lds_bytes = 0;
+  size_t lds_budget = 0;  // dynamic LDS the window kernel may use
   std::vector<DevPop> pops;
   std::vector<DevChan> chans;
   std::vector<Tile> tiles;
@@ -1622,8 +1918,18 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   net->chans.resize(d->n_channels);
   net->h_cdf.assign(d->n_channels, {});
   net->h_guide.assign(d->n_channels, {});
-  for (int c = 0; c < d->n_channels; ++c) memset(&net->chans[c], 0, sizeof(DevChan));
+  for (int c = 0; c < d->n_channels; ++c) {
+    memset(&net->chans[c], 0, sizeof(DevChan));
+    net->chans[c].plastic = -1;
+    net->chans[c].tail = -1;
+  }
   std::vector<int> chan_pop(d->n_channels, -1);
+  // channels fed by plastic projections take no fixed-point arrival slot:
+  // their sums are formed in the update loop
+  std::vector<char> chan_plastic(d->n_channels, 0);
+  for (int q = 0; q < d->n_projs; ++q)
+    if (d->projs[q].plastic && d->projs[q].channel >= 0 && d->projs[q].channel < d->n_channels)
+      chan_plastic[d->projs[q].channel] = 1;
   for (int p = 0; p < d->n_pops; ++p) {
     const bs_population& sp = d->pops[p];
     DevPop& dp = net->pops[p];
@@ -1668,7 +1974,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
         dc.half = ncount % 2;
         ++ncount;
       } else if (ch.kind == BS_CH_FIXED) {
-        dc.fx = nfx++;
+        dc.fx = chan_plastic[c] ? -1 : nfx++;
         dc.shift = ch.fixed_shift;
         if (ch.fixed_shift < 0 || ch.fixed_shift > 1000) BAD("bad fixed_shift");
       } else if (ch.kind == BS_CH_POISSON) {
@@ -1884,54 +2190,67 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
       if (!pj.weight) BAD("a plastic projection needs per-edge initial weights");
       if (!(pj.stdp.Wmax != 0.0) || !(pj.stdp.tau_plus > 0) || !(pj.stdp.tau_minus > 0))
         BAD("invalid stdp_synapse parameters");
+      if (pj.n_edges < 0 || pj.n_edges >= (1ll << 32)) BAD("projection edge count out of range");
+      DevPop& post_dp = net->pops[pj.post_pop];
+      DevPop& pre_dp = net->pops[pj.pre_pop];
+      if (post_dp.n_pl >= BS_MAX_PL || pre_dp.n_plpre >= BS_MAX_PL)
+        BAD("more than 4 plastic projections into or out of one population");
       const int npre = d->pops[pj.pre_pop].n, npost = d->pops[pj.post_pop].n;
+      const int nt = (npost + TS - 1) / TS;
       const long long E = pj.n_edges;
       int pdmin = 1 << 30, pdmax = 0;
+      // stored order: stable by (pre, post tile), the gather segments; reference order: stable by pre
+      std::vector<uint32_t> seg((size_t)npre * (nt + 1) + 1, 0);
       for (long long e = 0; e < E; ++e) {
         const int a = pj.pre[e], b = pj.post[e];
         if (a < 0 || a >= npre || b < 0 || b >= npost) BAD("edge index out of range");
         const int dd = pj.delay ? pj.delay[e] : pj.delay_steps;
         if (dd < 1) BAD("plastic projections need delays of at least one step");
+        if (dd >= (1 << 24)) BAD("delay must be in [0, 2^24) steps");
         pdmin = std::min(pdmin, dd);
         pdmax = std::max(pdmax, dd);
+        seg[(size_t)a * (nt + 1) + b / TS + 1]++;
         net->outdeg[net->pops[pj.pre_pop].gid0 + a]++;
       }
       if (E == 0) pdmin = pdmax = 1;
-      const int nd1 = pdmax - pdmin + 2;
-      // stored order: stable by (pre, delay); reference order: stable by pre
-      std::vector<long long> ord(E), ref(E);
-      for (long long e = 0; e < E; ++e) ord[e] = ref[e] = e;
-      auto dly = [&](long long e) { return pj.delay ? pj.delay[e] : pj.delay_steps; };
-      std::stable_sort(ord.begin(), ord.end(), [&](long long x, long long y) {
-        if (pj.pre[x] != pj.pre[y]) return pj.pre[x] < pj.pre[y];
-        return dly(x) < dly(y);
-      });
-      std::stable_sort(ref.begin(), ref.end(), [&](long long x, long long y) { return pj.pre[x] < pj.pre[y]; });
-      std::vector<long long> spos(E);
-      for (long long k = 0; k < E; ++k) spos[ord[k]] = k;
-      std::vector<long long> wref(E);
-      for (long long k = 0; k < E; ++k) wref[k] = spos[ref[k]];
-      std::vector<int> epost(E), epre(E), edly(E);
-      std::vector<double> w(E);
-      std::vector<unsigned> roff((size_t)npre * nd1, 0), cptr(npost + 1, 0), ceid(E);
-      for (long long k = 0; k < E; ++k) {
-        const long long e = ord[k];
-        epost[k] = pj.post[e];
-        epre[k] = pj.pre[e];
-        edly[k] = dly(e);
-        w[k] = pj.weight[e];
-        roff[(size_t)epre[k] * nd1 + (edly[k] - pdmin) + 1]++;
-        cptr[epost[k] + 1]++;
-      }
-      {  // offsets: row i, delay slot t -> first edge; slot nd1-1 is the row end
-        unsigned acc = 0;
-        for (int a = 0; a < npre; ++a)
-          for (int t = 0; t < nd1; ++t) {
-            const unsigned c = t + 1 < nd1 ? roff[(size_t)a * nd1 + t + 1] : 0;
-            roff[(size_t)a * nd1 + t] = acc;
+      {  // exclusive prefix over (pre, tile); entry (a, nt) is the row end
+        uint32_t acc = 0;
+        for (int a = 0; a < npre; ++a) {
+          for (int t = 0; t < nt; ++t) {
+            const uint32_t c = seg[(size_t)a * (nt + 1) + t + 1];
+            seg[(size_t)a * (nt + 1) + t] = acc;
             acc += c;
           }
-        for (int b = 0; b < npost; ++b) cptr[b + 1] += cptr[b];
+          seg[(size_t)a * (nt + 1) + nt] = acc;
+        }
+        seg.resize((size_t)npre * (nt + 1));
+      }
+      std::vector<long long> spos(E);
+      std::vector<uint32_t> ew(E);
+      std::vector<int> epre(E), epost(E);
+      std::vector<double> w(E);
+      {
+        std::vector<uint32_t> cur(seg);
+        for (long long e = 0; e < E; ++e) {
+          const int a = pj.pre[e], b = pj.post[e];
+          const int dd = pj.delay ? pj.delay[e] : pj.delay_steps;
+          const uint32_t k = cur[(size_t)a * (nt + 1) + b / TS]++;
+          spos[e] = k;
+          ew[k] = ((uint32_t)dd << 8) | (uint32_t)(b % TS);
+          epre[k] = a;
+          epost[k] = b;
+          w[k] = pj.weight[e];
+        }
+      }
+      std::vector<long long> ref(E);
+      for (long long e = 0; e < E; ++e) ref[e] = e;
+      std::stable_sort(ref.begin(), ref.end(), [&](long long x, long long y) { return pj.pre[x] < pj.pre[y]; });
+      std::vector<long long> wref(E);
+      for (long long k = 0; k < E; ++k) wref[k] = spos[ref[k]];
+      std::vector<unsigned> cptr(npost + 1, 0), ceid(E);
+      for (long long k = 0; k < E; ++k) cptr[epost[k] + 1]++;
+      for (int b = 0; b < npost; ++b) cptr[b + 1] += cptr[b];
+      {
         std::vector<unsigned> pos(cptr.begin(), cptr.end() - 1);
         for (long long k = 0; k < E; ++k) ceid[pos[epost[k]]++] = (unsigned)k;
       }
@@ -1945,7 +2264,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
       S.n_post = npost;
       S.d_min = pdmin;
       S.d_max = pdmax;
-      S.Lu = pdmax + 2;
+      S.nt = nt;
       S.shift = d->channels[pj.channel].fixed_shift;
       S.decay_plus = pj.stdp.decay_plus;
       S.decay_minus = pj.stdp.decay_minus;
@@ -1954,35 +2273,38 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
       S.mup = pj.stdp.mu_plus;
       S.mum = pj.stdp.mu_minus;
       S.Wmax = pj.stdp.Wmax;
-      int *dpost, *dpre, *ddly;
-      unsigned *droff, *dcptr, *dceid;
-      double *dw, *dU, *dV;
-      long long* dpacc;
-      CHK(upload(net, &dpost, epost.data(), epost.size()));
+      uint32_t *dseg, *dew;
+      int *dpre, *dla;
+      unsigned *dcptr, *dceid;
+      double *dw, *dV;
+      ew.resize(E + 1, 0u);  // no zero-length device arrays
+      epre.resize(E + 1, 0);
+      ceid.resize(E + 1, 0u);
+      w.resize(E + 1, 0.0);
+      CHK(upload(net, &dseg, seg.data(), seg.size()));
+      CHK(upload(net, &dew, ew.data(), ew.size()));
       CHK(upload(net, &dpre, epre.data(), epre.size()));
-      CHK(upload(net, &ddly, edly.data(), edly.size()));
-      CHK(upload(net, &droff, roff.data(), roff.size()));
       CHK(upload(net, &dcptr, cptr.data(), cptr.size()));
       CHK(upload(net, &dceid, ceid.data(), ceid.size()));
       CHK(upload(net, &dw, w.data(), w.size()));
-      CHK(dmalloc(&dU, (size_t)S.Lu * npre));
-      net->owned.push_back(dU);
-      CHK(dmalloc(&dV, (size_t)npost));
+      w.resize(E);
+      CHK(dmalloc(&dla, (size_t)E + 1));
+      net->owned.push_back(dla);
+      CHK(dmalloc(&dV, (size_t)std::max(1, npost)));
       net->owned.push_back(dV);
-      CHK(dmalloc(&dpacc, (size_t)npost));
-      net->owned.push_back(dpacc);
-      S.e_post = dpost;
+      S.seg = dseg;
+      S.ew = dew;
       S.e_pre = dpre;
-      S.e_dly = ddly;
-      S.roff = droff;
       S.csc_ptr = dcptr;
       S.csc_eid = dceid;
       S.w = dw;
-      S.U = dU;
-      S.V = dV;
-      S.pacc = dpacc;
-      net->chans[pj.channel].pacc = dpacc;
-      net->proj_stdp.back() = (int)net->stdp.size();
+      S.la = dla;
+      S.V = dV;  // U: allocated once the window length (and with it Lu) is known
+      const int qs = (int)net->stdp.size();
+      net->chans[pj.channel].plastic = post_dp.n_pl;
+      post_dp.pl[post_dp.n_pl++] = qs;
+      pre_dp.plpre[pre_dp.n_plpre++] = qs;
+      net->proj_stdp.back() = qs;
       net->stdp.push_back(S);
       net->w0.push_back(w);
       net->w_ref.push_back(wref);
@@ -2079,10 +2401,41 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
 
   // ---- window length: W = d_min + 1 (holder lag) capped by the LDS budget
   int W = dmin + 1;
-  if (!net->stdp.empty()) W = 1;  // plastic updates interleave with every step
   const int maxw = d->max_window > 0 ? d->max_window : 32;
   if (W > maxw) W = maxw;
   if (W < 1) W = 1;
+  // tail channels: static delta channels summed after a plastic channel of
+  // the same receptor wait in LDS for the update loop
+  for (size_t q = 0; q < net->pops.size(); ++q) {
+    DevPop& p = net->pops[q];
+    for (int rc = 0; rc < 2; ++rc) {
+      bool seen = false;
+      for (int c = p.ch_begin; c < p.ch_begin + p.ch_count; ++c) {
+        DevChan& dc = net->chans[c];
+        if (dc.receptor != rc) continue;
+        if (dc.plastic >= 0) seen = true;
+        else if (seen && (dc.kind == BS_CH_COUNT || dc.kind == BS_CH_FIXED || dc.kind == BS_CH_POISSON))
+          dc.tail = p.n_tail++;
+      }
+    }
+  }
+  // LDS budget: 64 KB (two blocks per CU), or with plastic projections most
+  // of a CU's LDS, for the window's arrival records
+  size_t lds_max = kLdsBudget;  // the kernels' dynamic-LDS attribute (one value for every network)
+  {
+    int mx = 0;
+    HCHK(hipDeviceGetAttribute(&mx, hipDeviceAttributeMaxSharedMemoryPerBlock, net->device));
+    lds_max = std::max<size_t>(kLdsBudget, std::min<size_t>(mx > 4096 ? (size_t)mx - 4096 : 0, 150 * 1024));
+  }
+  const size_t budget = net->stdp.empty() ? kLdsBudget : lds_max;
+  net->lds_budget = lds_max;
+  int pl_span = 0;  // source steps of a plastic gather, beyond the window's W
+  for (const StdpDev& S : net->stdp) pl_span = std::max(pl_span, S.d_max - S.d_min);
+  const int kPlMin = 512;  // arrival records a sub-window must be able to hold
+  auto pl_end = [&](const DevPop& p, int w, size_t base, int cap) {
+    if (p.n_pl == 0 && p.n_plpre == 0) return base;
+    return pl_layout(base, w, TS, p.n_pl, pl_span + w, p.n_tail, (int)net->tsize, p.n_pl ? cap : 0).end;
+  };
   // LDS layout of k_window (win_layout), first without staged tables
   // each window gathers the spikes of the previous one (at most W steps)
   auto span_of = [&](int w) { return w; };
@@ -2096,27 +2449,29 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
     const int SP = span_of(w) + 1;
     for (size_t q = 0; q < net->pops.size(); ++q) {
       const DevPop& p = net->pops[q];
-      best = std::max(best, win_layout(w, hz(w), TS, p.n_cw, p.n_fx, p.gp_count, SP, p.ch_count,
+      best = std::max(best, pl_end(p, w, win_layout(w, hz(w), TS, p.n_cw, p.n_fx, p.gp_count, SP, p.ch_count,
                                        (p.model == BS_AEIF_COND_ALPHA ? 2 : 1) + (p.n_cur > 0 ? 1 : 0), (int)net->tsize, 0,
-                                       net->pop_pc[q]).end);
+                                       net->pop_pc[q]).end, kPlMin));
     }
     return best;
   };
-  if (lds_of(W) > kLdsBudget) full_h = false;
-  while (W > 1 && lds_of(W) > kLdsBudget) --W;
-  if (lds_of(W) > kLdsBudget) BAD("too many input channels per population for the LDS budget");
+  if (lds_of(W) > budget) full_h = false;
+  while (W > 1 && lds_of(W) > budget) --W;
+  if (lds_of(W) > budget) BAD("too many input channels per population for the LDS budget");
   net->W = W;
   net->H = hz(W);
   net->max_span = span_of(W);
-  // stage count tables and Poisson CDFs into LDS while they fit
+  net->psp = pl_span + W;
+  // stage count tables and Poisson CDFs into LDS while they fit (leaving a
+  // plastic population room for its minimum of arrival records)
   {
     const int SP = net->max_span + 1;
     size_t need = 0;
     for (size_t q = 0; q < net->pops.size(); ++q) {
       DevPop& p = net->pops[q];
-      const size_t base_end = win_layout(W, net->H, TS, p.n_cw, p.n_fx, p.gp_count, SP, p.ch_count,
+      const size_t base_end = pl_end(p, W, win_layout(W, net->H, TS, p.n_cw, p.n_fx, p.gp_count, SP, p.ch_count,
                                          (p.model == BS_AEIF_COND_ALPHA ? 2 : 1) + (p.n_cur > 0 ? 1 : 0), (int)net->tsize, 0,
-                                         net->pop_pc[q]).end;
+                                         net->pop_pc[q]).end, kPlMin);
       size_t off = 0;
       for (int c = p.ch_begin; c < p.ch_begin + p.ch_count; ++c) {
         DevChan& dc = net->chans[c];
@@ -2129,17 +2484,32 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
           sz = al8((size_t)dc.lds_len * net->tsize);
         }
         else if (dc.kind == BS_CH_POISSON) sz = al8((size_t)dc.cdf_len * 8 + 64 * 2);
-        if (sz && base_end + off + sz + 16 <= kLdsBudget) {
+        if (sz && base_end + off + sz + 16 <= budget) {
           dc.lds_off = (long long)off;
           off += sz;
         }
       }
       p.tab_bytes = (long long)off;
-      need = std::max(need, win_layout(W, net->H, TS, p.n_cw, p.n_fx, p.gp_count, SP, p.ch_count,
-                                       (p.model == BS_AEIF_COND_ALPHA ? 2 : 1) + (p.n_cur > 0 ? 1 : 0), (int)net->tsize,
-                                       (size_t)off, net->pop_pc[q]).end);
+      const size_t wend = win_layout(W, net->H, TS, p.n_cw, p.n_fx, p.gp_count, SP, p.ch_count,
+                                     (p.model == BS_AEIF_COND_ALPHA ? 2 : 1) + (p.n_cur > 0 ? 1 : 0), (int)net->tsize,
+                                     (size_t)off, net->pop_pc[q]).end;
+      if (p.n_pl > 0) {  // the rest of the budget holds arrival records
+        const size_t e0 = pl_end(p, W, wend, 0);
+        const long long room = ((long long)budget - (long long)e0 - 64) / 12;
+        p.pl_cap = (int)std::max<long long>(kPlMin, std::min<long long>(room, 16384));
+        if (d->plastic_records > 0) p.pl_cap = std::min(p.pl_cap, (int)d->plastic_records);
+      }
+      need = std::max(need, pl_end(p, W, wend, p.pl_cap));
     }
+    if (need > budget) BAD("LDS layout exceeds the budget");
     net->lds_bytes = need;
+  }
+  for (StdpDev& S : net->stdp) {  // pre-trace rings: U_i(m) of m in [n0 - d_max, n0 + W]
+    S.Lu = S.d_max + W + 2;
+    double* dU = nullptr;
+    CHK(dmalloc(&dU, (size_t)S.Lu * std::max(1, S.n_pre)));
+    net->owned.push_back(dU);
+    S.U = dU;
   }
   net->Ls = dmax + 2 * W + 2;
 
@@ -2387,10 +2757,14 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   CHK(dmalloc(&net->d_tap_col, std::max<size_t>(1, net->vm_tap_col.size())));
   if (!net->vm_tap_col.empty())
     HCHK(hipMemcpy(net->d_tap_col, net->vm_tap_col.data(), 4 * net->vm_tap_col.size(), hipMemcpyHostToDevice));
-  HCHK(hipFuncSetAttribute((const void*)k_window<double>,
-                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget));
-  HCHK(hipFuncSetAttribute((const void*)k_window<float>,
-                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget));
+  HCHK(hipFuncSetAttribute((const void*)k_window<double, false>,
+                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)net->lds_budget));
+  HCHK(hipFuncSetAttribute((const void*)k_window<float, false>,
+                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)net->lds_budget));
+  HCHK(hipFuncSetAttribute((const void*)k_window<double, true>,
+                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)net->lds_budget));
+  HCHK(hipFuncSetAttribute((const void*)k_window<float, true>,
+                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)net->lds_budget));
   {
     int rc = upload_state(net);
     if (rc != BS_OK) {
@@ -2414,11 +2788,11 @@ static int upload_state(bs_net* net) {
   if (gid) HIPCHK(hipMemcpy(net->d_ref, net->ref0.data(), 4 * gid, hipMemcpyHostToDevice));
   HIPCHK(hipMemset(net->d_sb_count, 0, 4 * (size_t)net->Ls * net->pops.size()));
   HIPCHK(hipMemset(net->d_flags, 0, sizeof(int)));
-  for (size_t q = 0; q < net->stdp.size(); ++q) {  // traces 0, initial weights
+  for (size_t q = 0; q < net->stdp.size(); ++q) {  // traces 0, no arrivals yet, initial weights
     const StdpDev& S = net->stdp[q];
     HIPCHK(hipMemset(S.U, 0, 8 * (size_t)S.Lu * std::max(1, S.n_pre)));
     HIPCHK(hipMemset(S.V, 0, 8 * (size_t)std::max(1, S.n_post)));
-    HIPCHK(hipMemset(S.pacc, 0, 8 * (size_t)std::max(1, S.n_post)));
+    HIPCHK(hipMemset(S.la, 0xFF, 4 * (net->w0[q].size() + 1)));
     if (!net->w0[q].empty())
       HIPCHK(hipMemcpy(S.w, net->w0[q].data(), 8 * net->w0[q].size(), hipMemcpyHostToDevice));
   }
@@ -2460,6 +2834,8 @@ static void launch_window(bs_net* net, int n0, int W, long long vm_step0, int pr
   a.base_ptr = base_ptr;
   a.widx = widx;
   a.spk_hist = net->d_spk_hist;
+  a.stdp = net->d_stdp;
+  a.psp = net->psp;
   a.tiles = net->d_tiles;
   a.blob = net->d_blob;
   a.pops = net->d_pops;
@@ -2492,8 +2868,12 @@ static void launch_window(bs_net* net, int n0, int W, long long vm_step0, int pr
   a.pjobs = net->d_pjobs;
   a.n_pjobs = (int)net->pjobs.size();
   a.pjob_n = net->pjob_n;
-  hipLaunchKernelGGL(k_window<T>, dim3((unsigned)(net->tiles.size() + net->n_prod)), dim3(256),
-                     net->lds_bytes, net->stream, a);
+  if (net->stdp.empty())
+    hipLaunchKernelGGL((k_window<T, false>), dim3((unsigned)(net->tiles.size() + net->n_prod)), dim3(256),
+                       net->lds_bytes, net->stream, a);
+  else
+    hipLaunchKernelGGL((k_window<T, true>), dim3((unsigned)(net->tiles.size() + net->n_prod)), dim3(256),
+                       net->lds_bytes, net->stream, a);
 }
 
 // Poisson counts of the window starting at n0 (before a run's first window).
@@ -2509,26 +2889,6 @@ static void launch_pois_fill(bs_net* net, int n0) {
   a.n_pjobs = (int)net->pjobs.size();
   a.pjob_n = net->pjob_n;
   hipLaunchKernelGGL(k_pois_fill, dim3((unsigned)net->n_prod), dim3(256), 0, net->stream, a);
-}
-
-static void launch_stdp(bs_net* net, int n, const long long* base_ptr, int widx) {
-  StdpArgs a;
-  a.P = net->d_stdp;
-  a.n_proj = (int)net->stdp.size();
-  a.spk_hist = net->d_spk_hist;
-  a.sb = net->d_sb;
-  a.sb_count = net->d_sb_count;
-  a.Ntot = net->Ntot;
-  a.Ls = net->Ls;
-  a.n_pops = (int)net->pops.size();
-  a.n = n;
-  a.base_ptr = base_ptr;
-  a.widx = widx;
-  long long nmax = 1;
-  for (const StdpDev& S : net->stdp) nmax = std::max<long long>(nmax, std::max(S.n_pre, S.n_post));
-  const unsigned tb = (unsigned)std::min<long long>((nmax + 255) / 256, 1024);
-  hipLaunchKernelGGL(k_stdp_traces, dim3(tb), dim3(256), 0, net->stream, a);
-  hipLaunchKernelGGL(k_stdp_events, dim3(512), dim3(256), 0, net->stream, a);
 }
 
 // One step's weights of a plastic projection in the reference's CSR order.
@@ -2560,7 +2920,6 @@ static int chunk_graph(bs_net* net, int par, int nwin) {
   hipGraph_t g;
   HIPCHK(hipStreamBeginCapture(net->stream, hipStreamCaptureModeThreadLocal));
   for (int w = 0; w < nwin; ++w) {
-    if (!net->stdp.empty()) launch_stdp(net, 0, net->d_base, w);
     if (net->tsize == 8)
       launch_window<double>(net, 0, net->W, 0, net->W, net->d_base, w);
     else
@@ -2607,17 +2966,17 @@ static int enqueue_chunk(bs_net* net, const ChunkJob& j, bool pin_vm, std::vecto
     net->n_update += j.nwin;
     net->b_update += win_bytes * j.nwin + step_bytes * j.chunk;
   } else {
+    const int Wr = net->wtaps.empty() ? net->W : 1;  // record_weight taps: one step per launch
     for (int w = 0; w < j.nwin; ++w) {
-      const long long n0 = j.c0 + (long long)w * net->W;
-      const int Wc = (int)std::min<long long>(net->W, j.c0 + j.chunk - n0);
+      const long long n0 = j.c0 + (long long)w * Wr;
+      const int Wc = (int)std::min<long long>(Wr, j.c0 + j.chunk - n0);
       if (net->timing) HIPCHK(hipEventRecord(evs[2 * w], net->stream));
-      if (!net->stdp.empty()) launch_stdp(net, (int)n0, nullptr, 0);
       if (net->tsize == 8)
         launch_window<double>(net, (int)n0, Wc, j.c0, net->last_win);
       else
         launch_window<float>(net, (int)n0, Wc, j.c0, net->last_win);
       net->last_win = Wc;
-      for (const bs_net::WTap& t : net->wtaps)  // record_weight: plastic networks step one at a time
+      for (const bs_net::WTap& t : net->wtaps)  // the weights after this step
         if (t.E > 0)
           hipLaunchKernelGGL(k_weight_tap, dim3((unsigned)std::min<long long>(1024, (t.E + 255) / 256)), dim3(256), 0,
                              net->stream, (const double*)net->stdp[t.q].w, (const long long*)t.d_ref, t.E,
@@ -2648,6 +3007,8 @@ static int drain_chunk(bs_net* net, const ChunkJob& j, bool pin_vm, double* vm_o
   if (flags & FLAG_OVERFLOW) return fail(BS_ECAPACITY, "spike list overflow");
   if (flags & FLAG_COUNT) return fail(BS_ECAPACITY, "arrival event count exceeded the channel's in-degree");
   if (flags & FLAG_UNSTABLE) return fail(BS_ENUMERIC, "Numerical instability in aeif_cond_alpha dynamics.");
+  if (flags & FLAG_PLASTIC)
+    return fail(BS_ECAPACITY, "plastic arrivals of one step exceeded the window kernel's LDS record capacity");
   if (net->timing) {
     for (int w = 0; w < j.nwin; ++w) {
       float a = 0;
@@ -2834,7 +3195,8 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
       cur.chunk = std::min<long long>(chunk_max, n_steps - done);
       cur.done = done;
       cur.par = par;
-      cur.nwin = (int)((cur.chunk + net->W - 1) / net->W);
+      const int Wr = net->wtaps.empty() ? net->W : 1;
+      cur.nwin = (int)((cur.chunk + Wr - 1) / Wr);
       rc = enqueue_chunk(net, cur, pin_vm, evs, win_bytes, step_bytes);
       if (rc != BS_OK) break;
       queued = true;
@@ -2905,7 +3267,7 @@ __global__ void k_vm_taps(const double* __restrict__ vbuf, int n_vm_dev, const i
 }
 __global__ void k_run_status(const int* __restrict__ flags, int32_t* __restrict__ status) {
   const int f = flags[0];
-  status[0] = (f & (FLAG_OVERFLOW | FLAG_COUNT)) ? BS_ECAPACITY : ((f & FLAG_UNSTABLE) ? BS_ENUMERIC : BS_OK);
+  status[0] = (f & (FLAG_OVERFLOW | FLAG_COUNT | FLAG_PLASTIC)) ? BS_ECAPACITY : ((f & FLAG_UNSTABLE) ? BS_ENUMERIC : BS_OK);
 }
 
 int bs_net_run_device(bs_net* net, int64_t n_steps, void* stream, uint32_t* raster, double* vm, int32_t* status) {
@@ -2950,7 +3312,6 @@ int bs_net_run_device(bs_net* net, int64_t n_steps, void* stream, uint32_t* rast
       for (int w = 0; w < nwin; ++w) {
         const long long n0 = c0 + (long long)w * net->W;
         const int Wc = (int)std::min<long long>(net->W, c0 + chunk - n0);
-        if (!net->stdp.empty()) launch_stdp(net, (int)n0, nullptr, 0);
         if (net->tsize == 8)
           launch_window<double>(net, (int)n0, Wc, c0, net->last_win);
         else
@@ -3005,7 +3366,7 @@ std::vector<Part> state_parts(bs_net* net) {
     v.push_back({S.w, 8 * net->w0[q].size()});
     v.push_back({S.U, 8 * (size_t)S.Lu * S.n_pre});
     v.push_back({S.V, 8 * (size_t)S.n_post});
-    v.push_back({S.pacc, 8 * (size_t)S.n_post});
+    v.push_back({S.la, 4 * net->w0[q].size()});
   }
   return v;
 }

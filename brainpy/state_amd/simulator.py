@@ -283,10 +283,15 @@ class Simulator:
         edges: the host's NumPy streams, or the engine's counter-based device
         samplers, whose edges stay in device memory through engine creation.
         The two give different (equally distributed) networks for one seed.
+    plastic_records : int
+        Plastic arrivals one block of the window kernel holds at once (0 =
+        what its LDS leaves). A window with more arrivals runs in
+        sub-windows; results do not depend on it.
     """
     __module__ = 'brainpy.state_amd'
 
-    def __init__(self, *, dt, dtype='float64', max_window=0, tile_size=0, connectivity='host'):
+    def __init__(self, *, dt, dtype='float64', max_window=0, tile_size=0, connectivity='host',
+                 plastic_records=0):
         self._dt_q = dt
         self.dt = u.scalar(dt, 'time')
         if self.dt <= 0:
@@ -301,6 +306,9 @@ class Simulator:
         if int(tile_size) not in (0, 32, 64, 128, 256):
             raise ValueError('tile_size must be 0, 32, 64, 128 or 256')
         self.tile_size = int(tile_size)
+        if int(plastic_records) < 0:
+            raise ValueError('plastic_records must be >= 0')
+        self.plastic_records = int(plastic_records)
         if connectivity not in ('host', 'device'):
             raise ValueError(f"connectivity must be 'host' or 'device', got {connectivity!r}")
         self.connectivity = connectivity
@@ -1032,6 +1040,7 @@ class _NetBuilder:
         d.n_projs = len(projs)
         d.max_window = sim.max_window
         d.tile_size = sim.tile_size
+        d.plastic_records = getattr(sim, 'plastic_records', 0)
         d.pops = cpops
         d.channels = chans
         d.projs = cprojs

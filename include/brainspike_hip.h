@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define BS_ABI_VERSION 8
+#define BS_ABI_VERSION 9
 
 /* ---- return codes ------------------------------------------------------ */
 #define BS_OK 0
@@ -177,7 +177,10 @@ typedef struct bs_net_desc {
   int32_t tile_size;         /* post neurons per block: 32, 64, 128 or 256; 0 =
                                 the largest giving >= 1024 tiles, and 32 when
                                 even 64 gives fewer tiles than the GPU has CUs   */
-  int32_t _pad2;
+  int32_t plastic_records;   /* plastic arrivals a block holds at once (0 =
+                                what the LDS leaves, up to 16384); a window
+                                with more runs in sub-windows. No counterpart
+                                in the reference (a capacity, not a result)     */
 } bs_net_desc;
 
 typedef struct bs_net bs_net;
@@ -222,7 +225,7 @@ int bs_net_set_state(bs_net* net, int32_t pop, int32_t var, const double* in);
  * BPTT restart from (state0, n0) (bptt.py:536-571): a snapshot of everything
  * the next step depends on -- neuron state and refractory counters, the
  * spike buckets of the delay horizon, the arrival rings, spike history and
- * plastic traces, weights and deposits, and the step counter (the position
+ * plastic traces, weights and last-arrival steps, and the step counter (the position
  * of every counter-based random stream). Restoring it into a handle created
  * from the same descriptor continues the run bit for bit. The snapshot is an
  * opaque host byte string of bs_net_state_bytes() bytes; a snapshot of a

@@ -95,6 +95,37 @@ def test_stdp_chunks_and_continue():
                                orc.weights(desc.proj_index[id(proj)], proj.n_edges), rtol=1e-12)
 
 
+def test_stdp_runs_in_windows_of_d_min_plus_one_steps():
+    """Plastic projections no longer shorten the window: with a 1 ms minimum
+    delay every launch advances d_min + 1 = 11 steps."""
+    sim, _ = _plastic_net(n=200, delay=1.0)
+    assert sim._ensure_engine().window() == 11
+
+
+@pytest.mark.parametrize('records', [64, 128])
+def test_stdp_sub_windows_equal_the_oracle(records):
+    """A block holding only ``records`` plastic arrivals at once runs a busier
+    window in sub-windows (each one's arrival lists gathered in turn); the
+    raster, V_m and weights still equal the oracle's step-by-step run. (This
+    network's 32-neuron tiles see up to 47 plastic arrivals in one step and
+    about 125 per 11-step window.)"""
+    sim, proj = _plastic_net(n=300)
+    sim.plastic_records = records
+    _compare(sim, proj, 1500)
+
+
+def test_stdp_record_capacity_error_is_reported():
+    """More plastic arrivals in one step than a block holds is an error the
+    run reports (not a silent drop)."""
+    from brainpy.state_amd._lib import EngineError
+    sim, _ = _plastic_net(n=300)
+    sim.plastic_records = 1
+    eng = sim._ensure_engine()
+    eng.reset()
+    with pytest.raises(EngineError, match='record capacity'):
+        eng.run(1500)
+
+
 @pytest.mark.parametrize('model,unit', [(iaf_psc_exp, 'pA'), (aeif_cond_alpha, 'nS')])
 def test_record_weight_trajectory_equals_the_oracle_step_by_step(model, unit):
     """record_weight (simulator.py:884-910): the (n_steps, n_edges) weight

@@ -21,7 +21,7 @@ def test_no_kernel_uses_scratch():
         for r in ex.map(B.resource_usage, B.SRC):
             usage.update(r)
     names = ' '.join(usage)
-    for k in ('k_window', 'k_stdp_events', 'k_stdp_traces', 'k_pois_fill', 'k_bptt_fwd', 'k_bptt_bwd',
+    for k in ('k_window', 'k_pois_fill', 'k_bptt_fwd', 'k_bptt_bwd',
               'k_bptt_wgrad'):
         assert k in names, k
     bad = {k: v['ScratchSize'] for k, v in usage.items() if v.get('ScratchSize', 0) != 0}
