@@ -18,7 +18,8 @@ The JSON line carries:
                includes sorting and copying the spike records to the host)
   rtf          wall time / simulated time
   roofline     the window kernel's algorithmic bytes per launch (delivery:
-               8 B per gathered spike + 4 B per edge; update: state and
+               8 B per gathered spike + one edge word per edge, 2 B (4 B when
+               a network's delay * stride + lane needs 32 bits); update: state and
                refractory counter read and written once per window; 8 B per
                spike record and V_m sample) over its mean HIP-event duration,
                on a steady-state window after the timed ones; ``traffic`` is

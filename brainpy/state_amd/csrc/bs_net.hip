@@ -73,6 +73,10 @@ using namespace bs;
 #ifndef BS_GATHER_CHUNKS
 #define BS_GATHER_CHUNKS 3
 #endif
+// 16-bit edge words: eight per chunk, so fewer chunks cover a segment
+#ifndef BS_GATHER_CHUNKS16
+#define BS_GATHER_CHUNKS16 2
+#endif
 
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_err;
@@ -158,6 +162,7 @@ static_assert(sizeof(Tile) % 16 == 0, "tile records are read as 16-byte words");
 struct GProj {
   int pre_pop, chan, d_min, d_max, nt;
   int rs;  // step stride of the edge words (LDS counter rows of the post population)
+  long long ne;  // edges
   const uint32_t* seg;
   const uint32_t* edges;
   const long long* q;  // FIXED channels: 2^F fixed-point weights per edge
@@ -873,8 +878,9 @@ __global__ __launch_bounds__(256) void k_pois_fill(WinArgs a) {
 }
 
 // PLASTIC: the network has plastic projections (a separate instantiation, so the
-// static kernel carries none of the plastic path's registers)
-template <typename T, bool PLASTIC>
+// static kernel carries none of the plastic path's registers). NARROW: the
+// static edge words are 16-bit (every projection's delay * stride + lane fits).
+template <typename T, bool PLASTIC, bool NARROW>
 __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   if (a.base_ptr) {
@@ -1081,7 +1087,8 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
 
   // ---- 1. gather: one spike per thread per pass, loads in phases; a
   // segment's edge words come in as aligned 16-byte chunks, three per round
-  constexpr int kQ = BS_GATHER_SPIKES, kCH = BS_GATHER_CHUNKS;
+  constexpr int kQ = BS_GATHER_SPIKES, kCH = NARROW ? BS_GATHER_CHUNKS16 : BS_GATHER_CHUNKS;
+  constexpr uint32_t kEPC = NARROW ? 8u : 4u;  // edge words per 16-byte chunk
   const int total = gtot[ngp];
   for (int b0 = 0; b0 < total; b0 += kQ * 256) {
     uint32_t si[kQ], e0[kQ], e1[kQ];
@@ -1137,8 +1144,8 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
       // arrival offset dn + d is below the horizon H, i.e. off < (H - dn) RS
       baseq[q] = dn[q] * rsq[q] + (kc[q] ? cwq[q] : fxq[q]) * TS;
       limq[q] = (uint32_t)(H - dn[q]) * (uint32_t)rsq[q];
-      a0[q] = e0[q] & ~3u;
-      nchk[q] = e1[q] > e0[q] ? (e1[q] - a0[q] + 3u) >> 2 : 0u;
+      a0[q] = e0[q] & ~(kEPC - 1u);
+      nchk[q] = e1[q] > e0[q] ? (e1[q] - a0[q] + kEPC - 1u) / kEPC : 0u;
       rounds = max(rounds, (int)((nchk[q] + kCH - 1) / kCH));
     }
     for (int rd = 0; rd < rounds; ++rd) {
@@ -1148,16 +1155,30 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
 #pragma unroll
         for (int c = 0; c < kCH; ++c) {
           const uint32_t ck = (uint32_t)(rd * kCH + c);
-          v[q][c] = ck < nchk[q] ? ep[q][(a0[q] >> 2) + ck] : make_uint4(0u, 0u, 0u, 0u);
+          v[q][c] = ck < nchk[q] ? ep[q][a0[q] / kEPC + ck] : make_uint4(0u, 0u, 0u, 0u);
         }
 #pragma unroll
       for (int q = 0; q < kQ; ++q)
 #pragma unroll
         for (int c = 0; c < kCH; ++c) {
-          const uint32_t wv[4] = {v[q][c].x, v[q][c].y, v[q][c].z, v[q][c].w};
+          // the chunk's edge words: four 32-bit or eight 16-bit ones
+          uint32_t wv[kEPC];
+          if constexpr (NARROW) {
+            const uint32_t raw[4] = {v[q][c].x, v[q][c].y, v[q][c].z, v[q][c].w};
 #pragma unroll
-          for (int m = 0; m < 4; ++m) {
-            const uint32_t e = a0[q] + (uint32_t)((rd * kCH + c) * 4 + m);
+            for (int m = 0; m < 4; ++m) {
+              wv[2 * m] = raw[m] & 0xFFFFu;
+              wv[2 * m + 1] = raw[m] >> 16;
+            }
+          } else {
+            wv[0] = v[q][c].x;
+            wv[1] = v[q][c].y;
+            wv[2] = v[q][c].z;
+            wv[3] = v[q][c].w;
+          }
+#pragma unroll
+          for (int m = 0; m < (int)kEPC; ++m) {
+            const uint32_t e = a0[q] + (uint32_t)((rd * kCH + c) * (int)kEPC + m);
             if (e - e0[q] < e1[q] - e0[q]) {
               const uint32_t off = wv[m];
               if (off < limq[q]) {  // an LDS counter (this window's steps, and later ones when they fit)
@@ -1401,6 +1422,7 @@ struct bs_net {
   long long Ntot = 0;
   int W = 1, d_min = 0, d_max = 0, Ls = 1, max_span = 1, TS = 256;
   int psp = 1;            // plastic gather: source steps per projection and window, at most
+  bool narrow = false;    // static edge words are 16-bit
   size_t lds_bytes = 0;
   size_t lds_budget = 0;  // dynamic LDS the window kernel may use
   std::vector<DevPop> pops;
@@ -1738,6 +1760,14 @@ __global__ void k_seg_edges(const uint32_t* __restrict__ order, long long E, con
   const int dd = delay ? delay[e] : delay_steps;
   ew[k] = (uint32_t)dd * (uint32_t)RS + (uint32_t)(post[e] % TS);
   if (eq) eq[k] = (long long)llrint(ldexp(w[e], F));
+}
+
+// 16-bit copy of a projection's edge words (each below 2^16), zero-padded by
+// eight words: the gather reads whole 16-byte chunks past a segment's end.
+__global__ void k_narrow_edges(const uint32_t* __restrict__ in, long long E, uint16_t* __restrict__ out) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < E + 8;
+       k += (long long)gridDim.x * blockDim.x)
+    out[k] = k < E ? (uint16_t)in[k] : (uint16_t)0;
 }
 
 // Scratch of the device segment build, grown to the largest projection.
@@ -2325,8 +2355,10 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
       CHK(build_segments_device(net, pj, kind, d->channels[pj.channel].fixed_shift, RS, d->pops[pj.pre_pop].n,
                                 d->pops[pj.post_pop].n, gid0_of[pj.pre_pop], seg_scratch, G, pdmin, pdmax));
       G.rs = RS;
+      G.ne = pj.n_edges;
     } else {
       StaticSeg& S = sseg[q];
+      G.ne = pj.n_edges;
       if (!S.err.empty()) BAD(S.err);
       pdmin = S.pdmin;
       pdmax = S.pdmax;
@@ -2368,6 +2400,38 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   }
   net->d_min = dmin;
   net->d_max = dmax;
+  // 16-bit edge words when every static projection's words (delay * stride +
+  // lane) fit and the (pre, tile) segments are long enough for the halved
+  // bytes and chunk rounds to pay for the unpacking: measured on one box,
+  // +4 % on the 256-network ensemble (12.8 edges per segment) and -3 % on
+  // Brunel-11250 (3.2 per segment with 32-neuron tiles). The 32-bit arrays
+  // are converted on the device and released.
+  {
+    bool nar = !gp_sorted.empty();
+    double ne = 0.0, nseg = 0.0;
+    for (const GProj& G : gp_sorted) {
+      if ((long long)G.d_max * G.rs + TS > 65535) nar = false;
+      ne += (double)G.ne;
+      nseg += (double)net->pops[G.pre_pop].n * G.nt;
+    }
+    if (ne < 6.0 * nseg) nar = false;
+    if (nar) {
+      for (GProj& G : gp_sorted) {
+        uint16_t* dn = nullptr;
+        CHK(dmalloc(&dn, (size_t)G.ne + 8));
+        net->owned.push_back(dn);
+        hipLaunchKernelGGL(k_narrow_edges, dim3((unsigned)std::min<long long>((G.ne + 8 + 255) / 256, 65535)),
+                           dim3(256), 0, net->stream, G.edges, G.ne, dn);
+        HCHK(hipGetLastError());
+        HCHK(hipStreamSynchronize(net->stream));
+        void* old = (void*)G.edges;
+        net->owned.erase(std::remove(net->owned.begin(), net->owned.end(), old), net->owned.end());
+        HCHK(hipFree(old));
+        G.edges = (const uint32_t*)dn;
+      }
+      net->narrow = true;
+    }
+  }
 
   // ---- Poisson producers: when a launch of one or two rounds of tiles leaves
   // at least 8 block slots free, blocks in those slots draw each population's
@@ -2757,14 +2821,14 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
   CHK(dmalloc(&net->d_tap_col, std::max<size_t>(1, net->vm_tap_col.size())));
   if (!net->vm_tap_col.empty())
     HCHK(hipMemcpy(net->d_tap_col, net->vm_tap_col.data(), 4 * net->vm_tap_col.size(), hipMemcpyHostToDevice));
-  HCHK(hipFuncSetAttribute((const void*)k_window<double, false>,
-                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)net->lds_budget));
-  HCHK(hipFuncSetAttribute((const void*)k_window<float, false>,
-                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)net->lds_budget));
-  HCHK(hipFuncSetAttribute((const void*)k_window<double, true>,
-                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)net->lds_budget));
-  HCHK(hipFuncSetAttribute((const void*)k_window<float, true>,
-                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)net->lds_budget));
+  {
+    const void* kw[8] = {(const void*)k_window<double, false, false>, (const void*)k_window<float, false, false>,
+                         (const void*)k_window<double, true, false>,  (const void*)k_window<float, true, false>,
+                         (const void*)k_window<double, false, true>,  (const void*)k_window<float, false, true>,
+                         (const void*)k_window<double, true, true>,   (const void*)k_window<float, true, true>};
+    for (const void* f : kw)
+      HCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)net->lds_budget));
+  }
   {
     int rc = upload_state(net);
     if (rc != BS_OK) {
@@ -2868,12 +2932,19 @@ static void launch_window(bs_net* net, int n0, int W, long long vm_step0, int pr
   a.pjobs = net->d_pjobs;
   a.n_pjobs = (int)net->pjobs.size();
   a.pjob_n = net->pjob_n;
-  if (net->stdp.empty())
-    hipLaunchKernelGGL((k_window<T, false>), dim3((unsigned)(net->tiles.size() + net->n_prod)), dim3(256),
-                       net->lds_bytes, net->stream, a);
-  else
-    hipLaunchKernelGGL((k_window<T, true>), dim3((unsigned)(net->tiles.size() + net->n_prod)), dim3(256),
-                       net->lds_bytes, net->stream, a);
+  const dim3 grid((unsigned)(net->tiles.size() + net->n_prod));
+  const bool pl = !net->stdp.empty();
+  if (net->narrow) {
+    if (pl)
+      hipLaunchKernelGGL((k_window<T, true, true>), grid, dim3(256), net->lds_bytes, net->stream, a);
+    else
+      hipLaunchKernelGGL((k_window<T, false, true>), grid, dim3(256), net->lds_bytes, net->stream, a);
+  } else {
+    if (pl)
+      hipLaunchKernelGGL((k_window<T, true, false>), grid, dim3(256), net->lds_bytes, net->stream, a);
+    else
+      hipLaunchKernelGGL((k_window<T, false, false>), grid, dim3(256), net->lds_bytes, net->stream, a);
+  }
 }
 
 // Poisson counts of the window starting at n0 (before a run's first window).
@@ -3090,8 +3161,9 @@ static int drain_chunk(bs_net* net, const ChunkJob& j, bool pin_vm, double* vm_o
   blk.n = w;
   net->n_out += (long long)w;
   net->n_events += ev;
-  // gather: each spike's bucket entry and segment bounds, 4 B per edge; 8 B record
-  net->b_deliver += 8.0 * (double)cnt + 4.0 * (double)ev;
+  // gather: each spike's bucket entry and segment bounds, one edge word per
+  // edge (2 B, or 4 B when the words need 32 bits); 8 B record
+  net->b_deliver += 8.0 * (double)cnt + (net->narrow ? 2.0 : 4.0) * (double)ev;
   net->b_update += 8.0 * (double)cnt;
   for (bs_net::WTap& t : net->wtaps)
     if (t.E > 0 && j.chunk > 0)

@@ -6,7 +6,7 @@
 * a count table too long for LDS (in-degree 20,000 from one projection: the
   inputs phase reads the table from global memory);
 * delays far beyond the window (up to 40 ms): arrivals past the LDS horizon
-  go through the arrival ring;
+  go through the arrival ring, with 16-bit and with 32-bit edge words;
 * only some populations recorded, runs of 0 and 1 steps, and a silent network.
 """
 import numpy as np
@@ -97,11 +97,19 @@ def test_count_table_beyond_lds_uses_global_table():
     assert (g >= 2000).any()
 
 
-def test_long_delays_through_the_arrival_ring():
-    sim = Simulator(dt=0.1 * u.ms)
+@pytest.mark.parametrize('tile_size,k_ab', [(0, 40), (0, 200), (256, 40)])
+def test_long_delays_through_the_arrival_ring(tile_size, k_ab):
+    """Delays up to 40 ms, through both gather paths. Automatic tiles are 32
+    neurons here, so the edge words fit 16 bits (delay * stride + lane <=
+    400 * 32 + 31); they are 16-bit when the (pre, tile) segments hold six
+    edges or more on average (over both projections: 11 with in-degree 200;
+    3 with 40, 32-bit). With
+    256-neuron tiles 400 * 256 + 255 needs 32-bit words."""
+    sim = Simulator(dt=0.1 * u.ms, tile_size=tile_size)
     a = sim.create(iaf_psc_alpha, 300, params=dict(I_e=400.0))
     b = sim.create(iaf_psc_alpha, 300)
-    sim.connect(a, b, rule=fixed_indegree(40), weight=80. * u.pA, delay=UniformGrid(0.2, 40.0, 0.1), seed=6)
+    w_ab = 80. * 40 / k_ab
+    sim.connect(a, b, rule=fixed_indegree(k_ab), weight=w_ab * u.pA, delay=UniformGrid(0.2, 40.0, 0.1), seed=6)
     sim.connect(b, a, rule=fixed_indegree(20), weight=-40. * u.pA, delay=UniformGrid(5.0, 35.0, 0.1), seed=7)
     ra, rb = sim.create(spike_recorder), sim.create(spike_recorder)
     sim.connect(a, ra)
