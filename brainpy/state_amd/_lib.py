@@ -110,6 +110,7 @@ _SIGS = {
     'bs_net_load_state': (C.c_int, [_P, _P, C.c_int64]),
     'bs_net_run_device': (C.c_int, [_P, C.c_int64, _P, _P, _P, _P]),
     'bs_net_set_weight_taps': (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
+    'bs_net_set_drive': (C.c_int, [_P, C.c_int32, C.c_int64, C.c_int64, C.c_int32, C.POINTER(C.c_int32)]),
     'bs_net_copy_weight_trace': (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
     'bs_net_set_timing': (C.c_int, [_P, C.c_int]),
     'bs_net_last_timing': (C.c_int, [_P] + [C.POINTER(C.c_double)] * 3 + [C.POINTER(C.c_int64)] * 2),

@@ -139,6 +139,11 @@ struct DevChan {
   double frac;                   // POISSON: InputDelay lerp weight of frame delay + 1 (0: whole steps)
   const long long* sched;        // SCHEDULE: [2 * cdf_len] plateau start, stop steps
   const double* sval;            // SCHEDULE: [cdf_len] plateau values
+  // POISSON: counts injected by bs_net_set_drive for generator steps
+  // [d0, d0 + dn) ([dn][dw], element e at e), used instead of the draws
+  const int32_t* drive;
+  long long d0, dn;
+  int dw, n_gen;                 // n_gen: generator elements (max gen_index + 1, or the population size)
 };
 
 // One block's tile: everything its prologue needs before the population's
@@ -338,6 +343,9 @@ struct ChReg {
   double frac;                  // POISSON: lerp weight of the frame one step further back
   const long long* sched;       // SCHEDULE: plateau steps and values (global memory)
   const double* sval;
+  const int32_t* drive;         // POISSON: injected counts (see DevChan)
+  long long d0, dn;
+  int dw;
 };
 
 template <typename T>
@@ -365,6 +373,10 @@ __device__ __forceinline__ ChReg<T> load_ch(const DevChan& ch, const unsigned ch
   c.frac = ch.frac;
   c.sched = ch.sched;
   c.sval = ch.sval;
+  c.drive = ch.drive;
+  c.d0 = ch.d0;
+  c.dn = ch.dn;
+  c.dw = ch.dw;
   const bool staged = ch.lds_off >= 0;
   c.tab = staged ? (const T*)(tab + ch.lds_off) : tables + ch.table_off;
   c.tabg = tables + ch.table_off;
@@ -406,6 +418,15 @@ __device__ __forceinline__ double sched_at(const long long* se, const double* sv
   return (lo > 0 && n < se[2 * (lo - 1) + 1]) ? sv[lo - 1] : 0.0;
 }
 
+// The count of generator element e at its step sgs: the injected count when
+// bs_net_set_drive covers sgs, else the channel's Threefry draw.
+__device__ __forceinline__ uint32_t pois_count(const int32_t* drive, long long d0, long long dn, int dw,
+                                               uint32_t seed, long long sgs, int e, const double* cdf,
+                                               const unsigned short* guide, int cdf_len, int parts) {
+  if (drive && sgs >= d0 && sgs < d0 + dn) return (uint32_t)drive[(sgs - d0) * dw + e];
+  return poisson_inv(seed, (uint32_t)sgs, (uint32_t)e, cdf, guide, cdf_len, parts);
+}
+
 // Poisson drive of neuron j at step `step`: the generator's count of step
 // step - 1 - delay times the weight, or with a fractional delay the InputDelay
 // lerp (delay.py:114-128) of frames d and d + 1, x_lo (1 - frac) + x_hi frac,
@@ -418,12 +439,12 @@ __device__ __forceinline__ T pois_value(const ChReg<T>& c, const double* cdf, co
   const long long sgs = (long long)step - 1 - c.delay;
   uint32_t cn = 0;
   if (e >= 0 && sgs >= 0 && sgs > c.tmin && sgs <= c.tmax)
-    cn = poisson_inv(c.seed, (uint32_t)sgs, (uint32_t)e, cdf, guide, c.cdf_len, c.parts);
+    cn = pois_count(c.drive, c.d0, c.dn, c.dw, c.seed, sgs, e, cdf, guide, c.cdf_len, c.parts);
   if (c.frac == 0.0) return (T)cn * c.weight;
   const long long sgh = sgs - 1;
   uint32_t ch = 0;
   if (e >= 0 && sgh >= 0 && sgh > c.tmin && sgh <= c.tmax)
-    ch = poisson_inv(c.seed, (uint32_t)sgh, (uint32_t)e, cdf, guide, c.cdf_len, c.parts);
+    ch = pois_count(c.drive, c.d0, c.dn, c.dw, c.seed, sgh, e, cdf, guide, c.cdf_len, c.parts);
   const T f = (T)c.frac;
   return ((T)cn * ((T)1 - f) + (T)ch * f) * c.weight;
 }
@@ -866,7 +887,7 @@ __device__ __forceinline__ void produce_poisson(const WinArgs& a, int n1, int bl
     const long long sgs = (long long)step - 1 - c.delay;
     uint32_t cn = 0;
     if (e >= 0 && sgs >= 0 && sgs > c.tmin && sgs <= c.tmax)
-      cn = poisson_inv(c.seed, (uint32_t)sgs, (uint32_t)e, c.cdf, c.guide, c.cdf_len, c.parts);
+      cn = pois_count(c.drive, c.d0, c.dn, c.dw, c.seed, sgs, e, c.cdf, c.guide, c.cdf_len, c.parts);
     a.pcnt[(long long)(step % L2) * a.Ntot + J.gid0 + i] = (unsigned short)cn;
   }
 }
@@ -1513,6 +1534,8 @@ struct bs_net {
   long long pjob_n = 0;
   int n_prod = 0;
   std::vector<void*> owned;  // per-projection gather arrays, generator maps
+  std::vector<int32_t*> drive;     // per channel: counts injected by bs_net_set_drive (owned), or null
+  std::vector<long long> pop_blob;  // per population: offset of its blob in d_blob (-1: none)
   // host copies
   std::vector<double> state0;  // [BS_NSTATE][Ntot]
   std::vector<int> ref0;
@@ -1560,6 +1583,7 @@ static void free_net(bs_net* net) {
   F(net->d_rec[0]); F(net->d_rec[1]); F(net->d_rec_sorted); F(net->d_sort_tmp);
   F(net->d_rec_count); F(net->d_flags); F(net->d_vm_col); F(net->d_vm); F(net->d_tap_col);
   for (void* p : net->owned) F(p);
+  for (int32_t* p : net->drive) F(p);
   for (auto& t : net->wtaps) {
     F(t.d_ref);
     F(t.d_traj);
@@ -2029,10 +2053,14 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
         if (ch.delay_steps < 0) BAD("Poisson delay must be >= 0");
         if (!(ch.delay_frac >= 0.0 && ch.delay_frac < 1.0)) BAD("Poisson delay_frac must be in [0, 1)");
         if (ch.lam == 0.0) dc.tmax = -1;  // rate <= 0: all-zero output
+        dc.n_gen = sp.n;
         if (ch.gen_index) {
           int* g = nullptr;
           CHK(upload(net, &g, ch.gen_index, (size_t)sp.n));
           dc.gen_index = g;
+          int mx = -1;
+          for (int j = 0; j < sp.n; ++j) mx = std::max(mx, ch.gen_index[j]);
+          dc.n_gen = mx + 1;
         }
       } else if (ch.kind == BS_CH_CURRENT) {
         if (!(ch.weight == ch.weight)) BAD("current channel value must not be NaN");
@@ -2645,6 +2673,7 @@ int bs_net_create(const bs_net_desc* d, bs_net** out) {
       T.blob_off = boff[T.pop];
       T.tab_bytes = dp.tab_bytes;
     }
+    net->pop_blob = boff;
   }
 
   // ---- launch order of the tiles. A launch with more tiles than two per CU
@@ -3539,6 +3568,55 @@ int bs_net_set_state(bs_net* net, int32_t pop, int32_t var, const double* in) {
     std::vector<float> f(in, in + p.n);
     HIPCHK(hipMemcpy((float*)net->d_state + off, f.data(), 4 * (size_t)p.n, hipMemcpyHostToDevice));
   }
+  return BS_OK;
+}
+
+int bs_net_set_drive(bs_net* net, int32_t channel, int64_t step0, int64_t n_steps, int32_t n_gen,
+                     const int32_t* counts) {
+  if (!net) return fail(BS_EINVAL, "bs_net_set_drive: null handle");
+  if (channel < 0 || channel >= (int)net->chans.size() || net->chans[channel].kind != BS_CH_POISSON)
+    return fail(BS_EINVAL, "bs_net_set_drive: channel is not a Poisson channel");
+  DevChan& dc = net->chans[channel];
+  if (step0 < 0 || n_steps < 0 || step0 + n_steps >= (1ll << 31))
+    return fail(BS_EINVAL, "bs_net_set_drive: step range out of bounds");
+  if (n_steps > 0 && (!counts || n_gen < dc.n_gen))
+    return fail(BS_EINVAL, "bs_net_set_drive: counts must hold n_steps rows of at least " +
+                               std::to_string(dc.n_gen) + " generator elements");
+  const int w = std::max(1, dc.n_gen);
+  std::vector<int32_t> rows((size_t)n_steps * w, 0);
+  for (long long t = 0; t < n_steps; ++t)
+    for (int e = 0; e < dc.n_gen; ++e) {
+      const int32_t v = counts[(size_t)t * n_gen + e];
+      if (v < 0 || v > 65535) return fail(BS_EINVAL, "bs_net_set_drive: counts must be in [0, 65535]");
+      rows[(size_t)t * w + e] = v;
+    }
+  HIPCHK(hipSetDevice(net->device));
+  HIPCHK(hipStreamSynchronize(net->stream));
+  if (net->drive.empty()) net->drive.assign(net->chans.size(), nullptr);
+  int32_t* old = net->drive[channel];
+  int32_t* dd = nullptr;
+  if (n_steps > 0) {
+    HIPCHK(hipMalloc(&dd, sizeof(int32_t) * rows.size()));
+    HIPCHK(hipMemcpy(dd, rows.data(), sizeof(int32_t) * rows.size(), hipMemcpyHostToDevice));
+  }
+  dc.drive = dd;
+  dc.d0 = step0;
+  dc.dn = n_steps;
+  dc.dw = w;
+  net->drive[channel] = dd;
+  // the channel record lives in the channel array and in its population's blob
+  HIPCHK(hipMemcpy(net->d_chans + channel, &dc, sizeof(DevChan), hipMemcpyHostToDevice));
+  for (size_t p = 0; p < net->pops.size(); ++p) {
+    const DevPop& dp = net->pops[p];
+    if (channel < dp.ch_begin || channel >= dp.ch_begin + dp.ch_count) continue;
+    if (p < net->pop_blob.size() && net->pop_blob[p] >= 0) {
+      const size_t o = (size_t)net->pop_blob[p] + blob_layout(dp.ch_count, dp.gp_count, (size_t)dp.tab_bytes).ch +
+                       sizeof(DevChan) * (size_t)(channel - dp.ch_begin);
+      memcpy(&net->h_blob[o], &dc, sizeof(DevChan));
+      HIPCHK(hipMemcpy(net->d_blob + o, &dc, sizeof(DevChan), hipMemcpyHostToDevice));
+    }
+  }
+  if (old) HIPCHK(hipFree(old));
   return BS_OK;
 }
 

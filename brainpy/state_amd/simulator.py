@@ -318,6 +318,7 @@ class Simulator:
         self._analog_taps = {}       # id(multimeter) -> (pop, idx, names)
         self._weight_taps = {}
         self._records = []           # projections and generator channels, ordinal order
+        self._drives = {}            # id(generator connection) -> (conn, start step, counts)
         self._proj_counter = itertools.count()
         self._engine = None
         self._base_step = 0
@@ -598,7 +599,39 @@ class Simulator:
         if self._engine is None:
             L.require_device()
             self._engine = _Engine(self.build_desc())
+            for conn, step0, counts in self._drives.values():
+                self._engine.set_drive(self._engine.desc.chan_index[id(conn)], step0, counts)
         return self._engine
+
+    def set_poisson_drive(self, conn, counts, *, start_step=0):
+        """Inject the counts of a connected poisson_generator instead of its
+        random draws: ``conn`` is the handle ``connect(generator, target)``
+        returned, ``counts[t][e]`` the count of generator element ``e`` (one
+        per target neuron, in the target's order) at generator step
+        ``start_step + t`` (the step whose update emits it; it arrives after
+        the connection's delay). The generator's start/stop window still
+        applies. For runs against the reference's own Poisson stream
+        (``jax.random.poisson``, poisson_generator.py:501-576): feed it the
+        reference's counts. ``counts=None`` removes the injection. This is a
+        parity hook with no counterpart in the reference."""
+        if not isinstance(conn, _GenChannel):
+            raise TypeError('set_poisson_drive takes the handle of a poisson_generator connection')
+        if counts is None:
+            self._drives.pop(id(conn), None)
+            if self._engine is not None:
+                self._engine.set_drive(self._engine.desc.chan_index[id(conn)], 0, np.zeros((0, 1), np.int32))
+            return
+        c = np.ascontiguousarray(counts, dtype=np.int64)
+        if c.ndim != 2 or c.shape[1] != len(conn.post_idx):
+            raise ValueError(f'counts must be [n_steps][{len(conn.post_idx)}] (one column per target neuron)')
+        if c.size and (c.min() < 0 or c.max() > 65535):
+            raise ValueError('counts must be in [0, 65535]')
+        if int(start_step) < 0:
+            raise ValueError('start_step must be >= 0')
+        c = c.astype(np.int32)
+        self._drives[id(conn)] = (conn, int(start_step), c)
+        if self._engine is not None:
+            self._engine.set_drive(self._engine.desc.chan_index[id(conn)], int(start_step), c)
 
     def reset_rollout(self, *, dt=None):
         eng = self._ensure_engine()
@@ -793,6 +826,7 @@ class _Desc:
         self.tap_cols = {}
         self.n_taps = 0
         self.proj_index = {}
+        self.chan_index = {}             # id(generator connection) -> Poisson channel
         self.device_edges = False        # some projection passes device pointers
 
     def arr(self, a, ctype):
@@ -935,6 +969,7 @@ class _NetBuilder:
                     table_off.append(-1)
                 elif isinstance(r, _GenChannel):
                     g = r.gen
+                    D.chan_index[id(r)] = c
                     ch.kind = L.BS_CH_POISSON
                     ch.weight = r.weight
                     ch.lam = g.lam(dt)
@@ -1110,6 +1145,11 @@ class _Engine:
         a, b, n = C.c_double(), C.c_double(), C.c_int64()
         L.check(self.lib.bs_net_last_bytes(self.h, C.byref(a), C.byref(b), C.byref(n)))
         return dict(deliver_bytes=a.value, update_bytes=b.value, n_events=n.value)
+
+    def set_drive(self, channel, step0, counts):
+        c = np.ascontiguousarray(counts, dtype=np.int32)
+        L.check(self.lib.bs_net_set_drive(self.h, int(channel), int(step0), c.shape[0], c.shape[1],
+                                          c.ctypes.data_as(C.POINTER(C.c_int32))))
 
     def set_weight_taps(self, proj_indices):
         arr = np.ascontiguousarray(proj_indices, dtype=np.int32)

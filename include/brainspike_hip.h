@@ -251,6 +251,18 @@ int bs_net_load_state(bs_net* net, const void* buf, int64_t size);
  * run (events), so calls on one handle stay ordered with each other. */
 int bs_net_run_device(bs_net* net, int64_t n_steps, void* stream, uint32_t* raster, double* vm,
                       int32_t* status);
+/* Drive injection for parity (the contract's bs_net_set_drive): Poisson
+ * channel `channel` takes its generator's counts for the generator steps
+ * step0 .. step0 + n_steps - 1 from counts[t][e] (n_steps rows of n_gen >=
+ * the generator's size, values in [0, 65535]) instead of drawing them; the
+ * generator's activity window still applies (poisson_generator.py:570-576:
+ * where(t_min < step <= t_max, counts, 0)), and a count drawn at step n
+ * arrives at n + 1 + delay as before. Replaces the reference's
+ * jax.random.poisson draws (poisson_generator.py:501-569), so its own counts
+ * can be fed in for a run against its raster. n_steps = 0 removes the
+ * injection. Steps outside the range draw from the channel's stream. */
+int bs_net_set_drive(bs_net* net, int32_t channel, int64_t step0, int64_t n_steps, int32_t n_gen,
+                     const int32_t* counts);
 /* Plastic weights of projection proj in the reference's CSR (sorted-by-pre,
  * stable) edge order (event_plastic.py:293-296). */
 int bs_net_get_weights(bs_net* net, int32_t proj, double* out);

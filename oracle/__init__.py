@@ -96,6 +96,7 @@ def lib():
             'orc_copy_spikes': (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int64]),
             'orc_get_state': (C.c_int, [P, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
             'orc_get_weights': (C.c_int, [P, C.c_int32, C.POINTER(C.c_double)]),
+            'orc_set_drive': (C.c_int, [P, C.c_int32, C.c_int64, C.c_int64, C.c_int32, C.POINTER(C.c_int32)]),
             'orc_threefry2x32': (None, [C.c_uint32] * 4 + [C.POINTER(C.c_uint32)] * 2),
             'orc_neuron_step': (C.c_int, [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                           C.POINTER(C.c_int), C.c_double, C.c_double, C.c_int]),
@@ -245,6 +246,15 @@ class OracleNet:
         out = np.zeros(max(1, n))
         _check(lib().orc_get_state(self.h, pop, var, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out[:n]
+
+    def set_drive(self, channel, step0, counts):
+        """Inject generator counts [T][n_gen] for channel ``channel`` from
+        generator step ``step0`` on (see orc_set_drive)."""
+        c = np.ascontiguousarray(counts, dtype=np.int32)
+        if c.ndim != 2:
+            raise ValueError('counts must be [n_steps][n_gen]')
+        _check(lib().orc_set_drive(self.h, int(channel), int(step0), c.shape[0], c.shape[1],
+                                   c.ctypes.data_as(C.POINTER(C.c_int32))))
 
     def weights(self, proj, n):
         out = np.zeros(max(1, n))

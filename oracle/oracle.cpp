@@ -225,6 +225,11 @@ struct orc_net {
   std::vector<bs_channel> chans;
   std::vector<std::vector<int>> gen_index;  // per channel (POISSON)
   std::vector<PoissonTable> ptab;           // per channel (POISSON)
+  // per channel (POISSON): counts injected by orc_set_drive for generator
+  // steps [drive0, drive0 + drive_n), rows of drive_w elements
+  std::vector<std::vector<int32_t>> drive;
+  std::vector<int64_t> drive0, drive_n;
+  std::vector<int> drive_w;
   std::vector<std::vector<int64_t>> sched;  // per channel (SCHEDULE): plateau [start, stop) steps
   std::vector<std::vector<double>> sched_val;
   std::vector<OrcProj> projs;
@@ -412,6 +417,10 @@ extern "C" int orc_create(const bs_net_desc* d, orc_net** out) {
     net->hist.emplace_back(net->Lh, std::vector<double>(sp.n, 0.0));
   }
   net->ghist.resize(d->n_channels);
+  net->drive.assign(d->n_channels, {});
+  net->drive0.assign(d->n_channels, 0);
+  net->drive_n.assign(d->n_channels, 0);
+  net->drive_w.assign(d->n_channels, 0);
   net->cring.resize(d->n_channels);
   net->qring.resize(d->n_channels);
   net->chan_pop.assign(d->n_channels, 0);
@@ -834,8 +843,19 @@ static int run_t(orc_net* net, int64_t n_steps, double* vm_out) {
       std::vector<double>& h = net->ghist[c][hs];
       const std::vector<int>& gi = net->gen_index[c];
       const bool active = (ch.t_min_step < n) && (n <= ch.t_max_step) && ch.lam > 0.0;
-      for (size_t j = 0; j < h.size(); ++j)
-        h[j] = (active && gi[j] >= 0) ? (double)draw_poisson(net->ptab[c], ch.seed, (uint32_t)n, (uint32_t)gi[j]) : 0.0;
+      // poisson_generator.update (poisson_generator.py:501-576): counts of
+      // this step, kept inside the activity window; an injected count
+      // stands for the draw
+      const bool inj = !net->drive[c].empty() && n >= net->drive0[c] && n < net->drive0[c] + net->drive_n[c];
+      for (size_t j = 0; j < h.size(); ++j) {
+        if (!active || gi[j] < 0) {
+          h[j] = 0.0;
+        } else if (inj) {
+          h[j] = (double)net->drive[c][(size_t)(n - net->drive0[c]) * net->drive_w[c] + gi[j]];
+        } else {
+          h[j] = (double)draw_poisson(net->ptab[c], ch.seed, (uint32_t)n, (uint32_t)gi[j]);
+        }
+      }
     }
     for (int p = 0; p < np; ++p) {
       const bs_population& sp = net->pops[p];
@@ -917,6 +937,25 @@ extern "C" int orc_get_state(orc_net* net, int32_t pop, int32_t var, double* out
     return fail(BS_EINVAL, "orc_get_state: bad arguments");
   const int n = net->pops[pop].n;
   for (int i = 0; i < n; ++i) out[i] = var < 0 ? (double)net->ref[pop][i] : net->st[pop][(size_t)var * n + i];
+  return BS_OK;
+}
+
+// Injected generator counts (the engine's bs_net_set_drive): channel c's
+// generator steps step0 .. step0 + n_steps - 1 take counts[t][e] (rows of
+// n_gen elements) instead of draws; n_steps = 0 removes them.
+extern "C" int orc_set_drive(orc_net* net, int32_t c, int64_t step0, int64_t n_steps, int32_t n_gen,
+                             const int32_t* counts) {
+  if (!net || c < 0 || c >= (int)net->chans.size() || net->chans[c].kind != BS_CH_POISSON)
+    return fail(BS_EINVAL, "orc_set_drive: channel is not a Poisson channel");
+  if (step0 < 0 || n_steps < 0 || (n_steps > 0 && (!counts || n_gen <= 0)))
+    return fail(BS_EINVAL, "orc_set_drive: bad arguments");
+  int width = 0;
+  for (int g : net->gen_index[c]) width = std::max(width, g + 1);
+  if (n_steps > 0 && n_gen < width) return fail(BS_EINVAL, "orc_set_drive: rows narrower than the generator");
+  net->drive[c].assign(counts, counts + (size_t)n_steps * (n_steps > 0 ? n_gen : 0));
+  net->drive0[c] = step0;
+  net->drive_n[c] = n_steps;
+  net->drive_w[c] = n_gen;
   return BS_OK;
 }
 

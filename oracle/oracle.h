@@ -38,6 +38,7 @@ int orc_spike_count(orc_net* net, int64_t* n);
 int orc_copy_spikes(orc_net* net, int32_t* gid, int32_t* step, int64_t cap);
 int orc_get_state(orc_net* net, int32_t pop, int32_t var, double* out);
 int orc_get_weights(orc_net* net, int32_t proj, double* out);
+int orc_set_drive(orc_net* net, int32_t c, int64_t step0, int64_t n_steps, int32_t n_gen, const int32_t* counts);
 
 /* One update of a single neuron with explicit summed inputs; returns the
  * spike flag (0/1) or a negative error. */
