@@ -258,13 +258,17 @@ __global__ __launch_bounds__(kTile) void k_bptt_fwd(FwdArgs a) {
     }
     __syncthreads();
     // ---- gather: every event of the chunk in list order (per step: inputs,
-    // then presynaptic neurons, ascending), kBatch loads in flight, summed
-    // into this neuron's LDS slot of its step
+    // then presynaptic neurons, ascending), kBatch loads in flight. The list
+    // is ordered by step, so each step's two sums run in registers (the same
+    // additions in the same order as summing in place) and are written to
+    // this neuron's LDS slots when the list moves to the next step.
     for (int k = 0; k < nc; ++k) {
       xs[0][k][tid] = 0.f;
       xs[1][k][tid] = 0.f;
     }
     const int E = ev_off[nc];
+    float ae = 0.f, ai = 0.f;
+    int kcur = 0;
     for (int e0 = 0; e0 < E; e0 += kBatch) {
       float v[kBatch];
 #pragma unroll
@@ -272,13 +276,22 @@ __global__ __launch_bounds__(kTile) void k_bptt_fwd(FwdArgs a) {
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         if (e0 + u < E) {
+          const int k = ev_k[e0 + u];
+          if (k != kcur) {  // block-uniform: the list is shared
+            xs[0][kcur][tid] = ae;
+            xs[1][kcur][tid] = ai;
+            ae = ai = 0.f;
+            kcur = k;
+          }
           const int row = ev_row[e0 + u];
           const float x = row < a.n_in ? (float)ev_sc[e0 + u] * v[u] : v[u];
-          float* acc = &xs[row < nex_rows ? 0 : 1][ev_k[e0 + u]][tid];
-          *acc = *acc + x;
+          if (row < nex_rows) ae = ae + x;
+          else ai = ai + x;
         }
       }
     }
+    xs[0][kcur][tid] = ae;
+    xs[1][kcur][tid] = ai;
     // ---- the steps
     for (int k = 0; k < nc; ++k) {
       const int n = a.n0 + c0 + k;
