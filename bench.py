@@ -229,6 +229,7 @@ def main():
         'ms_per_step': round(ms_step, 4),
         'step_unit': f'one simulated window of {args.window_ms:g} ms ({win} network steps of dt = {sim.dt} ms)',
         'ms_per_step_median': round(float(np.median(walls)) * 1e3, 4),
+        'ms_per_window': [round(w * 1e3, 3) for w in walls],
         'ms_per_network_step': round(ms_step / win, 6),
         'higher_is_better': True,
         'scaling': 'weak',
