@@ -282,6 +282,13 @@ __device__ __forceinline__ bool step_aeif(const double* __restrict__ P, double* 
   h = h > minh ? h : minh;
   double tl = 0.0;
   const int maxit = (int)P[C_MAXIT];
+  // k0 = f(y) of the coming substep: evaluated before the first one, and
+  // after each accepted substep (from the new state and refractory count,
+  // next to the step-size update, which it does not depend on); a rejected
+  // substep leaves y and r unchanged, so its k0 serves again. The values are
+  // those of evaluating k0 at the top of every substep.
+  AeifY k0;
+  aeif_rhs(P, R, y, r > 0, istim, k0);
   for (int it = 0; it < maxit && tl < dt; ++it) {
     {
       const double rem = dt - tl;
@@ -289,8 +296,7 @@ __device__ __forceinline__ bool step_aeif(const double* __restrict__ P, double* 
       h = minh > hm ? minh : hm;
     }
     const bool refr = r > 0;
-    AeifY k0, k1, k2, k3, k4, k5, t;
-    aeif_rhs(P, R, y, refr, istim, k0);
+    AeifY k1, k2, k3, k4, k5, t;
 #define BS_STAGE1(f) t.f = y.f + h * (A10 * k0.f);
     BS_STAGE1(V) BS_STAGE1(dgex) BS_STAGE1(gex) BS_STAGE1(dgin) BS_STAGE1(gin) BS_STAGE1(w)
     aeif_rhs(P, R, t, refr, istim, k1);
@@ -337,20 +343,24 @@ __device__ __forceinline__ bool step_aeif(const double* __restrict__ P, double* 
         if (refn > 0) r = refn + 1;
       }
     }
+    if (accept && tl < dt) aeif_rhs(P, R, y, r > 0, istim, k0);
     const double errsg = err > 1e-30 ? err : 1e-30;
     const double inv = 1.0 / errsg;
+    // one pow for both branches (err^(-1/5) after acceptance, err^(-1/4)
+    // after rejection), so a wave with both kinds of lanes runs it once
+    const double pw = pow_p(inv, accept ? 0.2 : 0.25);
     if (accept) {
       double fa;
       if (errsg <= 1e-30) {
         fa = 5.0;
       } else {
-        fa = 0.9 * pow_p(inv, 0.2);
+        fa = 0.9 * pw;
         fa = fa < 0.2 ? 0.2 : (fa > 5.0 ? 5.0 : fa);
       }
       const double hn = h * fa;
       h = minh > hn ? minh : hn;
     } else {
-      double fr = 0.9 * pow_p(inv, 0.25);
+      double fr = 0.9 * pw;
       fr = fr < 0.2 ? 0.2 : (fr > 1.0 ? 1.0 : fr);
       const double hn = h * fr;
       h = minh > hn ? minh : hn;

@@ -343,11 +343,11 @@ struct ChReg {
   double frac;                  // POISSON: lerp weight of the frame one step further back
   const long long* sched;       // SCHEDULE: plateau steps and values (global memory)
   const double* sval;
-  const int32_t* drive;         // POISSON: injected counts (see DevChan)
-  long long d0, dn;
-  int dw;
 };
 
+// pre: bit 0 the counts come from the producer blocks' ring, bit 1 counts
+// injected by bs_net_set_drive cover some steps (their fields stay in the LDS
+// record, read only on that path).
 template <typename T>
 __device__ __forceinline__ ChReg<T> load_ch(const DevChan& ch, const unsigned char* tab,
                                             const T* tables) {
@@ -362,7 +362,7 @@ __device__ __forceinline__ ChReg<T> load_ch(const DevChan& ch, const unsigned ch
   c.delay = ch.delay;
   c.parts = ch.parts;
   c.cdf_len = ch.cdf_len;
-  c.pre = ch.pre;
+  c.pre = ch.pre | (ch.drive ? 2 : 0);
   c.seed = ch.seed;
   c.tmin = ch.tmin;
   c.tmax = ch.tmax;
@@ -373,10 +373,6 @@ __device__ __forceinline__ ChReg<T> load_ch(const DevChan& ch, const unsigned ch
   c.frac = ch.frac;
   c.sched = ch.sched;
   c.sval = ch.sval;
-  c.drive = ch.drive;
-  c.d0 = ch.d0;
-  c.dn = ch.dn;
-  c.dw = ch.dw;
   const bool staged = ch.lds_off >= 0;
   c.tab = staged ? (const T*)(tab + ch.lds_off) : tables + ch.table_off;
   c.tabg = tables + ch.table_off;
@@ -432,26 +428,32 @@ __device__ __forceinline__ uint32_t pois_count(const int32_t* drive, long long d
 // lerp (delay.py:114-128) of frames d and d + 1, x_lo (1 - frac) + x_hi frac,
 // then times the weight. cdf and guide point at the staged LDS copy or at
 // global memory.
-template <typename T>
+// DRV: the channel may carry injected counts (c.pre bit 1; dch its LDS record).
+template <typename T, bool DRV>
 __device__ __forceinline__ T pois_value(const ChReg<T>& c, const double* cdf, const unsigned short* guide,
-                                        int j, int step) {
+                                        int j, int step, const DevChan* dch) {
   const int e = c.gen_index ? c.gen_index[j] : j;
   const long long sgs = (long long)step - 1 - c.delay;
+  auto count = [&](long long sg) -> uint32_t {
+    if constexpr (DRV) {
+      if (c.pre & 2) return pois_count(dch->drive, dch->d0, dch->dn, dch->dw, c.seed, sg, e, cdf, guide, c.cdf_len,
+                                       c.parts);
+    }
+    return poisson_inv(c.seed, (uint32_t)sg, (uint32_t)e, cdf, guide, c.cdf_len, c.parts);
+  };
   uint32_t cn = 0;
-  if (e >= 0 && sgs >= 0 && sgs > c.tmin && sgs <= c.tmax)
-    cn = pois_count(c.drive, c.d0, c.dn, c.dw, c.seed, sgs, e, cdf, guide, c.cdf_len, c.parts);
+  if (e >= 0 && sgs >= 0 && sgs > c.tmin && sgs <= c.tmax) cn = count(sgs);
   if (c.frac == 0.0) return (T)cn * c.weight;
   const long long sgh = sgs - 1;
   uint32_t ch = 0;
-  if (e >= 0 && sgh >= 0 && sgh > c.tmin && sgh <= c.tmax)
-    ch = pois_count(c.drive, c.d0, c.dn, c.dw, c.seed, sgh, e, cdf, guide, c.cdf_len, c.parts);
+  if (e >= 0 && sgh >= 0 && sgh > c.tmin && sgh <= c.tmax) ch = count(sgh);
   const T f = (T)c.frac;
   return ((T)cn * ((T)1 - f) + (T)ch * f) * c.weight;
 }
 
 // Input of one channel at window step w for lane tid (neuron j).
 template <typename T>
-__device__ __forceinline__ T chan_value(const UpdCtx<T>& u, const ChReg<T>& c, int w, int step,
+__device__ __forceinline__ T chan_value(const UpdCtx<T>& u, const ChReg<T>& c, int ci, int w, int step,
                                         int lane) {
   if (c.kind == BS_CH_COUNT) {
     uint32_t k = (u.cnt[(w * u.ncw + c.cw) * u.TS + lane] >> (16 * c.half)) & 0xFFFFu;
@@ -462,8 +464,8 @@ __device__ __forceinline__ T chan_value(const UpdCtx<T>& u, const ChReg<T>& c, i
     return k < (uint32_t)c.tab_n ? c.tab[k] : c.tabg[k];
   }
   if (c.kind == BS_CH_POISSON) {
-    if (c.pre) return (T)u.pc[w * u.TS + lane] * c.weight;
-    return pois_value<T>(c, c.cdf, c.guide, u.start + lane, step);
+    if (c.pre & 1) return (T)u.pc[w * u.TS + lane] * c.weight;
+    return pois_value<T, true>(c, c.cdf, c.guide, u.start + lane, step, &u.chs[ci]);
   }
   if (c.kind == BS_CH_CURRENT) {  // dc_generator: amplitude * weight inside [tmin, tmax)
     const int j = u.start + lane;
@@ -902,7 +904,7 @@ __global__ __launch_bounds__(256) void k_pois_fill(WinArgs a) {
 // static kernel carries none of the plastic path's registers). NARROW: the
 // static edge words are 16-bit (every projection's delay * stride + lane fits).
 template <typename T, bool PLASTIC, bool NARROW>
-__global__ __launch_bounds__(256) void k_window(WinArgs a) {
+__device__ __forceinline__ void window_body(WinArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   if (a.base_ptr) {
     const long long b = *a.base_ptr;
@@ -1323,13 +1325,13 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
 #pragma unroll
           for (int i = 0; i < kIT; ++i) {
             const int idx = base0 + tid + 256 * i, w = idx >> lgTS, l = idx & (TS - 1);
-            v[i] = ok[i] ? pois_value<T>(cc, cdf, guide, tl.start + l, a.n0 + w) : (T)0;
+            v[i] = ok[i] ? pois_value<T, false>(cc, cdf, guide, tl.start + l, a.n0 + w, nullptr) : (T)0;
           }
         } else {
 #pragma unroll
           for (int i = 0; i < kIT; ++i) {
             const int idx = base0 + tid + 256 * i, w = idx >> lgTS, l = idx & (TS - 1);
-            v[i] = ok[i] ? chan_value<T>(u, cc, w, a.n0 + w, l) : (T)0;
+            v[i] = ok[i] ? chan_value<T>(u, cc, c, w, a.n0 + w, l) : (T)0;
           }
         }
         if (cc.tail >= 0) {  // summed after a plastic channel: kept for the update loop
@@ -1430,6 +1432,22 @@ __global__ __launch_bounds__(256) void k_window(WinArgs a) {
         atomicOr(a.flags, FLAG_OVERFLOW);
     }
   }
+}
+
+template <typename T, bool PLASTIC, bool NARROW>
+__global__ __launch_bounds__(256) void k_window(WinArgs a) {
+  window_body<T, PLASTIC, NARROW>(a);
+}
+// float networks without plastic projections: four waves per SIMD (the
+// compiler keeps the body within 128 VGPRs; the ensemble's latency-bound
+// blocks need the occupancy)
+template <>
+__global__ __launch_bounds__(256, 4) void k_window<float, false, false>(WinArgs a) {
+  window_body<float, false, false>(a);
+}
+template <>
+__global__ __launch_bounds__(256, 4) void k_window<float, false, true>(WinArgs a) {
+  window_body<float, false, true>(a);
 }
 
 // ============================================================== host side
