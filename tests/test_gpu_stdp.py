@@ -95,6 +95,53 @@ def test_stdp_chunks_and_continue():
                                orc.weights(desc.proj_index[id(proj)], proj.n_edges), rtol=1e-12)
 
 
+def _two_plastic_net():
+    sim = Simulator(dt=0.1 * u.ms)
+    kw = dict(V_initializer=Uniform(-70., -56., seed=5))
+    e = sim.create(iaf_psc_exp, 300, **kw)
+    i = sim.create(iaf_psc_exp, 75, **kw)
+    noise = sim.create(poisson_generator, rate=14000. * u.Hz)
+    sim.connect(noise, e + i, weight=40., delay=1.0)
+    see = stdp_synapse(weight=40., tau_plus=20., tau_minus=20., lambda_=0.05, alpha=1.1,
+                       mu_plus=0.0, mu_minus=1.0, Wmax=120.)
+    sie = stdp_synapse(weight=-160., tau_plus=15., tau_minus=30., lambda_=0.03, alpha=0.9,
+                       mu_plus=1.0, mu_minus=1.0, Wmax=-480.)
+    pee = sim.connect(e, e, rule=fixed_indegree(20), synapse=see, delay=2.0, seed=3)
+    pie = sim.connect(i, e, rule=fixed_indegree(5), synapse=sie, delay=1.0, seed=4)
+    sim.connect(e, i, weight=40., delay=1.5, rule=fixed_indegree(20), seed=5)
+    sim.connect(i, i, weight=-160., delay=1.0, rule=fixed_indegree(5), seed=6)
+    sim.connect(e, e, weight=10., delay=2.0, rule=fixed_indegree(10), seed=7)   # static, after both
+    sim.connect(e, sim.create(spike_recorder))
+    sim.connect(i, sim.create(spike_recorder))
+    vm = sim.create(voltmeter)
+    sim.connect(vm, e[:4])
+    return sim, (pee, pie)
+
+
+def test_two_plastic_projections_into_one_population_and_a_static_tail():
+    """Excitatory and inhibitory stdp_synapse projections both onto E, each
+    with its own traces and time constants (the inhibitory one with negative
+    weights and Wmax); a static projection registered after them, whose input
+    joins the sum after both plastic channels, in channel order; E both pre
+    and post of a plastic projection. Raster, V_m and both weight sets equal
+    the oracle."""
+    sim, projs = _two_plastic_net()
+    desc = sim.build_desc()
+    eng = sim._ensure_engine()
+    assert eng.window() == 11
+    eng.reset()
+    vm_g, (g_g, s_g) = eng.run(2000)
+    orc = oracle.OracleNet(desc)
+    vm_o, g_o, s_o = orc.run(2000)
+    assert len(g_o) > 200
+    assert np.array_equal(g_g, g_o) and np.array_equal(s_g, s_o)
+    np.testing.assert_allclose(vm_g, vm_o, rtol=1e-12, atol=0)
+    for proj in projs:
+        w_o = orc.weights(desc.proj_index[id(proj)], proj.n_edges)
+        assert np.any(w_o != np.asarray(proj.weight, np.float64)), 'no plasticity happened'
+        np.testing.assert_allclose(sim.get_weights(proj), w_o, rtol=1e-12, atol=0)
+
+
 def test_stdp_runs_in_windows_of_d_min_plus_one_steps():
     """Plastic projections no longer shorten the window: with a 1 ms minimum
     delay every launch advances d_min + 1 = 11 steps."""
