@@ -256,10 +256,61 @@ __device__ __forceinline__ double nanmax(double a, double b) {
   return a > b ? a : b;
 }
 
-// One simulation step of aeif_cond_alpha for one neuron. Returns spike flag;
-// sets *unstable when the reference's guard would fire.
-__device__ __forceinline__ bool step_aeif(const double* __restrict__ P, double* s, int& r,
-                                          double wex_in, double win_in, double cur, int* unstable) {
+// The integration of one aeif_cond_alpha step in resumable pieces, so that
+// the lanes of a wave can each be at a different step of the window: a lane
+// whose step needs many substeps (a neuron crossing threshold needs hundreds)
+// then holds up only its own later steps, not every lane's. Each piece is a
+// slice of the reference's substep loop (utils.py:583-723) with the same
+// operations in the same order; step_aeif below chains them into one step.
+struct AeifLane {
+  AeifY y, k0;  // state; k0 = f(y) of the coming substep
+  double h, tl, istim;
+  int it;
+  bool spiked;
+};
+
+__device__ __forceinline__ AeifR aeif_recips(const double* __restrict__ P) {
+  return AeifR{1.0 / P[C_DTSAFE], 1.0 / P[C_CM], 1.0 / P[C_TAUEX], 1.0 / P[C_TAUIN], 1.0 / P[C_TAUW]};
+}
+
+__device__ __forceinline__ void aeif_load(AeifLane& L, const double* s) {
+  L.y = AeifY{s[0], s[1], s[2], s[3], s[4], s[5]};
+  L.h = s[6];
+  L.istim = s[7];
+}
+
+__device__ __forceinline__ void aeif_store(const AeifLane& L, double* s) {
+  s[0] = L.y.V;
+  s[1] = L.y.dgex;
+  s[2] = L.y.gex;
+  s[3] = L.y.dgin;
+  s[4] = L.y.gin;
+  s[5] = L.y.w;
+  s[6] = L.h;
+  s[7] = L.istim;
+}
+
+// Start of a step (the caller then evaluates k0 = f(y)).
+__device__ __forceinline__ void aeif_begin(const double* __restrict__ P, AeifLane& L) {
+  const double minh = P[C_MINH];
+  L.h = L.h > minh ? L.h : minh;
+  L.tl = 0.0;
+  L.it = 0;
+  L.spiked = false;
+}
+
+// Whether the step's substep loop runs another substep.
+__device__ __forceinline__ bool aeif_more(const double* __restrict__ P, const AeifLane& L) {
+  return L.it < (int)P[C_MAXIT] && L.tl < P[C_DT];
+}
+
+// One substep: the five further stages from k0, the embedded error, accept
+// (state, instability guard, refractory clamp, spike/reset/adaptation) or
+// reject, the new step size. Returns whether it was accepted; after an
+// accepted substep k0 no longer matches y and the caller re-evaluates it
+// before the next substep.
+__device__ __forceinline__ bool aeif_substep(const double* __restrict__ P, const AeifR& R, AeifLane& L, int& r,
+                                             int* unstable) {
   const double A10 = 1.0 / 4.0;
   const double A20 = 3.0 / 32.0, A21 = 9.0 / 32.0;
   const double A30 = 1932.0 / 2197.0, A31 = -7200.0 / 2197.0, A32 = 7296.0 / 2197.0;
@@ -271,113 +322,120 @@ __device__ __forceinline__ bool step_aeif(const double* __restrict__ P, double* 
   const double E0 = 16.0 / 135.0 - 25.0 / 216.0, E2 = 6656.0 / 12825.0 - 1408.0 / 2565.0,
                E3 = 28561.0 / 56430.0 - 2197.0 / 4104.0, E4 = -9.0 / 50.0 - -1.0 / 5.0,
                E5 = 2.0 / 55.0 - 0.0;
-
-  AeifY y{s[0], s[1], s[2], s[3], s[4], s[5]};
-  double h = s[6];
-  const double istim = s[7];
   const double dt = P[C_DT], minh = P[C_MINH], atol = P[C_ATOL];
-  const AeifR R{1.0 / P[C_DTSAFE], 1.0 / P[C_CM], 1.0 / P[C_TAUEX], 1.0 / P[C_TAUIN], 1.0 / P[C_TAUW]};
   const int refn = (int)P[C_REFN];
-  bool spiked = false;
-  h = h > minh ? h : minh;
-  double tl = 0.0;
-  const int maxit = (int)P[C_MAXIT];
-  // k0 = f(y) of the coming substep: evaluated before the first one, and
-  // after each accepted substep (from the new state and refractory count,
-  // next to the step-size update, which it does not depend on); a rejected
-  // substep leaves y and r unchanged, so its k0 serves again. The values are
-  // those of evaluating k0 at the top of every substep.
-  AeifY k0;
-  aeif_rhs(P, R, y, r > 0, istim, k0);
-  for (int it = 0; it < maxit && tl < dt; ++it) {
-    {
-      const double rem = dt - tl;
-      const double hm = h < rem ? h : rem;
-      h = minh > hm ? minh : hm;
-    }
-    const bool refr = r > 0;
-    AeifY k1, k2, k3, k4, k5, t;
+  const double istim = L.istim;
+  const AeifY& y = L.y;
+  const AeifY& k0 = L.k0;
+  double h = L.h;
+  {
+    const double rem = dt - L.tl;
+    const double hm = h < rem ? h : rem;
+    h = minh > hm ? minh : hm;
+  }
+  const bool refr = r > 0;
+  AeifY k1, k2, k3, k4, k5, t;
 #define BS_STAGE1(f) t.f = y.f + h * (A10 * k0.f);
-    BS_STAGE1(V) BS_STAGE1(dgex) BS_STAGE1(gex) BS_STAGE1(dgin) BS_STAGE1(gin) BS_STAGE1(w)
-    aeif_rhs(P, R, t, refr, istim, k1);
+  BS_STAGE1(V) BS_STAGE1(dgex) BS_STAGE1(gex) BS_STAGE1(dgin) BS_STAGE1(gin) BS_STAGE1(w)
+  aeif_rhs(P, R, t, refr, istim, k1);
 #define BS_STAGE2(f) t.f = y.f + h * (A20 * k0.f + A21 * k1.f);
-    BS_STAGE2(V) BS_STAGE2(dgex) BS_STAGE2(gex) BS_STAGE2(dgin) BS_STAGE2(gin) BS_STAGE2(w)
-    aeif_rhs(P, R, t, refr, istim, k2);
+  BS_STAGE2(V) BS_STAGE2(dgex) BS_STAGE2(gex) BS_STAGE2(dgin) BS_STAGE2(gin) BS_STAGE2(w)
+  aeif_rhs(P, R, t, refr, istim, k2);
 #define BS_STAGE3(f) t.f = y.f + h * (A30 * k0.f + A31 * k1.f + A32 * k2.f);
-    BS_STAGE3(V) BS_STAGE3(dgex) BS_STAGE3(gex) BS_STAGE3(dgin) BS_STAGE3(gin) BS_STAGE3(w)
-    aeif_rhs(P, R, t, refr, istim, k3);
+  BS_STAGE3(V) BS_STAGE3(dgex) BS_STAGE3(gex) BS_STAGE3(dgin) BS_STAGE3(gin) BS_STAGE3(w)
+  aeif_rhs(P, R, t, refr, istim, k3);
 #define BS_STAGE4(f) t.f = y.f + h * (A40 * k0.f + A41 * k1.f + A42 * k2.f + A43 * k3.f);
-    BS_STAGE4(V) BS_STAGE4(dgex) BS_STAGE4(gex) BS_STAGE4(dgin) BS_STAGE4(gin) BS_STAGE4(w)
-    aeif_rhs(P, R, t, refr, istim, k4);
+  BS_STAGE4(V) BS_STAGE4(dgex) BS_STAGE4(gex) BS_STAGE4(dgin) BS_STAGE4(gin) BS_STAGE4(w)
+  aeif_rhs(P, R, t, refr, istim, k4);
 #define BS_STAGE5(f) t.f = y.f + h * (A50 * k0.f + A51 * k1.f + A52 * k2.f + A53 * k3.f + A54 * k4.f);
-    BS_STAGE5(V) BS_STAGE5(dgex) BS_STAGE5(gex) BS_STAGE5(dgin) BS_STAGE5(gin) BS_STAGE5(w)
-    aeif_rhs(P, R, t, refr, istim, k5);
-    AeifY yh, ye;
+  BS_STAGE5(V) BS_STAGE5(dgex) BS_STAGE5(gex) BS_STAGE5(dgin) BS_STAGE5(gin) BS_STAGE5(w)
+  aeif_rhs(P, R, t, refr, istim, k5);
+  AeifY yh, ye;
 #define BS_HIGH(f) yh.f = y.f + h * (B0 * k0.f + B2 * k2.f + B3 * k3.f + B4 * k4.f + B5 * k5.f);
-    BS_HIGH(V) BS_HIGH(dgex) BS_HIGH(gex) BS_HIGH(dgin) BS_HIGH(gin) BS_HIGH(w)
+  BS_HIGH(V) BS_HIGH(dgex) BS_HIGH(gex) BS_HIGH(dgin) BS_HIGH(gin) BS_HIGH(w)
 #define BS_ERR(f) ye.f = h * (E0 * k0.f + E2 * k2.f + E3 * k3.f + E4 * k4.f + E5 * k5.f);
-    BS_ERR(V) BS_ERR(dgex) BS_ERR(gex) BS_ERR(dgin) BS_ERR(gin) BS_ERR(w)
-    // scaled error |e| / (atol + 0 * max(|y0|, |y1|)), max over the six leaves
-    double err;
-    {
+  BS_ERR(V) BS_ERR(dgex) BS_ERR(gex) BS_ERR(dgin) BS_ERR(gin) BS_ERR(w)
+  // scaled error |e| / (atol + 0 * max(|y0|, |y1|)), max over the six leaves
+  double err;
+  {
 #define BS_SE(f) (__builtin_fabs(ye.f) / (atol + 0.0 * nanmax(__builtin_fabs(y.f), __builtin_fabs(yh.f))))
-      err = BS_SE(V);
-      err = nanmax(err, BS_SE(dgex));
-      err = nanmax(err, BS_SE(gex));
-      err = nanmax(err, BS_SE(dgin));
-      err = nanmax(err, BS_SE(gin));
-      err = nanmax(err, BS_SE(w));
-    }
-    if (err != err) err = __builtin_inf();
-    const bool accept = (err <= 1.0) || (h <= minh);
-    if (accept) {
-      y = yh;
-      tl = tl + h;
-      // event_fn: instability guard, refractory clamp, spike/reset/adaptation
-      if (y.V < -1e3 || y.w < -1e6 || y.w > 1e6) *unstable = 1;
-      if (r > 0) y.V = P[C_VRESET];
-      if (r <= 0 && y.V >= P[C_VPEAKDET]) {
-        spiked = true;
-        y.V = P[C_VRESET];
-        y.w = y.w + P[C_B];
-        if (refn > 0) r = refn + 1;
-      }
-    }
-    if (accept && tl < dt) aeif_rhs(P, R, y, r > 0, istim, k0);
-    const double errsg = err > 1e-30 ? err : 1e-30;
-    const double inv = 1.0 / errsg;
-    // one pow for both branches (err^(-1/5) after acceptance, err^(-1/4)
-    // after rejection), so a wave with both kinds of lanes runs it once
-    const double pw = pow_p(inv, accept ? 0.2 : 0.25);
-    if (accept) {
-      double fa;
-      if (errsg <= 1e-30) {
-        fa = 5.0;
-      } else {
-        fa = 0.9 * pw;
-        fa = fa < 0.2 ? 0.2 : (fa > 5.0 ? 5.0 : fa);
-      }
-      const double hn = h * fa;
-      h = minh > hn ? minh : hn;
-    } else {
-      double fr = 0.9 * pw;
-      fr = fr < 0.2 ? 0.2 : (fr > 1.0 ? 1.0 : fr);
-      const double hn = h * fr;
-      h = minh > hn ? minh : hn;
+    err = BS_SE(V);
+    err = nanmax(err, BS_SE(dgex));
+    err = nanmax(err, BS_SE(gex));
+    err = nanmax(err, BS_SE(dgin));
+    err = nanmax(err, BS_SE(gin));
+    err = nanmax(err, BS_SE(w));
+  }
+  if (err != err) err = __builtin_inf();
+  const bool accept = (err <= 1.0) || (h <= minh);
+  if (accept) {
+    L.y = yh;
+    L.tl = L.tl + h;
+    // event_fn: instability guard, refractory clamp, spike/reset/adaptation
+    if (L.y.V < -1e3 || L.y.w < -1e6 || L.y.w > 1e6) *unstable = 1;
+    if (r > 0) L.y.V = P[C_VRESET];
+    if (r <= 0 && L.y.V >= P[C_VPEAKDET]) {
+      L.spiked = true;
+      L.y.V = P[C_VRESET];
+      L.y.w = L.y.w + P[C_B];
+      if (refn > 0) r = refn + 1;
     }
   }
+  const double errsg = err > 1e-30 ? err : 1e-30;
+  const double inv = 1.0 / errsg;
+  // one pow for both branches (err^(-1/5) after acceptance, err^(-1/4)
+  // after rejection), so a wave with both kinds of lanes runs it once
+  const double pw = pow_p(inv, accept ? 0.2 : 0.25);
+  if (accept) {
+    double fa;
+    if (errsg <= 1e-30) {
+      fa = 5.0;
+    } else {
+      fa = 0.9 * pw;
+      fa = fa < 0.2 ? 0.2 : (fa > 5.0 ? 5.0 : fa);
+    }
+    const double hn = h * fa;
+    L.h = minh > hn ? minh : hn;
+  } else {
+    double fr = 0.9 * pw;
+    fr = fr < 0.2 ? 0.2 : (fr > 1.0 ? 1.0 : fr);
+    const double hn = h * fr;
+    L.h = minh > hn ? minh : hn;
+  }
+  ++L.it;
+  return accept;
+}
+
+// End of a step: refractory count, the step's synaptic inputs into the
+// conductance derivatives, the current input latched for the next step.
+__device__ __forceinline__ void aeif_finish(const double* __restrict__ P, AeifLane& L, int& r, double wex_in,
+                                            double win_in, double cur) {
   if (r > 0) r = r - 1;
-  y.dgex = y.dgex + P[C_PSCEX] * wex_in;
-  y.dgin = y.dgin + P[C_PSCIN] * win_in;
-  s[0] = y.V;
-  s[1] = y.dgex;
-  s[2] = y.gex;
-  s[3] = y.dgin;
-  s[4] = y.gin;
-  s[5] = y.w;
-  s[6] = h;
-  s[7] = cur;  // I_stim for the next step: x + sum_current_inputs (x = 0)
-  return spiked;
+  L.y.dgex = L.y.dgex + P[C_PSCEX] * wex_in;
+  L.y.dgin = L.y.dgin + P[C_PSCIN] * win_in;
+  L.istim = cur;  // I_stim for the next step: x + sum_current_inputs (x = 0)
+}
+
+// One simulation step of aeif_cond_alpha for one neuron. Returns spike flag;
+// sets *unstable when the reference's guard would fire. k0 = f(y) is
+// evaluated before the first substep and after each accepted one that does
+// not end the step; a rejected substep leaves y and r unchanged, so its k0
+// serves again (the values are those of evaluating k0 at the top of every
+// substep).
+__device__ __forceinline__ bool step_aeif(const double* __restrict__ P, double* s, int& r,
+                                          double wex_in, double win_in, double cur, int* unstable) {
+  const AeifR R = aeif_recips(P);
+  AeifLane L;
+  aeif_load(L, s);
+  aeif_begin(P, L);
+  aeif_rhs(P, R, L.y, r > 0, L.istim, L.k0);
+  while (aeif_more(P, L)) {
+    const bool acc = aeif_substep(P, R, L, r, unstable);
+    if (acc && L.tl < P[C_DT]) aeif_rhs(P, R, L.y, r > 0, L.istim, L.k0);
+  }
+  aeif_finish(P, L, r, wex_in, win_in, cur);
+  aeif_store(L, s);
+  return L.spiked;
 }
 
 }  // namespace bs

@@ -489,37 +489,150 @@ __device__ __forceinline__ T chan_value(const UpdCtx<T>& u, const ChReg<T>& c, i
   return (T)__builtin_ldexp((double)q, -c.shift);
 }
 
-// The W-step loop of one lane: inputs summed in channel order, model step,
-// voltmeter tap, spike buffered in LDS. HAS_CUR: the population has current
-// inputs (row nrec of inp); without them the model sees a zero current.
-template <typename T, bool HAS_CUR, typename F>
-__device__ __forceinline__ void update_loop(const UpdCtx<T>& u, int ns, F stepf) {
-  const WinArgs& a = u.a;
-  T* st = (T*)a.state;
+// ---- a lane's neuron over the window -----------------------------------
+// run(wa, wb, in, out) advances the lane's neuron through window steps
+// wa .. wb - 1: in(w, a0, a1, cur) gives the summed inputs of step w (read
+// after the step's integration, before they enter the state), out(w, spike,
+// v) takes the step's outcome (v = V_m after it); both are called once per
+// step, in step order.
+
+// Models whose step is one straight computation, f(s, r, a0, a1, cur).
+template <typename T, typename F>
+struct StepLane {
   T s[BS_NSTATE];
-  int r = 0;
+  int r;
+  int ns;
+  const F& f;
+  __device__ __forceinline__ StepLane(int ns_, const F& f_) : r(0), ns(ns_), f(f_) {
 #pragma unroll
-  for (int k = 0; k < BS_NSTATE; ++k) s[k] = (T)0;
-  if (!u.active) return;
-  for (int k = 0; k < ns; ++k) s[k] = st[(long long)k * a.Ntot + u.gid];
-  r = a.ref[u.gid];
-  const int vmc = a.n_vm > 0 ? a.vm_col[u.gid] : -1;
-  for (int w = 0; w < a.W; ++w) {
-    const int step = a.n0 + w;
-    const T acc0 = u.inp[w * u.TS + u.tid];
-    const T acc1 = u.nrec > 1 ? u.inp[(u.Wm + w) * u.TS + u.tid] : (T)0;
-    T cur = (T)0;
-    if constexpr (HAS_CUR) cur = u.inp[(u.nrec * u.Wm + w) * u.TS + u.tid];
-    const bool spike = stepf(s, r, acc0, acc1, cur);
-    if (vmc >= 0) a.vm_buf[(long long)(step - a.vm_step0) * a.n_vm + vmc] = (double)s[0];
-    if (a.spk_hist) a.spk_hist[(long long)(step % a.Ls) * a.Ntot + u.gid] = spike ? 1 : 0;
-    if (spike) {
-      const int p = atomicAdd(&u.spn[w], 1);
-      u.spl[w * u.TS + p] = (unsigned short)u.tid;
+    for (int k = 0; k < BS_NSTATE; ++k) s[k] = (T)0;
+  }
+  __device__ __forceinline__ void load(const WinArgs& a, long long gid) {
+    const T* st = (const T*)a.state;
+#pragma unroll
+    for (int k = 0; k < BS_NSTATE; ++k)
+      if (k < ns) s[k] = st[(long long)k * a.Ntot + gid];
+    r = a.ref[gid];
+  }
+  __device__ __forceinline__ void store(const WinArgs& a, long long gid) const {
+    T* st = (T*)a.state;
+#pragma unroll
+    for (int k = 0; k < BS_NSTATE; ++k)
+      if (k < ns) st[(long long)k * a.Ntot + gid] = s[k];
+    a.ref[gid] = r;
+  }
+  template <typename IN, typename OUT>
+  __device__ __forceinline__ void run(int wa, int wb, IN&& in, OUT&& out) {
+    for (int w = wa; w < wb; ++w) {
+      T a0, a1, cur;
+      in(w, a0, a1, cur);
+      const bool spike = f(s, r, a0, a1, cur);
+      out(w, spike, (double)s[0]);
     }
   }
-  for (int k = 0; k < ns; ++k) st[(long long)k * a.Ntot + u.gid] = s[k];
-  a.ref[u.gid] = r;
+};
+
+// aeif_cond_alpha: each lane runs its own substeps and moves to its next step
+// as soon as its current one is integrated, so a wave takes as many substep
+// iterations as its slowest lane needs over the whole window, not, at every
+// step, as many as that step's slowest lane. Per lane the operations and
+// their order are those of step_aeif.
+struct AeifRun {
+  AeifLane L;
+  int r;
+  int unstable;
+  const double* P;
+  AeifR R;
+  __device__ __forceinline__ explicit AeifRun(const double* P_) : r(0), unstable(0), P(P_), R(aeif_recips(P_)) {}
+  __device__ __forceinline__ void load(const WinArgs& a, long long gid) {
+    const double* st = (const double*)a.state;
+    double s[BS_NSTATE];
+#pragma unroll
+    for (int k = 0; k < BS_NSTATE; ++k) s[k] = st[(long long)k * a.Ntot + gid];
+    aeif_load(L, s);
+    r = a.ref[gid];
+  }
+  __device__ __forceinline__ void store(const WinArgs& a, long long gid) const {
+    double* st = (double*)a.state;
+    double s[BS_NSTATE];
+    aeif_store(L, s);
+#pragma unroll
+    for (int k = 0; k < BS_NSTATE; ++k) st[(long long)k * a.Ntot + gid] = s[k];
+    a.ref[gid] = r;
+  }
+  template <typename IN, typename OUT>
+  __device__ __forceinline__ void run(int wa, int wb, IN&& in, OUT&& out) {
+    int w = wa;
+    bool live = wa < wb;
+    if (live) {
+      aeif_begin(P, L);
+      aeif_rhs(P, R, L.y, r > 0, L.istim, L.k0);
+    }
+#ifndef BS_AEIF_SYNC
+#define BS_AEIF_SYNC 0
+#endif
+    bool wait = false;
+    while (live) {
+      bool need = false;  // k0 is stale
+      if (!wait) {
+        if (aeif_more(P, L)) need = aeif_substep(P, R, L, r, &unstable) && L.tl < P[C_DT];
+        wait = !aeif_more(P, L);
+      }
+      bool go = wait;
+      if (BS_AEIF_SYNC > 0) {
+        const int nl = __popcll(__ballot(1)), nw = __popcll(__ballot(wait));
+        go = wait && (nw * BS_AEIF_SYNC >= nl);
+      }
+      if (go) {  // step w integrated
+        wait = false;
+        double a0, a1, cur;
+        in(w, a0, a1, cur);
+        aeif_finish(P, L, r, a0, a1, cur);
+        out(w, L.spiked, L.y.V);
+        need = ++w < wb;
+        if (need) aeif_begin(P, L);
+        else live = false;
+      }
+      if (need) aeif_rhs(P, R, L.y, r > 0, L.istim, L.k0);
+    }
+  }
+};
+
+// The window of one lane: inputs summed in channel order, model step,
+// voltmeter tap, spike buffered in LDS. HAS_CUR: the population has current
+// inputs (row nrec of inp); without them the model sees a zero current.
+template <typename T, bool HAS_CUR, typename Lane>
+__device__ __forceinline__ void update_loop(const UpdCtx<T>& u, Lane& m) {
+  const WinArgs& a = u.a;
+  if (!u.active) return;
+  m.load(a, u.gid);
+  const int vmc = a.n_vm > 0 ? a.vm_col[u.gid] : -1;
+  unsigned smask = 0u;  // bit w: the neuron fired at step n0 + w
+  m.run(
+      0, a.W,
+      [&](int w, T& acc0, T& acc1, T& cur) __attribute__((always_inline)) {
+        acc0 = u.inp[w * u.TS + u.tid];
+        acc1 = u.nrec > 1 ? u.inp[(u.Wm + w) * u.TS + u.tid] : (T)0;
+        cur = (T)0;
+        if constexpr (HAS_CUR) cur = u.inp[(u.nrec * u.Wm + w) * u.TS + u.tid];
+      },
+      [&](int w, bool spike, double v) __attribute__((always_inline)) {
+        const int step = a.n0 + w;
+        if (vmc >= 0) a.vm_buf[(long long)(step - a.vm_step0) * a.n_vm + vmc] = v;
+        if (spike) {
+          smask |= 1u << w;
+          const int p = atomicAdd(&u.spn[w], 1);
+          u.spl[w * u.TS + p] = (unsigned short)u.tid;
+        }
+      });
+  if (a.spk_hist) {  // (read from the next window on)
+    const int hb = a.n0 % a.Ls;
+    for (int w = 0; w < a.W; ++w) {
+      const int hs = hb + w;
+      a.spk_hist[(long long)(hs < a.Ls ? hs : hs - a.Ls) * a.Ntot + u.gid] = (smask >> w) & 1u;
+    }
+  }
+  m.store(a, u.gid);
 }
 
 // ------------------------------------------------------------ plastic path
@@ -690,27 +803,20 @@ __device__ __forceinline__ double stdp_pending(const StdpDev& P, double w, unsig
 // order, and it writes its pre-trace history. After the loop the block
 // facilitates the incoming edges of the neurons that fired and that no later
 // arrival of the window has brought up to date.
-template <typename T, bool HAS_CUR, typename F>
-__device__ __forceinline__ void update_loop_pl(const UpdCtx<T>& u, const PlCtx& c, const DevPop& pop, int ns, F stepf) {
+template <typename T, bool HAS_CUR, typename Lane>
+__device__ __forceinline__ void update_loop_pl(const UpdCtx<T>& u, const PlCtx& c, const DevPop& pop, Lane& m) {
   const WinArgs& a = u.a;
   const int tid = u.tid, TS = u.TS;
   const bool act = u.active && tid < TS;
-  T* st = (T*)a.state;
-  T s[BS_NSTATE];
-  int r = 0;
-#pragma unroll
-  for (int k = 0; k < BS_NSTATE; ++k) s[k] = (T)0;
   double vtr[BS_MAX_PL], utr[BS_MAX_PL];
 #pragma unroll
   for (int q = 0; q < BS_MAX_PL; ++q) vtr[q] = utr[q] = 0.0;
   unsigned fmask = 0u;  // bit b: the neuron fired at step n0 + b - 1
+  unsigned smask = 0u;  // bit w: the neuron fired at step n0 + w
   const int il = u.start + tid;  // local index in the population
   int vmc = -1;
   if (act) {
-#pragma unroll
-    for (int k = 0; k < BS_NSTATE; ++k)
-      if (k < ns) s[k] = st[(long long)k * a.Ntot + u.gid];
-    r = a.ref[u.gid];
+    m.load(a, u.gid);
     vmc = a.n_vm > 0 ? a.vm_col[u.gid] : -1;
     if (a.n0 >= 1 && a.spk_hist[(long long)((a.n0 - 1) % a.Ls) * a.Ntot + u.gid]) fmask = 1u;
     // (constant indices into the trace arrays keep them in registers)
@@ -733,6 +839,37 @@ __device__ __forceinline__ void update_loop_pl(const UpdCtx<T>& u, const PlCtx& 
     block_exscan(c.poff, M, c.part, tid);
   }
   const T* tail = (const T*)c.tail;
+  // ring slots of window step 0 (step w: base + w, wrapped once; both rings
+  // are longer than a window), so that no step pays an integer division
+  const int hb = a.n0 % a.Ls;
+  int ub[BS_MAX_PL];
+#pragma unroll
+  for (int q = 0; q < BS_MAX_PL; ++q) ub[q] = q < c.nplpre ? (a.n0 + 1) % a.stdp[pop.plpre[q]].Lu : 0;
+  // the channels a step completes (plastic ones and the tail channels after
+  // them), as a mask over the first 64; any beyond are scanned
+  unsigned long long chm = 0ull;
+  if (c.npl > 0)
+    for (int ch = 0; ch < u.nch && ch < 64; ++ch)
+      if (u.chs[ch].plastic >= 0 || u.chs[ch].tail >= 0) chm |= 1ull << ch;
+  // per-step constants of the plastic projections, loaded once per window
+  double dmn[BS_MAX_PL], dpl[BS_MAX_PL];
+  double* Uq[BS_MAX_PL];  // this neuron's column of each pre-trace history
+  int Luq[BS_MAX_PL], npq[BS_MAX_PL];
+#pragma unroll
+  for (int q = 0; q < BS_MAX_PL; ++q) {
+    dmn[q] = q < c.npl ? a.stdp[pop.pl[q]].decay_minus : 0.0;
+    dpl[q] = 0.0;
+    Uq[q] = nullptr;
+    Luq[q] = 1;
+    npq[q] = 0;
+    if (q < c.nplpre) {
+      const StdpDev& P = a.stdp[pop.plpre[q]];
+      dpl[q] = P.decay_plus;
+      Uq[q] = P.U + il;
+      Luq[q] = P.Lu;
+      npq[q] = P.n_pre;
+    }
+  }
   for (int wa = 0; wa < a.W;) {
     int wb = a.W;
     if (c.npl > 0) {  // the longest run of steps whose records fit (at least one step)
@@ -747,84 +884,99 @@ __device__ __forceinline__ void update_loop_pl(const UpdCtx<T>& u, const PlCtx& 
       __syncthreads();
     }
     if (act) {
-      for (int w = wa; w < wb; ++w) {
-        const int step = a.n0 + w;
-        T acc0 = u.inp[w * TS + tid];
-        T acc1 = u.nrec > 1 ? u.inp[(u.Wm + w) * TS + tid] : (T)0;
-        T cur = (T)0;
-        if constexpr (HAS_CUR) cur = u.inp[(u.nrec * u.Wm + w) * TS + tid];
-        const bool post_fired = (fmask >> w) & 1u;
-        if (c.npl > 0) {
+      m.run(
+          wa, wb,
+          [&](int w, T& acc0, T& acc1, T& cur) __attribute__((always_inline)) {
+            const int step = a.n0 + w;
+            acc0 = u.inp[w * TS + tid];
+            acc1 = u.nrec > 1 ? u.inp[(u.Wm + w) * TS + tid] : (T)0;
+            cur = (T)0;
+            if constexpr (HAS_CUR) cur = u.inp[(u.nrec * u.Wm + w) * TS + tid];
+            const bool post_fired = (fmask >> w) & 1u;
+            if (c.npl > 0) {
 #pragma unroll
-          for (int p = 0; p < BS_MAX_PL; ++p)
-            if (p < c.npl) vtr[p] = vtr[p] * a.stdp[pop.pl[p]].decay_minus + (post_fired ? 1.0 : 0.0);
-          // the channels from the first plastic one of each receptor, in order
-          for (int ch = 0; ch < u.nch; ++ch) {
-            const DevChan& dc = u.chs[ch];
-            T v;
-            if (dc.plastic >= 0) {
-              const int p = dc.plastic;
-              const StdpDev& P = a.stdp[pop.pl[p]];
-              double vt = vtr[0];
-#pragma unroll
-              for (int q = 1; q < BS_MAX_PL; ++q)
-                if (q == p) vt = vtr[q];
-              const int key = (w * c.npl + p) * TS + tid;
-              const int b = key == wa * c.S ? 0 : min(c.poff[key - 1], c.cap);
-              const int e = min(c.poff[key], c.cap);
-              long long dep = 0;
-              for (int q = b; q < e; ++q) {
-                const uint32_t k = c.rk[q];
-                double wv = P.w[k];
-                const int la = P.la[k];
-                const int i = P.e_pre[k];
-                const int d = (int)(P.ew[k] >> 8);
-                // facilitations since the edge's last arrival (or the window start)
-                const int lo = la > a.n0 - 1 ? la : a.n0 - 1;
-                const unsigned long long m =
-                    (unsigned long long)fmask & ((1ull << w) - 1ull) & ~((1ull << (lo - a.n0 + 1)) - 1ull);
-                wv = stdp_pending(P, wv, m, a.n0, i, d);
-                if (post_fired) wv = stdp_facilitate(P, wv, c.rtr[q] - 1.0);
-                wv = stdp_depress(P, wv, vt - (post_fired ? 1.0 : 0.0));
-                P.w[k] = wv;
-                P.la[k] = step;
-                dep += (long long)__builtin_rint(__builtin_ldexp(wv, P.shift));
-              }
-              v = (T)__builtin_ldexp((double)dep, -P.shift);
-            } else if (dc.tail >= 0) {
-              v = tail[(dc.tail * u.Wm + w) * TS + tid];
-            } else {
-              continue;
+              for (int p = 0; p < BS_MAX_PL; ++p)
+                if (p < c.npl) vtr[p] = vtr[p] * dmn[p] + (post_fired ? 1.0 : 0.0);
+              // the channels from the first plastic one of each receptor, in order
+              const auto chan = [&](int ch) __attribute__((always_inline)) {
+                const DevChan& dc = u.chs[ch];
+                T v;
+                if (dc.plastic >= 0) {
+                  const int p = dc.plastic;
+                  const StdpDev& P = a.stdp[pop.pl[p]];
+                  // every trace loaded, then a select on the values: a load
+                  // under each branch would be merged into one load at index
+                  // p, which moves the trace array to scratch memory
+                  static_assert(BS_MAX_PL == 4, "trace select below");
+                  const double v0 = vtr[0], v1 = vtr[1], v2 = vtr[2], v3 = vtr[3];
+                  const double vt = p == 0 ? v0 : (p == 1 ? v1 : (p == 2 ? v2 : v3));
+                  const int key = (w * c.npl + p) * TS + tid;
+                  const int b = key == wa * c.S ? 0 : min(c.poff[key - 1], c.cap);
+                  const int e = min(c.poff[key], c.cap);
+                  long long dep = 0;
+                  for (int q = b; q < e; ++q) {
+                    const uint32_t k = c.rk[q];
+                    double wv = P.w[k];
+                    const int la = P.la[k];
+                    const int i = P.e_pre[k];
+                    const int d = (int)(P.ew[k] >> 8);
+                    // facilitations since the edge's last arrival (or the window start)
+                    const int lo = la > a.n0 - 1 ? la : a.n0 - 1;
+                    const unsigned long long mk =
+                        (unsigned long long)fmask & ((1ull << w) - 1ull) & ~((1ull << (lo - a.n0 + 1)) - 1ull);
+                    wv = stdp_pending(P, wv, mk, a.n0, i, d);
+                    if (post_fired) wv = stdp_facilitate(P, wv, c.rtr[q] - 1.0);
+                    wv = stdp_depress(P, wv, vt - (post_fired ? 1.0 : 0.0));
+                    P.w[k] = wv;
+                    P.la[k] = step;
+                    dep += (long long)__builtin_rint(__builtin_ldexp(wv, P.shift));
+                  }
+                  v = (T)__builtin_ldexp((double)dep, -P.shift);
+                } else if (dc.tail >= 0) {
+                  v = tail[(dc.tail * u.Wm + w) * TS + tid];
+                } else {
+                  return;
+                }
+                if (dc.receptor == 0) acc0 = acc0 + v;
+                else acc1 = acc1 + v;
+              };
+              for (unsigned long long mm = chm; mm; mm &= mm - 1ull) chan(__builtin_ctzll(mm));
+              for (int ch = 64; ch < u.nch; ++ch) chan(ch);
             }
-            if (dc.receptor == 0) acc0 = acc0 + v;
-            else acc1 = acc1 + v;
-          }
-        }
-        const bool spike = stepf(s, r, acc0, acc1, cur);
-        if (vmc >= 0) a.vm_buf[(long long)(step - a.vm_step0) * a.n_vm + vmc] = (double)s[0];
-        a.spk_hist[(long long)(step % a.Ls) * a.Ntot + u.gid] = spike ? 1 : 0;
-#pragma unroll
-        for (int q = 0; q < BS_MAX_PL; ++q)
-          if (q < c.nplpre) {
-            const StdpDev& P = a.stdp[pop.plpre[q]];
-            utr[q] = utr[q] * P.decay_plus + (spike ? 1.0 : 0.0);
-            P.U[(long long)((step + 1) % P.Lu) * P.n_pre + il] = utr[q];
-          }
-        if (spike) {
-          if (w + 1 < a.W) fmask |= 1u << (w + 1);
-          const int pp = atomicAdd(&u.spn[w], 1);
-          u.spl[w * TS + pp] = (unsigned short)tid;
-        }
-      }
+          },
+          [&](int w, bool spike, double vm) __attribute__((always_inline)) {
+            const int step = a.n0 + w;
+            if (vmc >= 0) a.vm_buf[(long long)(step - a.vm_step0) * a.n_vm + vmc] = vm;
+            if (spike) {
+              smask |= 1u << w;
+              if (w + 1 < a.W) fmask |= 1u << (w + 1);
+              const int pp = atomicAdd(&u.spn[w], 1);
+              u.spl[w * TS + pp] = (unsigned short)tid;
+            }
+          });
     }
     if (c.npl > 0) __syncthreads();
     wa = wb;
   }
   if (act) {
+    // the window's spike history and pre-trace history, written after the
+    // steps (nothing in this window reads them: arrivals read pre traces of
+    // steps up to n0, and the spike history is read from the next window on)
+    for (int w = 0; w < a.W; ++w) {
+      const int hs = hb + w;
+      a.spk_hist[(long long)(hs < a.Ls ? hs : hs - a.Ls) * a.Ntot + u.gid] = (smask >> w) & 1u;
+    }
 #pragma unroll
-    for (int k = 0; k < BS_NSTATE; ++k)
-      if (k < ns) st[(long long)k * a.Ntot + u.gid] = s[k];
-    a.ref[u.gid] = r;
+    for (int q = 0; q < BS_MAX_PL; ++q)
+      if (q < c.nplpre) {
+        double t = utr[q];
+        for (int w = 0; w < a.W; ++w) {
+          t = t * dpl[q] + (((smask >> w) & 1u) ? 1.0 : 0.0);
+          const int us = ub[q] + w;  // slot of step n0 + w + 1
+          Uq[q][(long long)(us < Luq[q] ? us : us - Luq[q]) * npq[q]] = t;
+        }
+      }
+    m.store(a, u.gid);
 #pragma unroll
     for (int p = 0; p < BS_MAX_PL; ++p)
       if (p < c.npl) a.stdp[pop.pl[p]].V[il] = vtr[p];
@@ -856,13 +1008,13 @@ __device__ __forceinline__ void update_loop_pl(const UpdCtx<T>& u, const PlCtx& 
   }
 }
 
-template <typename T, bool HAS_CUR, typename F>
+template <typename T, bool HAS_CUR, typename Lane>
 __device__ __forceinline__ void run_update(const UpdCtx<T>& u, bool plastic, const PlCtx& pl, const DevPop& pop,
-                                           int ns, F stepf) {
+                                           Lane& m) {
   if (plastic)
-    update_loop_pl<T, HAS_CUR>(u, pl, pop, ns, stepf);
+    update_loop_pl<T, HAS_CUR>(u, pl, pop, m);
   else if (u.tid < u.TS)
-    update_loop<T, HAS_CUR>(u, ns, stepf);
+    update_loop<T, HAS_CUR>(u, m);
 }
 
 // Producer blocks: draw the Poisson counts of steps n1 .. n1 + Wmax - 1 for
@@ -1373,34 +1525,33 @@ __device__ __forceinline__ void window_body(WinArgs a) {
     UpdCtx<T> u{a, cnt, fxa, spl, spn, chs, tab, tables, TS, ncw, nfx, nch, tl.start, tid,
                 tid < tl.count, (long long)gpop.gid0 + tl.start + tid, inp, nrec, Wm, pc, tl.ncur, nullptr};
     // each model with and without current inputs (a compile-time branch)
-#define BS_UPDATE(CUR)                                                                              \
-    switch (gpop.model) {                                                                          \
-      case BS_IAF_PSC_DELTA: {                                                                     \
-        const DeltaP<T> mp = load_delta<T>(P);                                                     \
-        run_update<T, CUR>(u, plp, plc, gpop, gpop.n_state, [&](T* s, int& r, T a0, T, T cur) {               \
-          return step_delta<T, CUR>(mp, s, r, a0, cur);                                            \
-        });                                                                                        \
-      } break;                                                                                     \
-      case BS_IAF_PSC_EXP: {                                                                       \
-        const ExpP<T> mp = load_exp<T>(P);                                                         \
-        run_update<T, CUR>(u, plp, plc, gpop, gpop.n_state,                                                   \
-                            [&](T* s, int& r, T a0, T, T cur) { return step_exp<T>(mp, s, r, a0, cur); }); \
-      } break;                                                                                     \
-      case BS_IAF_PSC_ALPHA: {                                                                     \
-        const AlphaP<T> mp = load_alpha<T>(P);                                                     \
-        run_update<T, CUR>(u, plp, plc, gpop, gpop.n_state,                                                   \
-                            [&](T* s, int& r, T a0, T, T cur) { return step_alpha<T>(mp, s, r, a0, cur); }); \
-      } break;                                                                                     \
-      case BS_AEIF_COND_ALPHA: {                                                                   \
-        if constexpr (std::is_same<T, double>::value) {                                            \
-          run_update<T, CUR>(u, plp, plc, gpop, gpop.n_state, [&](T* s, int& r, T a0, T a1, T cur) {           \
-            int unstable = 0;                                                                      \
-            const bool sp = step_aeif(P, s, r, a0, a1, cur, &unstable);                            \
-            if (unstable) atomicOr(a.flags, FLAG_UNSTABLE);                                        \
-            return sp;                                                                             \
-          });                                                                                      \
-        }                                                                                          \
-      } break;                                                                                     \
+#define BS_UPDATE(CUR)                                                                             \
+    switch (gpop.model) {                                                                         \
+      case BS_IAF_PSC_DELTA: {                                                                    \
+        const DeltaP<T> mp = load_delta<T>(P);                                                    \
+        const auto f = [&](T* s, int& r, T a0, T, T cur) { return step_delta<T, CUR>(mp, s, r, a0, cur); }; \
+        StepLane<T, decltype(f)> m(gpop.n_state, f);                                              \
+        run_update<T, CUR>(u, plp, plc, gpop, m);                                                 \
+      } break;                                                                                    \
+      case BS_IAF_PSC_EXP: {                                                                      \
+        const ExpP<T> mp = load_exp<T>(P);                                                        \
+        const auto f = [&](T* s, int& r, T a0, T, T cur) { return step_exp<T>(mp, s, r, a0, cur); }; \
+        StepLane<T, decltype(f)> m(gpop.n_state, f);                                              \
+        run_update<T, CUR>(u, plp, plc, gpop, m);                                                 \
+      } break;                                                                                    \
+      case BS_IAF_PSC_ALPHA: {                                                                    \
+        const AlphaP<T> mp = load_alpha<T>(P);                                                    \
+        const auto f = [&](T* s, int& r, T a0, T, T cur) { return step_alpha<T>(mp, s, r, a0, cur); }; \
+        StepLane<T, decltype(f)> m(gpop.n_state, f);                                              \
+        run_update<T, CUR>(u, plp, plc, gpop, m);                                                 \
+      } break;                                                                                    \
+      case BS_AEIF_COND_ALPHA: {                                                                  \
+        if constexpr (std::is_same<T, double>::value) {                                           \
+          AeifRun m(P);                                                                           \
+          run_update<T, CUR>(u, plp, plc, gpop, m);                                               \
+          if (m.unstable) atomicOr(a.flags, FLAG_UNSTABLE);                                       \
+        }                                                                                         \
+      } break;                                                                                    \
     }
     if (tl.ncur) {
       BS_UPDATE(true)
