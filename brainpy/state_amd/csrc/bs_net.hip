@@ -568,23 +568,12 @@ struct AeifRun {
       aeif_begin(P, L);
       aeif_rhs(P, R, L.y, r > 0, L.istim, L.k0);
     }
-#ifndef BS_AEIF_SYNC
-#define BS_AEIF_SYNC 0
-#endif
-    bool wait = false;
+    // (finishing a step as soon as it is integrated measured faster than
+    // holding finished lanes until a quarter or half of the wave is done)
     while (live) {
       bool need = false;  // k0 is stale
-      if (!wait) {
-        if (aeif_more(P, L)) need = aeif_substep(P, R, L, r, &unstable) && L.tl < P[C_DT];
-        wait = !aeif_more(P, L);
-      }
-      bool go = wait;
-      if (BS_AEIF_SYNC > 0) {
-        const int nl = __popcll(__ballot(1)), nw = __popcll(__ballot(wait));
-        go = wait && (nw * BS_AEIF_SYNC >= nl);
-      }
-      if (go) {  // step w integrated
-        wait = false;
+      if (aeif_more(P, L)) need = aeif_substep(P, R, L, r, &unstable) && L.tl < P[C_DT];
+      if (!aeif_more(P, L)) {  // step w integrated
         double a0, a1, cur;
         in(w, a0, a1, cur);
         aeif_finish(P, L, r, a0, a1, cur);
