@@ -28,7 +28,7 @@ def source_hash() -> str:
 # -packed-fp32-ops off: no packed FP32 instructions are generated.
 FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared', '-ffp-contract=off',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
-# rocBLAS: the dense transposed-delivery GEMM of the BPTT backward pass
+# the HIP runtime's directory, for the loader (no other ROCm library is linked)
 LIBS = ['-L/opt/rocm/lib', '-Wl,-rpath,/opt/rocm/lib']
 
 
