@@ -161,6 +161,20 @@ def test_stdp_sub_windows_equal_the_oracle(records):
     _compare(sim, proj, 1500)
 
 
+@pytest.mark.parametrize('tile,records', [(32, 64), (64, 128)])
+def test_stdp_aeif_sub_windows_equal_the_oracle(tile, records):
+    """aeif_cond_alpha lanes move through a window independently (each lane
+    takes its next step once its own substeps for the current one are done);
+    with the window's plastic arrivals split into sub-windows, the lanes meet
+    again at every sub-window's end. Raster, V_m and weights equal the
+    oracle. (This network's E tiles see up to 37 / 68 plastic arrivals in one
+    step and 170 / 320 in one 11-step window at 32 / 64 neurons per tile.)"""
+    sim, proj = _plastic_net(model=aeif_cond_alpha, n=300, delay=('uniform', 1.0, 5.0))
+    sim.tile_size = tile
+    sim.plastic_records = records
+    _compare(sim, proj, 1500)
+
+
 def test_stdp_record_capacity_error_is_reported():
     """More plastic arrivals in one step than a block holds is an error the
     run reports (not a silent drop)."""
