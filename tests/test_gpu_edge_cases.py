@@ -7,7 +7,9 @@
   inputs phase reads the table from global memory);
 * delays far beyond the window (up to 40 ms): arrivals past the LDS horizon
   go through the arrival ring, with 16-bit and with 32-bit edge words;
-* only some populations recorded, runs of 0 and 1 steps, and a silent network.
+* only some populations recorded, runs of 0 and 1 steps, and a silent network;
+* aeif_cond_alpha's instability guard: the run reports it as the oracle does,
+  and a run below the guard's threshold equals the oracle.
 """
 import numpy as np
 import pytest
@@ -148,3 +150,17 @@ def test_silent_network():
     sim.connect(vm, a[:2])
     g = _both(sim, 1000)
     assert len(g) == 0
+
+
+def test_aeif_instability_is_reported_like_the_oracle():
+    """A -1e7 pA current drives one aeif_cond_alpha neuron past -1e3 mV in its
+    first step with the current: the run fails with the reference's message
+    (aeif_cond_alpha.py:703-706); at -500 pA it runs and equals the oracle."""
+    from brainpy.state_amd._lib import EngineError
+    from test_aeif_unstable import unstable_net
+    sim = unstable_net(-1.0e7)   # (the simulator owns the engine: keep it)
+    eng = sim._ensure_engine()
+    eng.reset()
+    with pytest.raises(EngineError, match='Numerical instability in aeif_cond_alpha dynamics'):
+        eng.run(40)
+    _both(unstable_net(-500.0), 40)
