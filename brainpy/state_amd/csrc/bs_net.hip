@@ -1415,7 +1415,8 @@ __device__ __forceinline__ void window_body(WinArgs a) {
     __shared__ double schv[32];  // a SCHEDULE channel's value per window step (Wmax <= 32)
     UpdCtx<T> u{a, cnt, fxa, spl, spn, chs, tab, tables, TS, ncw, nfx, nch, tl.start, tid,
                 tid < tl.count, (long long)gpop.gid0 + tl.start + tid, inp, nrec, Wm, pc, tl.ncur, schv};
-    constexpr int kIT = 4;
+    // (two items per thread in float: fewer registers, more waves per SIMD)
+    constexpr int kIT = sizeof(T) == 4 ? 2 : 4;
     const int nit = W * TS;
     // current inputs are summed apart from the delta inputs, starting from
     // I_e + x (x = 0) for iaf_psc_delta and from x = 0 for the latching models
@@ -1578,15 +1579,17 @@ template <typename T, bool PLASTIC, bool NARROW>
 __global__ __launch_bounds__(256) void k_window(WinArgs a) {
   window_body<T, PLASTIC, NARROW>(a);
 }
-// float networks without plastic projections: four waves per SIMD (the
-// compiler keeps the body within 128 VGPRs; the ensemble's latency-bound
-// blocks need the occupancy)
+// float networks without plastic projections: six waves per SIMD (the body
+// fits in 80 VGPRs with two inputs-phase items per thread; the ensemble's
+// latency-bound blocks gain from every extra wave: 4 -> 5 -> 6 waves took a
+// 256-network window from 648 to 589 to 533 us, 7 waves with one item per
+// thread 579 us)
 template <>
-__global__ __launch_bounds__(256, 4) void k_window<float, false, false>(WinArgs a) {
+__global__ __launch_bounds__(256, 6) void k_window<float, false, false>(WinArgs a) {
   window_body<float, false, false>(a);
 }
 template <>
-__global__ __launch_bounds__(256, 4) void k_window<float, false, true>(WinArgs a) {
+__global__ __launch_bounds__(256, 6) void k_window<float, false, true>(WinArgs a) {
   window_body<float, false, true>(a);
 }
 

@@ -27,10 +27,10 @@ def test_no_kernel_uses_scratch():
     bad = {k: v['ScratchSize'] for k, v in usage.items() if v.get('ScratchSize', 0) != 0}
     assert not bad, f'kernels with scratch memory: {bad}'
     # the window kernel keeps two waves per SIMD (VGPR budget 256), and its
-    # float instantiations for static networks four (the ensemble's
-    # latency-bound blocks lose ~20 % at three)
+    # float instantiations for static networks six (the ensemble's
+    # latency-bound blocks lose ~10 % per wave fewer)
     for k, v in usage.items():
         if 'k_window' in k:
             assert v['Occupancy'] >= 2, (k, v)
         if 'k_windowIfLb0E' in k:
-            assert v['Occupancy'] >= 4, (k, v)
+            assert v['Occupancy'] >= 6, (k, v)
