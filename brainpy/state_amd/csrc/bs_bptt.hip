@@ -757,10 +757,12 @@ __global__ __launch_bounds__(kTile) void k_bptt_wgrad(WgArgs a) {
       }
       __syncthreads();
       const int m = nev;
-      for (int k = 0; k < m; k += 4) {
-        float v[4][16];
+      // two event rows in flight per pass (87 VGPRs, five waves per SIMD;
+      // four rows took 119 VGPRs and four waves, 0.4 ms more per gradient step)
+      for (int k = 0; k < m; k += 2) {
+        float v[2][16];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 2; ++u) {
           const float* rp = nullptr;
           if (k + u < m) {
             const long long e2 = base + ev[k + u];
@@ -775,7 +777,7 @@ __global__ __launch_bounds__(kTile) void k_bptt_wgrad(WgArgs a) {
           }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 2; ++u) {
           if (k + u < m) {
             const float f = evs[k + u];
 #pragma unroll
