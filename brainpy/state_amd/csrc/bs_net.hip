@@ -1660,6 +1660,9 @@ struct bs_net {
   int* h_prev_p = nullptr;
   unsigned long long* h_rec = nullptr;
   size_t h_rec_cap = 0;
+  // record buffers outgrown during a run: hipHostFree waits for the whole
+  // device, so they are freed after the run's last chunk, not between chunks
+  std::vector<void*> h_rec_old;
   double* h_vm_p = nullptr;                    // [2][vm_cap_steps * n_vm_dev]
   size_t h_vm_cap = 0;
   int* d_flags = nullptr;
@@ -1763,6 +1766,7 @@ static void free_net(bs_net* net) {
     if (p) (void)hipHostFree(p);
   };
   FH(net->h_cnt); FH(net->h_flags); FH(net->h_base_p); FH(net->h_prev_p); FH(net->h_rec); FH(net->h_vm_p);
+  for (void* p : net->h_rec_old) FH(p);
   if (net->stream2) (void)hipStreamDestroy(net->stream2);
   if (net->stream) (void)hipStreamDestroy(net->stream);
   delete net;
@@ -3280,8 +3284,11 @@ static int drain_chunk(bs_net* net, const ChunkJob& j, bool pin_vm, double* vm_o
   const unsigned long long cnt = net->h_cnt[j.par];
   if (cnt) {
     if (cnt > net->h_rec_cap) {
-      const size_t want = std::max<size_t>((size_t)cnt, 2 * net->h_rec_cap);
-      if (net->h_rec) HIPCHK(hipHostFree(net->h_rec));
+      // twice the count: a chunk a little busier than the largest so far
+      // does not grow the buffer again (each growth pins new host memory)
+      const size_t want = std::min<size_t>(std::max<size_t>(2 * (size_t)cnt, 2 * net->h_rec_cap),
+                                           (size_t)std::max<long long>(net->rec_cap, (long long)cnt));
+      if (net->h_rec) net->h_rec_old.push_back(net->h_rec);  // freed when the run ends
       net->h_rec = nullptr;
       net->h_rec_cap = 0;
       HIPCHK(hipHostMalloc((void**)&net->h_rec, sizeof(unsigned long long) * want, hipHostMallocDefault));
@@ -3491,6 +3498,8 @@ int bs_net_run(bs_net* net, int64_t n_steps, double* vm_out) {
   for (hipEvent_t e : evs) (void)hipEventDestroy(e);
   (void)hipEventDestroy(ev_all0);
   (void)hipEventDestroy(ev_all1);
+  for (void* p : net->h_rec_old) (void)hipHostFree(p);  // the device is idle here
+  net->h_rec_old.clear();
   return rc;
 }
 
