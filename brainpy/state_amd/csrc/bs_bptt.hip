@@ -21,8 +21,9 @@
 //                                        segment being reversed (-1: refractory)
 //   ibar_ex/in     float [T][B][N]       gradient of the loss w.r.t. the input
 //                                        arriving at (step, trial, neuron)
-//   sbar           float [W][B][N]       gradient w.r.t. the spikes of a window:
+//   sbar           float [S][W][B][N]    gradient w.r.t. the spikes of a window:
 //                                        W_rec . ibar(n + d), one GEMM per window
+//                                        in S = kTdSplit partial sums over K
 //   ibar operands  [T][B][Kp] per E/I    the GEMM's copies of ibar, K padded to a
 //                                        multiple of 64 with zeros: bf16 hi and lo
 //                                        parts (default) or float32
@@ -466,23 +467,35 @@ __global__ __launch_bounds__(kTile) void k_bptt_prep_w(int N, int Kp, int mode, 
 // float32 values (v_mfma_f32_32x32x2f32). Each output is summed in one fixed
 // order, so results repeat bit for bit.
 //
-// Block: 4 waves (2 over m x 2 over i), tile 64 m x 128 i; a wave owns 32 m x
-// 64 i (two 32x32 accumulator tiles). K advances in stages of 32: every stage
-// of both operands is loaded (16-byte loads, 8 k per load) into registers one
-// stage ahead and stored into the other half of a double-buffered LDS area.
-// Blocks are dealt round-robin over the 8 XCDs (block b on XCD b % 8), so each
-// XCD is given one contiguous range of (i-tile, m-tile) pairs, i-tile major:
-// the blocks of an XCD share their weight rows and the window's ibar rows in
-// its L2, and HBM sees each operand byte about once per launch.
-constexpr int kTdBM = 64, kTdBI = 128, kTdKC = 32;
+// Block: 4 waves (2 over m x 2 over i), tile 128 m x 128 i; a wave owns 64 m x
+// 64 i (four 32x32 accumulator tiles), so each fragment read from LDS feeds two
+// MFMA tiles. K advances in stages of 32: every stage of both operands is
+// loaded (16-byte loads, 8 k per load) into registers one stage ahead and
+// stored into the other half of a double-buffered LDS area (64 KiB: two blocks
+// per CU). Blocks are dealt round-robin over the 8 XCDs (block b on XCD b % 8),
+// so each XCD is given one contiguous range of (K range, i-tile, m-tile)
+// triples: the blocks of an XCD share their weight rows and the window's ibar
+// rows of one K range in its L2.
+constexpr int kTdBM = 128, kTdBI = 128, kTdKC = 32;
+// K is split into kTdSplit ranges, one block per (tile, range), each writing its
+// own partial sum; the backward kernel adds the partials in range order. The
+// benchmark window (640 rows x 4096 presynaptic rows) has 165 tiles: in thirds
+// 495 blocks, about two per CU (in halves 330 blocks leave 74 CUs with two and
+// the rest with one; measured per gradient step: halves 21.9 ms, thirds 20.7,
+// quarters 22.7).
+#ifndef BS_TD_SPLIT
+#define BS_TD_SPLIT 3
+#endif
+constexpr int kTdSplit = BS_TD_SPLIT;
 
 struct TdArgs {
   const void* w0;         // mode 0: W hi parts, mode 1: W float32   [N][Kp]
   const void* w1;         // mode 0: W lo parts
   const void* b0[2];      // per part: IB hi parts / float32, rows m [M][Kp]
   const void* b1[2];      // per part: IB lo parts
-  float* out;             // [M][N]
-  int M, N, Kp, n_exc;
+  float* out;             // [nks][M][N] partial sums, partial s at out + s * pstride
+  size_t pstride;
+  int M, N, Kp, n_exc, nks;
   int nti0, nti1, ntm;    // i-tiles of the excitatory and inhibitory rows, m-tiles
 };
 
@@ -512,17 +525,24 @@ __global__ __launch_bounds__(256) void k_bptt_tdeliver(TdArgs a) {
   constexpr int ESZ = MODE == 0 ? 2 : 4;            // element bytes
   constexpr int STG = NA * (BI + BM) * KB;          // 16-byte slots per stage
   __shared__ uint4 lds[2 * STG];
-  const int total = (a.nti0 + a.nti1) * a.ntm;
+  const int tiles = (a.nti0 + a.nti1) * a.ntm;
+  const int total = tiles * a.nks;
   const int per = (total + 7) / 8;
-  const int L = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
-  if (L >= total) return;
+  const int L0 = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+  if (L0 >= total) return;
+  // K range major, so an XCD's blocks share one range of both operands
+  const int ks = L0 / tiles, L = L0 - ks * tiles;
   const int it = L / a.ntm, mt = L - it * a.ntm;
+  const int nk_all = a.Kp / KC;
+  const int kbeg = ks * nk_all / a.nks, kend = (ks + 1) * nk_all / a.nks;
+  float* const out = a.out + (size_t)ks * a.pstride;
   const int part = it < a.nti0 ? 0 : 1;
   const int i0 = part == 0 ? it * BI : a.n_exc + (it - a.nti0) * BI;
   const int i1 = part == 0 ? a.n_exc : a.N;
   const int m0 = mt * BM;
   const int tid = (int)threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int wm = wv & 1, wi = wv >> 1;
+  constexpr int MT = BM / 64, IT = BI / 64;  // 32x32 accumulator tiles per wave along m and i
   const unsigned char* W0 = (const unsigned char*)a.w0;
   const unsigned char* W1 = (const unsigned char*)a.w1;
   const unsigned char* B0 = (const unsigned char*)a.b0[part];
@@ -562,18 +582,22 @@ __global__ __launch_bounds__(256) void k_bptt_tdeliver(TdArgs a) {
         s_[NA * BI * KB + q * BM * KB + td_slot<KB>(r, g)] = td_mask(rb[q][j], mk);     \
     }                                                                                   \
   } while (0)
-  td_f32x16 acc[2];
+  td_f32x16 acc[MT][IT];
 #pragma unroll
-  for (int t = 0; t < 2; ++t)
+  for (int mt_ = 0; mt_ < MT; ++mt_)
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[t][r] = 0.f;
-  const int nk = a.Kp / KC;
-  TD_LOAD(0);
+    for (int t = 0; t < IT; ++t)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[mt_][t][r] = 0.f;
+  const int nk = kend - kbeg;
+  if (nk > 0) {
+  TD_LOAD(kbeg * KC);
   TD_STORE(lds);
+  }
   __syncthreads();
-  const int rb_ = wm * 32 + (lane & 31);
+  const int rb_ = wm * (BM / 2) + (lane & 31);
   for (int k = 0; k < nk; ++k) {
-    if (k + 1 < nk) TD_LOAD((k + 1) * KC);
+    if (k + 1 < nk) TD_LOAD((kbeg + k + 1) * KC);
     const uint4* st = lds + (k & 1) * STG;
     const uint4* sW = st;
     const uint4* sB = st + NA * BI * KB;
@@ -581,16 +605,23 @@ __global__ __launch_bounds__(256) void k_bptt_tdeliver(TdArgs a) {
 #pragma unroll
       for (int s2 = 0; s2 < KC / 16; ++s2) {
         const int g = 2 * s2 + (lane >> 5);
-        const td_bf16x8 ah = __builtin_bit_cast(td_bf16x8, sB[td_slot<KB>(rb_, g)]);
-        const td_bf16x8 al = __builtin_bit_cast(td_bf16x8, sB[BM * KB + td_slot<KB>(rb_, g)]);
+        td_bf16x8 ah[MT], al[MT];
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-          const int r = wi * 64 + t * 32 + (lane & 31);
+        for (int mt_ = 0; mt_ < MT; ++mt_) {
+          ah[mt_] = __builtin_bit_cast(td_bf16x8, sB[td_slot<KB>(rb_ + 32 * mt_, g)]);
+          al[mt_] = __builtin_bit_cast(td_bf16x8, sB[BM * KB + td_slot<KB>(rb_ + 32 * mt_, g)]);
+        }
+#pragma unroll
+        for (int t = 0; t < IT; ++t) {
+          const int r = wi * (BI / 2) + t * 32 + (lane & 31);
           const td_bf16x8 wh = __builtin_bit_cast(td_bf16x8, sW[td_slot<KB>(r, g)]);
           const td_bf16x8 wl = __builtin_bit_cast(td_bf16x8, sW[BI * KB + td_slot<KB>(r, g)]);
-          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, wh, acc[t], 0, 0, 0);
-          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, wl, acc[t], 0, 0, 0);
-          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, wh, acc[t], 0, 0, 0);
+#pragma unroll
+          for (int mt_ = 0; mt_ < MT; ++mt_) {
+            acc[mt_][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mt_], wh, acc[mt_][t], 0, 0, 0);
+            acc[mt_][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mt_], wl, acc[mt_][t], 0, 0, 0);
+            acc[mt_][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[mt_], wh, acc[mt_][t], 0, 0, 0);
+          }
         }
       }
     } else {
@@ -599,15 +630,20 @@ __global__ __launch_bounds__(256) void k_bptt_tdeliver(TdArgs a) {
 #pragma unroll
       for (int s2 = 0; s2 < KC / 8; ++s2) {
         const int g = 2 * s2 + (lane >> 5);
-        const float4 av = __builtin_bit_cast(float4, sB[td_slot<KB>(rb_, g)]);
+        float4 av[MT];
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-          const int r = wi * 64 + t * 32 + (lane & 31);
+        for (int mt_ = 0; mt_ < MT; ++mt_) av[mt_] = __builtin_bit_cast(float4, sB[td_slot<KB>(rb_ + 32 * mt_, g)]);
+#pragma unroll
+        for (int t = 0; t < IT; ++t) {
+          const int r = wi * (BI / 2) + t * 32 + (lane & 31);
           const float4 wv4 = __builtin_bit_cast(float4, sW[td_slot<KB>(r, g)]);
-          acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, wv4.x, acc[t], 0, 0, 0);
-          acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, wv4.y, acc[t], 0, 0, 0);
-          acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, wv4.z, acc[t], 0, 0, 0);
-          acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, wv4.w, acc[t], 0, 0, 0);
+#pragma unroll
+          for (int mt_ = 0; mt_ < MT; ++mt_) {
+            acc[mt_][t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[mt_].x, wv4.x, acc[mt_][t], 0, 0, 0);
+            acc[mt_][t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[mt_].y, wv4.y, acc[mt_][t], 0, 0, 0);
+            acc[mt_][t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[mt_].z, wv4.z, acc[mt_][t], 0, 0, 0);
+            acc[mt_][t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[mt_].w, wv4.w, acc[mt_][t], 0, 0, 0);
+          }
         }
       }
     }
@@ -619,14 +655,16 @@ __global__ __launch_bounds__(256) void k_bptt_tdeliver(TdArgs a) {
   // accumulator layout of a 32x32 tile: lane l holds column i = l % 32 and
   // rows m = 8 (r / 4) + 4 (l / 32) + r % 4 for r = 0..15
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int i = i0 + wi * 64 + t * 32 + (lane & 31);
+  for (int t = 0; t < IT; ++t) {
+    const int i = i0 + wi * (BI / 2) + t * 32 + (lane & 31);
     if (i >= i1) continue;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int m = m0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-      if (m < a.M) a.out[(size_t)m * a.N + i] = acc[t][r];
-    }
+    for (int mt_ = 0; mt_ < MT; ++mt_)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int m = m0 + wm * (BM / 2) + 32 * mt_ + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        if (m < a.M) out[(size_t)m * a.N + i] = acc[mt_][t][r];
+      }
   }
 }
 
@@ -637,7 +675,9 @@ struct BwdArgs {
   const unsigned long long* bm;
   const float* act;
   const float* vbuf;
-  const float* sbar;
+  const float* sbar;     // nsplit partial sums, partial s at sbar + s * pstride
+  size_t pstride;
+  int nsplit;
   const float* grate;
   const float* target;
   float mscale;
@@ -669,7 +709,10 @@ __global__ __launch_bounds__(kTile) void k_bptt_bwd(BwdArgs a) {
     const float ds = ref ? 0.f : act;
     const bool hard = (a.bm[((size_t)n * a.B + b) * a.NW + j / 64] >> (j & 63)) & 1ull;
     if (a.loss == BS_LOSS_MEMBRANE_MSE) gv = gv + a.mscale * 2.0f * (a.vbuf[o] - tgt);
-    const float sb = a.sbar[((size_t)k * a.B + b) * a.N + j] + g_rate;
+    const size_t so = ((size_t)k * a.B + b) * a.N + j;
+    float sb = a.sbar[so];
+    for (int s = 1; s < a.nsplit; ++s) sb = sb + a.sbar[(size_t)s * a.pstride + so];
+    sb = sb + g_rate;
     a.ibex[(size_t)n * BN + bj] = ge;
     a.ibin[(size_t)n * BN + bj] = gi;
     const size_t ro = ((size_t)n * a.B + b) * a.Kp + j;  // the GEMM's operand rows
@@ -1013,7 +1056,7 @@ int bs_bptt_create(const bs_bptt_desc* d, bs_bptt** out) {
   }
   BALLOC(h->lpart_blocks, kLossBlocks);
 
-  BALLOC(h->sbar, (size_t)h->W * BN);
+  BALLOC(h->sbar, (size_t)kTdSplit * h->W * BN);
   BALLOC(h->ay, BN);
   BALLOC(h->ae, BN);
   BALLOC(h->ai, BN);
@@ -1113,6 +1156,8 @@ int bs_bptt_backward(bs_bptt* h, float* g_in, float* g_rec) {
   ba.act = h->act;
   ba.vbuf = h->vbuf;
   ba.sbar = h->sbar;
+  ba.pstride = (size_t)h->W * BN;
+  ba.nsplit = kTdSplit;
   ba.grate = h->grate;
   ba.target = h->target;
   ba.mscale = (float)(1.0 / ((double)BN * T));
@@ -1132,6 +1177,8 @@ int bs_bptt_backward(bs_bptt* h, float* g_in, float* g_rec) {
   ta.w0 = h->d.gemm == 0 ? (const void*)h->wh : (const void*)h->wf;
   ta.w1 = h->wl;
   ta.out = h->sbar;
+  ta.pstride = (size_t)h->W * BN;
+  ta.nks = kTdSplit;
   ta.N = N;
   ta.Kp = h->Kp;
   ta.n_exc = nE;
@@ -1153,8 +1200,9 @@ int bs_bptt_backward(bs_bptt* h, float* g_in, float* g_rec) {
       // sbar rows k: spikes at w0 + k reach their targets at w0 + k + d
       const int kmax = std::max(0, std::min(nsteps, T - d - w0));
       if (kmax < nsteps)
-        BCHK(hipMemsetAsync(h->sbar + (size_t)kmax * BN, 0, sizeof(float) * (nsteps - kmax) * BN,
-                            h->stream));
+        for (int s = 0; s < kTdSplit; ++s)
+          BCHK(hipMemsetAsync(h->sbar + (size_t)s * h->W * BN + (size_t)kmax * BN, 0,
+                              sizeof(float) * (nsteps - kmax) * BN, h->stream));
       if (kmax > 0) {
         const int M = kmax * B;
         const size_t src = (size_t)(w0 + d) * B * h->Kp;  // first operand row: (step w0 + d, trial 0)
@@ -1164,7 +1212,7 @@ int bs_bptt_backward(bs_bptt* h, float* g_in, float* g_rec) {
         }
         ta.M = M;
         ta.ntm = (M + kTdBM - 1) / kTdBM;
-        const int tiles = (ta.nti0 + ta.nti1) * ta.ntm;
+        const int tiles = (ta.nti0 + ta.nti1) * ta.ntm * ta.nks;
         const unsigned tgrid = (unsigned)((tiles + 7) / 8 * 8);
         if (h->d.gemm == 0)
           hipLaunchKernelGGL(k_bptt_tdeliver<0>, dim3(tgrid), dim3(256), 0, h->stream, ta);
