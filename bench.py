@@ -141,7 +141,15 @@ def main():
         eng.run(win)
     log(f'settle {settle} + warm-up {args.warmup} x {win} steps done')
 
-    # ---- timed windows (bs_net_run returns after its stream has drained)
+    # ---- timed windows (bs_net_run returns after its stream has drained).
+    # The interpreter's objects from the imports and the build are collected
+    # and frozen first: a full collection over them (tens of ms with torch
+    # loaded) would otherwise land inside whichever timed window the
+    # allocation count happens to trigger it in.
+    import gc
+    gc.collect()
+    gc.freeze()
+    gc2_before = gc.get_stats()[2]['collections']
     torch.cuda.synchronize()
     walls, n_events, n_spikes = [], 0, 0
     t1 = time.perf_counter()
@@ -153,6 +161,8 @@ def main():
         n_spikes += len(gids)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t1
+    gc.unfreeze()
+    log(f'full collections during the timed windows: {gc.get_stats()[2]["collections"] - gc2_before}')
     sim_s = args.steps * win * sim.dt / 1000.0
     value = n_events / wall if wall > 0 else 0.0
     rate = n_spikes / 11250 / sim_s if sim_s > 0 else 0.0
