@@ -32,8 +32,11 @@ The JSON line carries:
   bptt         the differentiation workload (recurrent iaf_psc_exp, 4096
                neurons, 10 % trainable recurrent and input weights, 200 Poisson
                inputs, batch 64, 1000 steps, checkpoint 100): gradient steps/s
-               of the device adjoint, and the CPU reference (torch float64
-               autograd) on a bounded sample
+               of the device adjoint; the same episode differentiated by torch
+               autograd in float32 on the same GPU (``autodiff_same_gpu``, the
+               framework reverse mode jax.grad stands for in the reference);
+               and the CPU reference (torch float64 autograd) on a bounded
+               sample
 
 The run exits with status 1 when the timed windows deliver no events.
 """
@@ -70,6 +73,9 @@ def main():
                     help='run the settle, warm-up and timed windows only (for rocprofv3)')
     ap.add_argument('--bptt-steps', type=int, default=5, help='timed gradient steps (0: skip)')
     ap.add_argument('--bptt-warmup', type=int, default=1)
+    ap.add_argument('--bptt-autodiff-steps', type=int, default=None,
+                    help='timed gradient steps of the torch.autograd restatement on the same GPU '
+                         '(0: skip; default 3, and 0 with --bptt-only)')
     ap.add_argument('--no-pmc', action='store_true',
                     help='skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes behind roofline.traffic')
     ap.add_argument('--ensemble', type=int, default=0, metavar='M',
@@ -121,6 +127,8 @@ def main():
     if args.fp32_report:
         print(json.dumps(run_fp32_report(args)), flush=True)
         return
+    if args.bptt_autodiff_steps is None:
+        args.bptt_autodiff_steps = 0 if args.bptt_only else 3
     if args.bptt_only:
         args.no_cpu = True
         print(json.dumps({'bptt': run_bptt(args)}), flush=True)
@@ -586,6 +594,14 @@ def run_bptt(args):
     _, counts = bp.spikes()
     log(f'bptt: {wall * 1e3:.2f} ms per gradient step ({1 / wall:.2f}/s), loss {loss:.3f}, '
         f'mean rate {counts.mean() / (T * net.dt * 1e-3):.1f} Hz')
+    autodiff = None
+    if args.bptt_autodiff_steps > 0:
+        autodiff = run_autodiff(net, dev, args.bptt_autodiff_steps, loss, g_in, g_rec)
+        autodiff['speedup_of_fused_adjoint'] = round(autodiff['ms_per_grad_step'] / (wall * 1e3), 2)
+        log(f'bptt torch.autograd on the same GPU: {autodiff["ms_per_grad_step"]:.1f} ms per gradient step '
+            f'(fused adjoint {autodiff["speedup_of_fused_adjoint"]:.1f}x faster), loss {autodiff["loss"]:.3f}, '
+            f'gradient difference {autodiff["grad_rel_diff_rec"]:.2e} (recurrent), '
+            f'{autodiff["grad_rel_diff_in"]:.2e} (input)')
     res = {'metric': 'BPTT gradient steps/s', 'value': round(1.0 / wall, 3), 'unit': 'grad-steps/s',
            'ms_per_grad_step': round(wall * 1e3, 3), 'steps': args.bptt_steps,
            'warmup': args.bptt_warmup, 'dtype': 'f32 (transposed delivery bf16x3 split, f32 accumulate)',
@@ -593,7 +609,7 @@ def run_bptt(args):
                                   '200x4096 10% input weights, Poisson 5-50 Hz inputs, rate_mse to '
                                   'U[2,20] Hz, fast-sigmoid surrogate 5/mV, delay 1 ms',
                       'batch': B, 'n_steps': T, 'checkpoint': 100, 'dt_ms': net.dt},
-           'vs_baseline': None, 'cpu_baseline': None}
+           'vs_baseline': None, 'autodiff_same_gpu': autodiff, 'cpu_baseline': None}
     if not args.no_cpu:
         from oracle import bptt_ref as R
         Bs, Ts = 2, 60
@@ -610,6 +626,86 @@ def run_bptt(args):
                       f'float64 autograd), {cw:.1f}s wall, scaled linearly to {B} x {T}'}
         log(f'bptt cpu reference: {cw:.1f}s for {Bs}x{Ts}, ~{per_step:.0f}s per full gradient step')
     return res
+
+
+def run_autodiff(net, dev, n_timed, loss_fused, g_in_fused, g_rec_fused):
+    """The differentiation workload's episode differentiated by a framework
+    instead of the fused adjoint: plain torch autograd in float32 on the same
+    GPU (JAX, whose jax.grad the reference uses, is not installed here). The
+    same network, inputs, update (_kernel/models/iaf_psc_exp.py:136-151), spike
+    rule with the fast-sigmoid straight-through surrogate (spiking.py:65-79),
+    detached reset and rate_mse loss (bptt.py:170-191); the forward computes
+    its own spikes; every step's activations are kept (no checkpointing: the
+    fastest form of framework reverse mode). Dense matrix products stand for
+    the event matmuls. Returns ms per gradient step, its loss and the relative
+    difference of its gradients from the fused adjoint's (its float32 sums run
+    in another order, so near-threshold spikes may differ)."""
+    import torch
+    w_in, w_rec, cnt, tgt = dev
+    k = net.constants()
+    T, B, _ = cnt.shape
+    N, nE = net.n, net.n_exc
+    d, d_in = net.delay_steps, net.delay_in_steps
+    f32 = torch.float32
+    alpha = k['surrogate_alpha']
+    span = k['V_th'] - k['V_reset']
+    reset_to = k['ref_steps'] + 1 if k['ref_steps'] > 0 else 0
+    T_s = T * net.dt * 1e-3
+    cntf = cnt.to(f32)
+
+    def episode(Win, Wrec):
+        V = torch.full((B, N), k['E_L'], dtype=f32, device=Win.device)
+        ie = torch.zeros_like(V)
+        ii = torch.zeros_like(V)
+        r = torch.zeros((B, N), dtype=torch.int32, device=Win.device)
+        spikes = []
+        for n in range(T):
+            xe = cntf[n - d_in] @ Win if n >= d_in else torch.zeros_like(V)
+            xi = torch.zeros_like(V)
+            if n >= d:
+                s = spikes[n - d]
+                xe = xe + s[:, :nE] @ Wrec[:nE]
+                xi = xi + s[:, nE:] @ Wrec[nE:]
+            ref = r > 0
+            y = V - k['E_L']
+            y_new = y * k['P22'] + ie * k['P21ex'] + ii * k['P21in'] + k['I_e'] * k['P20']
+            y = torch.where(ref, y, y_new)
+            ie = ie * k['P11ex'] + xe
+            ii = ii * k['P11in'] + xi
+            V = y + k['E_L']
+            m = (V - k['V_th']) / span
+            hard = (~ref) & (m.detach() >= 0)
+            g = m / (1.0 + alpha * m.abs())
+            spikes.append(hard.to(f32) + (g - g.detach()) * (~ref).to(f32))
+            V = torch.where(hard, torch.full_like(V, k['V_reset']), V)
+            r = torch.clamp(torch.where(hard, torch.full_like(r, reset_to), r) - 1, min=0)
+        rate = torch.stack(spikes).sum(0) / T_s
+        return ((rate - tgt[None, :]) ** 2).mean()
+
+    def grad_step():
+        Win = w_in.detach().clone().requires_grad_(True)
+        Wrec = w_rec.detach().clone().requires_grad_(True)
+        L = episode(Win, Wrec)
+        gi, gr = torch.autograd.grad(L, [Win, Wrec])
+        return float(L.detach()), gi, gr
+
+    grad_step()  # warm-up (library initialisation, allocator)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n_timed):
+        L, gi, gr = grad_step()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / n_timed
+
+    def rel(a, b):
+        b = torch.as_tensor(b, device=a.device, dtype=f32)
+        return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b).clamp_min(1e-30))
+
+    return {'metric': 'torch.autograd gradient steps/s, same GPU', 'value': round(1.0 / wall, 3),
+            'unit': 'grad-steps/s', 'ms_per_grad_step': round(wall * 1e3, 2), 'steps': n_timed,
+            'dtype': 'f32', 'loss': L, 'loss_fused_adjoint': float(loss_fused),
+            'grad_rel_diff_rec': rel(gr, g_rec_fused), 'grad_rel_diff_in': rel(gi, g_in_fused),
+            'form': 'plain reverse mode over all 1000 steps, dense matrix products, no checkpointing'}
 
 
 if __name__ == '__main__':
