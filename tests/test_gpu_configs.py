@@ -13,7 +13,9 @@
   the oracle bit for bit over their first 1000 steps.
 * BPTT 4096 neurons x batch 64 x 1000 steps: forward spikes of all 64 trials
   bit-exact against the float32 restatement; the gradient of a two-trial
-  episode within 1e-4 (relative, L2) of torch float64 autograd.
+  episode within 1e-4 (relative, L2) of torch float64 autograd, and the
+  gradient of all 64 trials within 1e-4 of torch float32 autograd of the same
+  episode on the GPU.
 """
 import numpy as np
 import pytest
@@ -132,3 +134,24 @@ def test_bptt_4096_batch64_t1000_forward_and_gradient():
     e_in, e_rec = rel(gi.cpu().numpy(), gi_ref), rel(gr.cpu().numpy(), gr_ref)
     print(f'gradient relative error: input {e_in:.2e}, recurrent {e_rec:.2e}', flush=True)
     assert e_in < 1e-4 and e_rec < 1e-4
+
+
+def test_bptt_4096_batch64_gradient_vs_autograd_on_the_gpu():
+    """All 64 trials: the fused adjoint's gradients against torch autograd of
+    the same episode in float32 on the same GPU (the benchmark's framework
+    reverse-mode leg, which computes its own forward spikes)."""
+    torch = pytest.importorskip('torch')
+    import bench
+    from brainpy.state_amd.bptt import BPTT, bptt_problem
+    net, w_in, w_rec, cnt, target = bptt_problem()
+    T, B = cnt.shape[:2]
+    print(f'bptt {net.n} neurons, batch {B}, {T} steps: fused adjoint vs torch autograd', flush=True)
+    dev = [torch.tensor(a).cuda() for a in (w_in, w_rec, cnt, target)]
+    bp = BPTT(net, batch=B, n_steps=T, checkpoint=100)
+    loss, g_in, g_rec = bp.loss_and_grad(*dev)
+    bp.close()
+    ad = bench.run_autodiff(net, dev, 1, loss, g_in, g_rec)
+    print(f"gradient relative difference: input {ad['grad_rel_diff_in']:.2e}, "
+          f"recurrent {ad['grad_rel_diff_rec']:.2e}", flush=True)
+    assert ad['loss'] == pytest.approx(loss, rel=1e-5)
+    assert ad['grad_rel_diff_in'] < 1e-4 and ad['grad_rel_diff_rec'] < 1e-4
