@@ -637,9 +637,13 @@ def run_autodiff(net, dev, n_timed, loss_fused, g_in_fused, g_rec_fused):
     detached reset and rate_mse loss (bptt.py:170-191); the forward computes
     its own spikes; every step's activations are kept (no checkpointing: the
     fastest form of framework reverse mode). Dense matrix products stand for
-    the event matmuls. Returns ms per gradient step, its loss and the relative
-    difference of its gradients from the fused adjoint's (its float32 sums run
-    in another order, so near-threshold spikes may differ)."""
+    the event matmuls. The network is padded with 64 disconnected neurons
+    (zero weights in and out, outside the per-class products and the loss),
+    so that no operand slice of those products ends at the end of its
+    allocation. Returns ms per gradient step,
+    its loss and the relative difference of its gradients from the fused
+    adjoint's (its float32 sums run in another order, so near-threshold spikes
+    may differ)."""
     import torch
     w_in, w_rec, cnt, tgt = dev
     k = net.constants()
@@ -652,12 +656,13 @@ def run_autodiff(net, dev, n_timed, loss_fused, g_in_fused, g_rec_fused):
     reset_to = k['ref_steps'] + 1 if k['ref_steps'] > 0 else 0
     T_s = T * net.dt * 1e-3
     cntf = cnt.to(f32)
+    NP = N + 64  # padded neuron count
 
     def episode(Win, Wrec):
-        V = torch.full((B, N), k['E_L'], dtype=f32, device=Win.device)
+        V = torch.full((B, NP), k['E_L'], dtype=f32, device=Win.device)
         ie = torch.zeros_like(V)
         ii = torch.zeros_like(V)
-        r = torch.zeros((B, N), dtype=torch.int32, device=Win.device)
+        r = torch.zeros((B, NP), dtype=torch.int32, device=Win.device)
         spikes = []
         for n in range(T):
             xe = cntf[n - d_in] @ Win if n >= d_in else torch.zeros_like(V)
@@ -665,7 +670,7 @@ def run_autodiff(net, dev, n_timed, loss_fused, g_in_fused, g_rec_fused):
             if n >= d:
                 s = spikes[n - d]
                 xe = xe + s[:, :nE] @ Wrec[:nE]
-                xi = xi + s[:, nE:] @ Wrec[nE:]
+                xi = xi + s[:, nE:N] @ Wrec[nE:N]
             ref = r > 0
             y = V - k['E_L']
             y_new = y * k['P22'] + ie * k['P21ex'] + ii * k['P21in'] + k['I_e'] * k['P20']
@@ -679,15 +684,19 @@ def run_autodiff(net, dev, n_timed, loss_fused, g_in_fused, g_rec_fused):
             spikes.append(hard.to(f32) + (g - g.detach()) * (~ref).to(f32))
             V = torch.where(hard, torch.full_like(V, k['V_reset']), V)
             r = torch.clamp(torch.where(hard, torch.full_like(r, reset_to), r) - 1, min=0)
-        rate = torch.stack(spikes).sum(0) / T_s
+        rate = torch.stack(spikes).sum(0)[:, :N] / T_s
         return ((rate - tgt[None, :]) ** 2).mean()
 
     def grad_step():
-        Win = w_in.detach().clone().requires_grad_(True)
-        Wrec = w_rec.detach().clone().requires_grad_(True)
+        Win = torch.zeros((w_in.shape[0], NP), dtype=f32, device=w_in.device)
+        Win[:, :N] = w_in
+        Wrec = torch.zeros((NP, NP), dtype=f32, device=w_rec.device)
+        Wrec[:N, :N] = w_rec
+        Win.requires_grad_(True)
+        Wrec.requires_grad_(True)
         L = episode(Win, Wrec)
         gi, gr = torch.autograd.grad(L, [Win, Wrec])
-        return float(L.detach()), gi, gr
+        return float(L.detach()), gi[:, :N], gr[:N, :N]
 
     grad_step()  # warm-up (library initialisation, allocator)
     torch.cuda.synchronize()
